@@ -1,0 +1,267 @@
+"""calcir_amd -- MI355X (gfx950) spectroscopy hot path of calcIR.
+
+Python mirror of calcIR's driver (calcIR.cu:9-919) over the C ABI of
+``libcalcir.so`` (``include/calcir.h``): the input-file keys and defaults,
+model/species strings, the sample x frame pipeline and the 13 output files.
+Device arrays are torch tensors on the HIP device; per-stage calls run on
+torch's current stream.
+
+The HIP library is required: importing works without it, but every call
+that needs it raises ``CalcirError`` naming the missing file (there is no CPU
+fallback in this package).
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "CalcirError", "Params", "Results", "lib", "default_params", "read_params", "write_outputs",
+    "synth_box_nm", "synth_trajectory", "run", "nomega", "resolve_model", "resolve_species",
+    "frame_index", "PREC_FP32", "PREC_FP64", "PREC_BF16X3", "MODE_PLAIN", "MODE_BATCHED", "NSITE",
+    "SITE_E", "SITE_W", "SITE_X", "SITE_P", "SITE_MUP", "SITE_U", "SITE_R", "SITE_MU", "SITE_A",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcalcir.so")
+CLI_PATH = os.path.join(HERE, "bin", "calcir")
+
+PREC_FP32, PREC_FP64, PREC_BF16X3 = 0, 1, 2
+MODE_PLAIN, MODE_BATCHED = 0, 1
+NSITE = 20
+SITE_E, SITE_W, SITE_X, SITE_P, SITE_MUP, SITE_U, SITE_R, SITE_MU, SITE_A = 0, 1, 2, 3, 4, 5, 8, 11, 14
+STRLEN = 256
+
+
+class CalcirError(RuntimeError):
+    """A libcalcir call returned a non-zero status (or the library is missing)."""
+
+    def __init__(self, status, msg):
+        super().__init__("%s (status %d)" % (msg, status))
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    """cir_params: the input keys of calcIR.cu:1556-1638 plus extensions."""
+
+    _fields_ = [
+        ("xtcf", ctypes.c_char * STRLEN), ("outf", ctypes.c_char * STRLEN),
+        ("cptf", ctypes.c_char * STRLEN), ("model", ctypes.c_char * STRLEN),
+        ("species", ctypes.c_char * STRLEN),
+        ("ntcfpoints", ctypes.c_int), ("nsamples", ctypes.c_int), ("sampleEvery", ctypes.c_float),
+        ("omegaStep", ctypes.c_int), ("natom_mol", ctypes.c_int), ("nchrom_mol", ctypes.c_int),
+        ("nzeros", ctypes.c_int), ("imap", ctypes.c_int),
+        ("dt", ctypes.c_float), ("t1", ctypes.c_float), ("avef", ctypes.c_float),
+        ("beginTime", ctypes.c_float), ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float),
+        ("cutoff", ctypes.c_float), ("precision", ctypes.c_int), ("mode", ctypes.c_int),
+        ("chunk", ctypes.c_int),
+    ]
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+class Results(ctypes.Structure):
+    """cir_results."""
+
+    _fields_ = [
+        ("tcf_ir", _D), ("tcf_vv", _D), ("tcf_vh", _D),
+        ("ftcf_ir", _D), ("ftcf_vv", _D), ("ftcf_vh", _D),
+        ("sw", _D), ("pw", _D), ("rw", _D), ("rmw", _D),
+        ("nsteps", ctypes.c_int64), ("nframes_used", ctypes.c_int64),
+        ("hist_skipped", ctypes.c_int64), ("seconds", ctypes.c_double),
+    ]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_ll = ctypes.c_longlong
+
+# name -> (restype, argtypes) for every symbol of include/calcir.h
+SIGNATURES = {
+    "cir_strerror": (ctypes.c_char_p, [_i]),
+    "cir_last_error": (ctypes.c_char_p, []),
+    "cir_params_default": (None, [ctypes.POINTER(Params)]),
+    "cir_params_read": (_i, [ctypes.c_char_p, ctypes.POINTER(Params), _i]),
+    "cir_params_echo": (None, [ctypes.POINTER(Params)]),
+    "cir_resolve_model": (_i, [ctypes.c_char_p, ctypes.POINTER(_i)]),
+    "cir_resolve_species": (_i, [ctypes.c_char_p, ctypes.POINTER(_i)]),
+    "cir_nomega": (_i, [ctypes.POINTER(Params)]),
+    "cir_minimage": (_f, [_f, _f]),
+    "cir_frame_index": (_i, [ctypes.POINTER(Params), _i, _i, _f, ctypes.POINTER(_f)]),
+    "cir_write_outputs": (_i, [ctypes.POINTER(Params), ctypes.POINTER(Results)]),
+    "cir_synth_box_nm": (_f, [_i]),
+    "cir_synth_trajectory": (_i, [_i, _i, ctypes.c_uint64, _f, _vp, _vp]),
+    "cir_main": (_i, [_i, ctypes.POINTER(ctypes.c_char_p)]),
+    "cir_device_count": (_i, [ctypes.POINTER(_i)]),
+    "cir_create": (_i, [ctypes.POINTER(Params), _i, _vp, ctypes.POINTER(_vp)]),
+    "cir_run": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "cir_run_device": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "cir_get_results": (_i, [_vp, ctypes.POINTER(Results)]),
+    "cir_destroy": (None, [_vp]),
+    "cir_eproj_sites": (_i, [_i, _i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
+    "cir_hamiltonian": (_i, [_i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
+    "cir_eigh_workspace": (ctypes.c_size_t, [_i, _i, _i, _i]),
+    "cir_eigh_batched": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "cir_ipr_hist": (_i, [_i, _i, _i, _i, _i, _f, _f, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cir_spectral_density": (_i, [_i, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cir_propagate": (_i, [_i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cir_correlate": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cir_spectrum_c2r": (_i, [_i, _i, _i, _vp, _vp, _vp]),
+    "cir_gemm": (_i, [_i, _i, _i, _i, _i, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll,
+                      ctypes.c_double, ctypes.c_double, _vp]),
+}
+
+
+def lib():
+    """The loaded libcalcir.so; raises CalcirError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CalcirError(-1, "libcalcir.so not found at %s: build it with "
+                                  "`python -m calcir_amd.build` (no CPU fallback exists)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        L = lib()
+        raise CalcirError(status, "%s: %s" % (L.cir_strerror(status).decode(), L.cir_last_error().decode()))
+
+
+def _setstr(p, field, value):
+    b = value.encode() if isinstance(value, str) else value
+    if len(b) >= STRLEN:
+        raise ValueError("%s longer than %d bytes" % (field, STRLEN - 1))
+    setattr(p, field, b)
+
+
+def default_params(**kw):
+    """Defaults of calcIR.cu:36-59, then keyword overrides (input-file key names;
+    ``map`` may be given for ``imap``; precision/mode accept their strings)."""
+    p = Params()
+    lib().cir_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "map":
+            k = "imap"
+        if k == "precision" and isinstance(v, str):
+            v = {"fp32": PREC_FP32, "fp64": PREC_FP64, "bf16x3": PREC_BF16X3}[v]
+        if k == "mode" and isinstance(v, str):
+            v = {"plain": MODE_PLAIN, "batched": MODE_BATCHED}[v]
+        if k in ("xtcf", "outf", "cptf", "model", "species"):
+            _setstr(p, k, v)
+        elif any(k == f[0] for f in Params._fields_):
+            setattr(p, k, v)
+        else:
+            raise KeyError("unknown parameter %r" % k)
+    return p
+
+
+def read_params(path, params=None, verbose=False):
+    """ir_init (calcIR.cu:1539-1643) on an input file, starting from params
+    (defaults if None)."""
+    p = params if params is not None else default_params()
+    check(lib().cir_params_read(path.encode(), ctypes.byref(p), 1 if verbose else 0))
+    return p
+
+
+def nomega(p):
+    return lib().cir_nomega(ctypes.byref(p))
+
+
+def resolve_model(model):
+    v = ctypes.c_int()
+    check(lib().cir_resolve_model(model.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def resolve_species(species):
+    v = ctypes.c_int()
+    check(lib().cir_resolve_species(species.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def frame_index(p, sample, point, frame_dt):
+    t = ctypes.c_float()
+    f = lib().cir_frame_index(ctypes.byref(p), sample, point, frame_dt, ctypes.byref(t))
+    return f, t.value
+
+
+def synth_box_nm(nwater):
+    return lib().cir_synth_box_nm(nwater)
+
+
+def synth_trajectory(nwater, nframes, seed, box_nm=None):
+    """Seeded synthetic rigid-water trajectory: (xyz [nframes, 4*nwater, 3]
+    float32 nm, box [nframes, 3] float32, times [nframes] float32 ps)."""
+    L = synth_box_nm(nwater) if box_nm is None else float(box_nm)
+    xyz = np.zeros((nframes, 4 * nwater, 3), np.float32)
+    times = np.zeros(nframes, np.float32)
+    check(lib().cir_synth_trajectory(nwater, nframes, seed, L, xyz.ctypes.data, times.ctypes.data))
+    box = np.full((nframes, 3), L, np.float32)
+    return xyz, box, times
+
+
+def _results_buffers(p):
+    nt, nom, nfft = p.ntcfpoints, nomega(p), p.ntcfpoints + p.nzeros
+    bufs = {k: np.zeros(2 * nt) for k in ("tcf_ir", "tcf_vv", "tcf_vh")}
+    bufs.update({k: np.zeros(nfft) for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh")})
+    bufs.update({k: np.zeros(nom) for k in ("sw", "pw", "rw", "rmw")})
+    r = Results()
+    for k, v in bufs.items():
+        setattr(r, k, v.ctypes.data_as(_D))
+    return r, bufs
+
+
+def run(p, xyz, box, times, stream=None):
+    """The whole calcIR job (every sample, calcIR.cu:360-919) on one GPU.
+
+    xyz: [nframes, natoms, 3] float32 nm (host numpy); box [nframes, 3];
+    times [nframes] ps.  Returns a dict with complex TCFs (decayed and
+    1/nsamples scaled), the C2R outputs, Sw, Pw, Rw, Rmw and counters."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    box = np.ascontiguousarray(box, np.float32)
+    times = np.ascontiguousarray(times, np.float32)
+    nframes, natoms = xyz.shape[0], xyz.shape[1]
+    L = lib()
+    ctx = ctypes.c_void_p()
+    check(L.cir_create(ctypes.byref(p), natoms, stream, ctypes.byref(ctx)))
+    try:
+        check(L.cir_run(ctx, nframes, xyz.ctypes.data, box.ctypes.data, times.ctypes.data))
+        r, bufs = _results_buffers(p)
+        check(L.cir_get_results(ctx, ctypes.byref(r)))
+    finally:
+        L.cir_destroy(ctx)
+    out = {}
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        out[k] = bufs[k][0::2] + 1j * bufs[k][1::2]
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw", "rw", "rmw"):
+        out[k] = bufs[k]
+    out.update(nsteps=r.nsteps, nframes_used=r.nframes_used, hist_skipped=r.hist_skipped,
+               seconds=r.seconds)
+    return out
+
+
+def write_outputs(p, res):
+    """The 13 output files of calcIR.cu:843-919 for a result dict of run()."""
+    bufs = {}
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        z = np.asarray(res[k], np.complex128)
+        a = np.empty(2 * z.size)
+        a[0::2], a[1::2] = z.real, z.imag
+        bufs[k] = a
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw", "rw", "rmw"):
+        bufs[k] = np.ascontiguousarray(res[k], np.float64)
+    r = Results()
+    for k, v in bufs.items():
+        setattr(r, k, v.ctypes.data_as(_D))
+    check(lib().cir_write_outputs(ctypes.byref(p), ctypes.byref(r)))

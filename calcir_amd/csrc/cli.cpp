@@ -1,0 +1,126 @@
+// cli.cpp -- the calcIR.exe command line (calcIR.cu:9-87, 173-227, 323-352,
+// 765-1011) over the libcalcir pipeline.
+//
+// Usage: calcir <input-file>.  The trajectory key xtcf names the frame
+// source; "synth:<nwater>:<nframes>:<seed>[:<box_nm>]" generates a seeded
+// synthetic rigid-water trajectory (frames 10 fs apart, from t = 0).
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "calcir.h"
+
+namespace {
+
+bool parse_synth(const char* spec, int* nwater, int* nframes, unsigned long long* seed, float* box) {
+    if (std::strncmp(spec, "synth:", 6) != 0) return false;
+    *box = 0.0f;
+    const int got = std::sscanf(spec + 6, "%d:%d:%llu:%f", nwater, nframes, seed, box);
+    if (got < 3 || *nwater <= 0 || *nframes < 2) return false;
+    if (got < 4 || !(*box > 0.0f)) *box = cir_synth_box_nm(*nwater);
+    return true;
+}
+
+int fail_msg(int st) {
+    std::printf("ERROR: %s: %s\n", cir_strerror(st), cir_last_error());
+    return EXIT_FAILURE;
+}
+
+}  // namespace
+
+extern "C" int cir_main(int argc, char** argv) {
+    if (argc != 2) {
+        std::printf("Usage:\n"
+                    "\tInclude as the first argument the name of an input file. No other arguments are allowed.\n");
+        return EXIT_FAILURE;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
+        std::printf("\nGPU INFO:\n\tDevice name: %s\n\tMemory: %g gb\n", prop.name, prop.totalGlobalMem / (1.E9));
+
+    std::printf("\n>>> Setting default parameters\n");
+    cir_params p;
+    cir_params_default(&p);
+    int st = cir_params_read(argv[1], &p, 1);
+    if (st != CIR_OK) return EXIT_FAILURE;
+    cir_params_echo(&p);
+
+    int imodel = 0, ispecies = 0;
+    if (cir_resolve_model(p.model, &imodel) != CIR_OK) {
+        std::printf("WARNING: model: %s is not recognized. Check input file. Aborting...\n", p.model);
+        return EXIT_FAILURE;
+    }
+    if (cir_resolve_species(p.species, &ispecies) != CIR_OK) {
+        std::printf("WARNING: species: %s is not recognized. Check input file. Aborting...\n", p.species);
+        return EXIT_FAILURE;
+    }
+
+    int nwater = 0, nframes = 0;
+    unsigned long long seed = 0;
+    float boxl = 0.0f;
+    if (!parse_synth(p.xtcf, &nwater, &nframes, &seed, &boxl)) {
+        std::printf("WARNING: The file %s could not be opened. Is the name correct?\n", p.xtcf);
+        std::printf("(this build reads the synthetic source \"synth:<nwater>:<nframes>:<seed>[:<box_nm>]\")\n");
+        return EXIT_FAILURE;
+    }
+    const int natoms = 4 * nwater;
+    if (p.natom_mol != 4) {
+        std::printf("WARNING: the synthetic source writes 4-site waters; natom_mol must be 4.\n");
+        return EXIT_FAILURE;
+    }
+    std::vector<float> xyz(size_t(nframes) * natoms * 3), times(nframes), box(size_t(nframes) * 3, boxl);
+    st = cir_synth_trajectory(nwater, nframes, seed, boxl, xyz.data(), times.data());
+    if (st != CIR_OK) return fail_msg(st);
+    const int nmol = natoms / p.natom_mol, nchrom = nmol * p.nchrom_mol;
+    std::printf(">>> Will read the trajectory from: %s.\n", p.xtcf);
+    std::printf(">>> Found %d atoms and %d molecules.\n", natoms, nmol);
+    std::printf(">>> Found %d chromophores.\n", nchrom);
+    std::printf(">>> Frame time offset is: %f (ps)\n", times[1] - times[0]);
+    std::printf("\n>>> Now calculating the absorption spectrum\n");
+    std::printf("----------------------------------------------------------\n");
+    std::fflush(stdout);
+
+    cir_ctx* ctx = nullptr;
+    st = cir_create(&p, natoms, nullptr, &ctx);
+    if (st != CIR_OK) return fail_msg(st);
+    st = cir_run(ctx, nframes, xyz.data(), box.data(), times.data());
+    if (st != CIR_OK) {
+        cir_destroy(ctx);
+        return fail_msg(st);
+    }
+    const int nt = p.ntcfpoints, nom = cir_nomega(&p), nfft = nt + p.nzeros;
+    std::vector<double> tc(3 * 2 * size_t(nt)), ft(3 * size_t(nfft)), dist(4 * size_t(nom));
+    cir_results r{};
+    r.tcf_ir = tc.data();
+    r.tcf_vv = tc.data() + 2 * nt;
+    r.tcf_vh = tc.data() + 4 * nt;
+    r.ftcf_ir = ft.data();
+    r.ftcf_vv = ft.data() + nfft;
+    r.ftcf_vh = ft.data() + 2 * size_t(nfft);
+    r.sw = dist.data();
+    r.pw = dist.data() + nom;
+    r.rw = dist.data() + 2 * nom;
+    r.rmw = dist.data() + 3 * nom;
+    st = cir_get_results(ctx, &r);
+    cir_destroy(ctx);
+    if (st != CIR_OK) return fail_msg(st);
+    if (r.hist_skipped > 0)
+        std::printf("WARNING: %lld mode-steps fell outside omegaStart..omegaStop and were not binned.\n",
+                    (long long)r.hist_skipped);
+    std::printf("\n\n----------------------------------------------------------\n");
+    std::printf("Finishing up...\n");
+    st = cir_write_outputs(&p, &r);
+    if (st != CIR_OK) return fail_msg(st);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("\n>>> Done with the calculation in %f seconds.\n", secs);
+    std::printf(">>> %lld (sample, tcf point) steps over %lld distinct frames, %.3f s on the device.\n",
+                (long long)r.nsteps, (long long)r.nframes_used, r.seconds);
+    return 0;
+}

@@ -1,0 +1,94 @@
+// common.hpp -- shared definitions of libcalcir's HIP sources (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "calcir.h"
+
+namespace cir {
+
+// calcIR.h:36-37
+constexpr double kHbar = 5.308837367;  // cm^-1 ps
+constexpr double kPi = 3.14159265359;
+constexpr double kTwoPiExact = 6.283185307179586476925286766559;
+// calcIR.cu:1036, 1197, 1198
+constexpr double kBohrPerNm = 18.8973;
+constexpr double kCmPerHartree = 2.1947463e5;
+// calcIR.cu:1136 (TIP4P O-M), 1289 (TDM offset), 1299 (Raman anisotropy),
+// 1152 and 1157 (site charges)
+constexpr double kOMTip4p = 0.0150;
+constexpr double kTdmOffset = 0.067;
+constexpr double kRamanAniso = 4.6;
+constexpr double kQH = 0.52;
+constexpr double kQM = 1.04;
+
+// Map coefficients (calcIR.cu:1203-1242), resolved once on the host.
+struct MapCoef {
+    double w[3], x[2], p[2], mup[3], wi[3];
+};
+bool map_coef(int imap, int ispecies, MapCoef* c);
+
+// Error plumbing: the detail text of the last failure (thread-local).
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define CIR_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t cir_e_ = (call);                                          \
+        if (cir_e_ != hipSuccess) return ::cir::hip_fail(cir_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define CIR_CHECK_LAUNCH() CIR_HIP(hipGetLastError())
+
+#define CIR_TRY(call)                       \
+    do {                                    \
+        int cir_s_ = (call);                \
+        if (cir_s_ != CIR_OK) return cir_s_; \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace cir
+
+// Device-side helpers (HIP translation units only) ----------------------------
+#if defined(__HIPCC__)
+namespace cir {
+
+// calcIR.cu:1457-1460.  Contraction is disabled so that dx - L*round(dx/L)
+// is rounded exactly as written (the FP32 cutoff decision depends on it).
+template <typename T>
+__device__ __forceinline__ T minimage(T dx, T boxl) {
+#pragma clang fp contract(off)
+    return dx - boxl * round(dx / boxl);
+}
+
+// Neumaier-compensated accumulation.
+template <typename T>
+struct Compensated {
+    T sum = T(0), c = T(0);
+    __device__ __forceinline__ void add(T v) {
+        T t = sum + v;
+        if (fabs(sum) >= fabs(v))
+            c += (sum - t) + v;
+        else
+            c += (v - t) + sum;
+        sum = t;
+    }
+    __device__ __forceinline__ T value() const { return sum + c; }
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace cir
+#endif  // __HIPCC__

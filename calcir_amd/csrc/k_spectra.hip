@@ -1,0 +1,400 @@
+// k_spectra.hip -- frequency distribution / IPR, spectral density,
+// propagation helpers, correlation functions and the C2R lineshape
+// transform (gfx950).
+
+#include <cfloat>
+
+#include "gemm.hpp"
+#include "spectra.hpp"
+
+namespace cir {
+
+constexpr int kSpNT = 256;
+
+// ---------------------------------------------------------------------------
+// IPR and frequency distribution, calcIR.cu:494-538.  One lane per mode; V is
+// row-major (component i, mode k) so lanes read consecutive modes of a row.
+// Per-block LDS histograms, flushed with FP64 atomics.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+ipr_hist_kernel(int n, int ld, int nchrom_mol, float avef, float omegaStart, int omegaStep, int nomega,
+                const T* __restrict__ Vall, const double* __restrict__ wall, const double* __restrict__ weight,
+                double* __restrict__ pw, double* __restrict__ rw, double* __restrict__ rmw,
+                long long* __restrict__ skipped, int use_lds) {
+    extern __shared__ double sh[];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const double wgt = weight ? weight[b] : 1.0;
+    if (wgt == 0.0) return;
+    if (use_lds) {
+        for (int q = tid; q < 3 * nomega; q += kSpNT) sh[q] = 0.0;
+        __syncthreads();
+    }
+    const int k = blockIdx.x * kSpNT + tid;
+    if (k < n) {
+        const T* V = Vall + size_t(b) * n * ld;
+        T s4 = T(0), outer = T(0), inner = T(0);
+        int cnt = 0;
+        for (int i = 0; i < n; ++i) {
+            const T c = V[size_t(i) * ld + k];
+            const T c2 = c * c;
+            s4 += c2 * c2;
+            inner += c2;
+            if (++cnt == nchrom_mol) {
+                outer += inner * inner;
+                inner = T(0);
+                cnt = 0;
+            }
+        }
+        const double ipr = double(T(1) / s4);
+        const double mipr = double(T(1) / outer);
+        const float wi = float(wall[size_t(b) * n + k]) + avef;
+        const int bin = int(roundf((wi - omegaStart) / float(omegaStep)));
+        if (bin < 0 || bin >= nomega) {
+            // calcIR.cu:530-537 warns and then writes out of bounds; the
+            // mode is skipped and counted here.
+            atomicAdd(reinterpret_cast<unsigned long long*>(skipped), (unsigned long long)(llrint(wgt)));
+        } else {
+            const double inv = wgt / double(omegaStep);
+            if (use_lds) {
+                atomicAdd(&sh[bin], inv);
+                atomicAdd(&sh[nomega + bin], inv * ipr);
+                atomicAdd(&sh[2 * nomega + bin], inv * mipr);
+            } else {
+                atomicAdd(&pw[bin], inv);
+                atomicAdd(&rw[bin], inv * ipr);
+                atomicAdd(&rmw[bin], inv * mipr);
+            }
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int q = tid; q < nomega; q += kSpNT) {
+            if (sh[q] != 0.0) {
+                atomicAdd(&pw[q], sh[q]);
+                atomicAdd(&rw[q], sh[nomega + q]);
+                atomicAdd(&rmw[q], sh[2 * nomega + q]);
+            }
+        }
+    }
+}
+
+// |V^T mu|^2 per mode (calcIR.cu:462-467 fused).
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+mu2_kernel(int n, int ld, const T* __restrict__ Vall, const T* __restrict__ sites,
+           const double* __restrict__ weight, double* __restrict__ m2all) {
+    const int b = blockIdx.y;
+    if (weight && weight[b] == 0.0) return;
+    const int k = blockIdx.x * kSpNT + threadIdx.x;
+    if (k >= n) return;
+    const T* V = Vall + size_t(b) * n * ld;
+    const T* mu = sites + size_t(b) * CIR_NSITE * n + size_t(CIR_SITE_MU) * n;
+    T ax = T(0), ay = T(0), az = T(0);
+    for (int i = 0; i < n; ++i) {
+        const T v = V[size_t(i) * ld + k];
+        ax += v * mu[i];
+        ay += v * mu[n + i];
+        az += v * mu[2 * n + i];
+    }
+    m2all[size_t(b) * n + k] = double(ax * ax + ay * ay + az * az);
+}
+
+// Lorentzian sum over modes per omega bin (calcIR.cu:1414-1444).
+__global__ void __launch_bounds__(kSpNT)
+sw_kernel(int n, int nomega, float omegaStart, int omegaStep, double gamma, float avef,
+          const double* __restrict__ m2all, const double* __restrict__ wall,
+          const double* __restrict__ weight, double* __restrict__ sw) {
+    const int b = blockIdx.y;
+    const double wgt = weight ? weight[b] : 1.0;
+    if (wgt == 0.0) return;
+    const int q = blockIdx.x * kSpNT + threadIdx.x;
+    if (q >= nomega) return;
+    const double om = double(float(omegaStart + float(omegaStep * q)));  // calcIR.cu:271
+    const double* m2 = m2all + size_t(b) * n;
+    const double* w = wall + size_t(b) * n;
+    double acc = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double dw = om - (w[k] + double(avef));
+        acc += m2[k] * gamma / (dw * dw + gamma * gamma) / kPi;
+    }
+    atomicAdd(&sw[q], wgt * acc);
+}
+
+// ---------------------------------------------------------------------------
+// Propagation helpers
+// ---------------------------------------------------------------------------
+// Row phase G'_k = exp(+i w_k dt / hbar) G_k (calcIR.cu:1512-1514) on a
+// row-major n x (2 ncols) block [Re | Im].
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+phase_kernel(int n, int ncols, int ldg, double dt, const double* __restrict__ w, T* __restrict__ G) {
+    const int64_t idx = int64_t(blockIdx.x) * kSpNT + threadIdx.x;
+    if (idx >= int64_t(n) * ncols) return;
+    const int k = int(idx / ncols), c = int(idx % ncols);
+    double s, co;
+    sincos(w[k] * dt / kHbar, &s, &co);
+    T* row = G + size_t(k) * ldg;
+    const T gr = row[c], gi = row[ncols + c];
+    row[c] = T(co) * gr - T(s) * gi;
+    row[ncols + c] = T(s) * gr + T(co) * gi;
+}
+
+// F = I as [Re | Im] (calcIR.cu:1521-1535).
+template <typename T>
+__global__ void __launch_bounds__(kSpNT) identity_kernel(int n, int ldf, T* __restrict__ F) {
+    const int64_t idx = int64_t(blockIdx.x) * kSpNT + threadIdx.x;
+    if (idx >= int64_t(n) * 2 * n) return;
+    const int r = int(idx / (2 * n)), c = int(idx % (2 * n));
+    F[size_t(r) * ldf + c] = (c == r) ? T(1) : T(0);
+}
+
+// Per-slot state update after a propagation step.  Actions:
+// 0 idle -> zero, 1 origin -> M(t) (real), 2 propagated -> keep, 3 hold -> old.
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+slot_fixup_kernel(int n, int nslot, int ldy, const int* __restrict__ action, const T* __restrict__ sites,
+                  const T* __restrict__ Yold, T* __restrict__ Ynew) {
+    const int ncols = 9 * nslot;
+    const int64_t idx = int64_t(blockIdx.x) * kSpNT + threadIdx.x;
+    if (idx >= int64_t(n) * ncols) return;
+    const int i = int(idx / ncols), c = int(idx % ncols);
+    const int s = c / 9, a = c % 9;
+    const int act = action[s];
+    T* yr = Ynew + size_t(i) * ldy + c;
+    T* yi = yr + ncols;
+    if (act == 0) {
+        *yr = T(0);
+        *yi = T(0);
+    } else if (act == 1) {
+        *yr = sites[size_t(CIR_SITE_MU + a) * n + i];
+        *yi = T(0);
+    } else if (act == 3) {
+        *yr = Yold[size_t(i) * ldy + c];
+        *yi = Yold[size_t(i) * ldy + ncols + c];
+    }
+}
+
+// (b, a) pairs of calcIR.cu:628-733, value = M_b(t)^T F M_a(0), with the nine
+// vectors ordered mu_x, mu_y, mu_z, a_xx, a_yy, a_zz, a_xy, a_yz, a_zx:
+//   p = 0..2 IR (a,a); 3..5 iiFii (a,a); 6..8 ijFij (a,a);
+//   p = 9..14 iiFjj: (4,3) (5,3) (3,4) (5,4) (3,5) (4,5).
+// One block per slot: the 15 bilinear forms, combined into IR, VV and VH
+// (calcIR.cu:646-648, 736-745) and stored at out[6*target[s] ...].
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+correlate_kernel(int n, int nslot, int ldy, const int* __restrict__ action, const int* __restrict__ target,
+                 const T* __restrict__ sites, const T* __restrict__ Y, double* __restrict__ out) {
+    __shared__ double red[kSpNT / 64][30];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int act = action ? action[s] : 2;
+    if (act != 1 && act != 2) return;
+    const int ncols = 9 * nslot;
+    double acc[30];
+#pragma unroll
+    for (int p = 0; p < 30; ++p) acc[p] = 0.0;
+    for (int i = tid; i < n; i += kSpNT) {
+        const T* yr = Y + size_t(i) * ldy + 9 * s;
+        const T* yi = yr + ncols;
+        T m[9];
+#pragma unroll
+        for (int b = 0; b < 9; ++b) m[b] = sites[size_t(CIR_SITE_MU + b) * n + i];
+#pragma unroll
+        for (int p = 0; p < 15; ++p) {
+            const int pb = (p < 9) ? p : ((p == 9 || p == 14) ? 4 : (p == 10 || p == 12) ? 5 : 3);
+            const int pa = (p < 9) ? p : ((p == 9 || p == 10) ? 3 : (p == 11 || p == 12) ? 4 : 5);
+            acc[2 * p] += double(m[pb] * yr[pa]);
+            acc[2 * p + 1] += double(m[pb] * yi[pa]);
+        }
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int p = 0; p < 30; ++p) {
+        const double v = wave_sum(acc[p]);
+        if (lane == 0) red[wave][p] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double t[30];
+        for (int p = 0; p < 30; ++p) {
+            t[p] = 0.0;
+            for (int w = 0; w < kSpNT / 64; ++w) t[p] += red[w][p];
+        }
+        double ir[2] = {0, 0}, iiii[2] = {0, 0}, ijij[2] = {0, 0}, iijj[2] = {0, 0};
+        for (int c = 0; c < 2; ++c) {
+            for (int p = 0; p < 3; ++p) ir[c] += t[2 * p + c];
+            for (int p = 3; p < 6; ++p) iiii[c] += t[2 * p + c];
+            for (int p = 6; p < 9; ++p) ijij[c] += t[2 * p + c];
+            for (int p = 9; p < 15; ++p) iijj[c] += t[2 * p + c];
+        }
+        double* o = out + 6 * size_t(target ? target[s] : 0);
+        for (int c = 0; c < 2; ++c) {
+            o[c] = ir[c];
+            o[2 + c] = (3.0 * iiii[c] + iijj[c] + 4.0 * ijij[c]) / 15.0;
+            o[4 + c] = (2.0 * iiii[c] - iijj[c] + 6.0 * ijij[c]) / 30.0;
+        }
+    }
+}
+
+// cuFFT C2R contract (calcIR.cu:790-791): y[k] = Re X0 + 2 sum_j Re(X_j
+// e^{+2 pi i jk/n}) over j = 1..min(nx-1, n/2), X[n/2] taken as real.
+__global__ void __launch_bounds__(kSpNT)
+c2r_kernel(int nfft, int nx, const double* __restrict__ X, double* __restrict__ y) {
+    const int sp = blockIdx.y;
+    const int k = blockIdx.x * kSpNT + threadIdx.x;
+    if (k >= nfft) return;
+    const double* x = X + 2 * size_t(sp) * nx;
+    const int half = nfft / 2;
+    const int jmax = min(nx - 1, half);
+    double acc = nx > 0 ? x[0] : 0.0;
+    for (int j = 1; j <= jmax; ++j) {
+        const long long jk = (static_cast<long long>(j) * k) % nfft;
+        double s, c;
+        sincospi(2.0 * double(jk) / double(nfft), &s, &c);
+        if ((nfft % 2 == 0) && j == half)
+            acc += x[2 * j] * c;
+        else
+            acc += 2.0 * (x[2 * j] * c - x[2 * j + 1] * s);
+    }
+    y[size_t(sp) * nfft + k] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+int ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart,
+             int omegaStep, int nomega, const void* V, const double* w, const double* weight, double* pw,
+             double* rw, double* rmw, long long* skipped, hipStream_t s) {
+    if (batch <= 0 || n <= 0 || ld < n || nomega <= 0 || omegaStep == 0 || nchrom_mol <= 0)
+        return fail(CIR_ERR_ARG, "ipr_hist: bad sizes");
+    const size_t sm = 3 * size_t(nomega) * sizeof(double);
+    const int use_lds = sm <= 96 * 1024 ? 1 : 0;
+    dim3 grid(unsigned(ceil_div(n, kSpNT)), unsigned(batch));
+    if (prec == CIR_PREC_FP64) {
+        CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ipr_hist_kernel<double>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(use_lds ? sm : 0)));
+        hipLaunchKernelGGL(ipr_hist_kernel<double>, grid, dim3(kSpNT), use_lds ? sm : 0, s, n, ld, nchrom_mol,
+                           avef, omegaStart, omegaStep, nomega, static_cast<const double*>(V), w, weight, pw,
+                           rw, rmw, skipped, use_lds);
+    } else {
+        CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ipr_hist_kernel<float>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(use_lds ? sm : 0)));
+        hipLaunchKernelGGL(ipr_hist_kernel<float>, grid, dim3(kSpNT), use_lds ? sm : 0, s, n, ld, nchrom_mol,
+                           avef, omegaStart, omegaStep, nomega, static_cast<const float*>(V), w, weight, pw, rw,
+                           rmw, skipped, use_lds);
+    }
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
+int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, float omegaStart,
+                     int omegaStep, int nomega, const void* V, const double* w, const void* sites,
+                     const double* weight, double* sw, double* m2work, hipStream_t s) {
+    if (batch <= 0 || n <= 0 || ld < n || nomega <= 0) return fail(CIR_ERR_ARG, "spectral_density: bad sizes");
+    dim3 g1(unsigned(ceil_div(n, kSpNT)), unsigned(batch));
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(mu2_kernel<double>, g1, dim3(kSpNT), 0, s, n, ld, static_cast<const double*>(V),
+                           static_cast<const double*>(sites), weight, m2work);
+    else
+        hipLaunchKernelGGL(mu2_kernel<float>, g1, dim3(kSpNT), 0, s, n, ld, static_cast<const float*>(V),
+                           static_cast<const float*>(sites), weight, m2work);
+    CIR_CHECK_LAUNCH();
+    const double gamma = kHbar / (double(t1) * 2.0);  // calcIR.cu:1426
+    dim3 g2(unsigned(ceil_div(nomega, kSpNT)), unsigned(batch));
+    hipLaunchKernelGGL(sw_kernel, g2, dim3(kSpNT), 0, s, n, nomega, omegaStart, omegaStep, gamma, avef, m2work,
+                       w, weight, sw);
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
+int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
+              const double* w, const void* Yin, void* Yout, void* G, hipStream_t s) {
+    if (n <= 0 || ncols <= 0 || ldy < 2 * ncols || ld < n) return fail(CIR_ERR_ARG, "propagate: bad sizes");
+    GemmArgs g;
+    // G = V^T Y   (V row-major: V(i,k) at V[i*ld+k]; A(m=k, kk=i))
+    g.M = n;
+    g.N = 2 * ncols;
+    g.K = n;
+    g.A = V;
+    g.a_m = 1;
+    g.a_k = ld;
+    g.B = Yin;
+    g.b_k = ldy;
+    g.b_n = 1;
+    g.C = G;
+    g.c_m = 2 * ncols;
+    g.c_n = 1;
+    CIR_TRY(gemm(gemm_prec, g, s));
+    const int64_t tot = int64_t(n) * ncols;
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(phase_kernel<double>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           ncols, 2 * ncols, dt, w, static_cast<double*>(G));
+    else
+        hipLaunchKernelGGL(phase_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n, ncols,
+                           2 * ncols, dt, w, static_cast<float*>(G));
+    CIR_CHECK_LAUNCH();
+    // Y = V G'
+    g = GemmArgs{};
+    g.M = n;
+    g.N = 2 * ncols;
+    g.K = n;
+    g.A = V;
+    g.a_m = ld;
+    g.a_k = 1;
+    g.B = G;
+    g.b_k = 2 * ncols;
+    g.b_n = 1;
+    g.C = Yout;
+    g.c_m = ldy;
+    g.c_n = 1;
+    return gemm(gemm_prec, g, s);
+}
+
+int set_identity(int prec, int n, int ldf, void* F, hipStream_t s) {
+    const int64_t tot = int64_t(n) * 2 * n;
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(identity_kernel<double>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           ldf, static_cast<double*>(F));
+    else
+        hipLaunchKernelGGL(identity_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n, ldf,
+                           static_cast<float*>(F));
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
+int slot_fixup(int prec, int n, int nslot, int ldy, const int* action, const void* sites, const void* Yold,
+               void* Ynew, hipStream_t s) {
+    const int64_t tot = int64_t(n) * 9 * nslot;
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(slot_fixup_kernel<double>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           nslot, ldy, action, static_cast<const double*>(sites), static_cast<const double*>(Yold),
+                           static_cast<double*>(Ynew));
+    else
+        hipLaunchKernelGGL(slot_fixup_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           nslot, ldy, action, static_cast<const float*>(sites), static_cast<const float*>(Yold),
+                           static_cast<float*>(Ynew));
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
+int correlate(int prec, int n, int nslot, int ldy, const int* action, const int* target, const void* sites,
+              const void* Y, double* out, hipStream_t s) {
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(correlate_kernel<double>, dim3(nslot), dim3(kSpNT), 0, s, n, nslot, ldy, action, target,
+                           static_cast<const double*>(sites), static_cast<const double*>(Y), out);
+    else
+        hipLaunchKernelGGL(correlate_kernel<float>, dim3(nslot), dim3(kSpNT), 0, s, n, nslot, ldy, action, target,
+                           static_cast<const float*>(sites), static_cast<const float*>(Y), out);
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
+int spectrum_c2r(int nspec, int nfft, int nx, const double* X, double* y, hipStream_t s) {
+    if (nspec <= 0 || nfft <= 0 || nx < 0 || nspec > 65535) return fail(CIR_ERR_ARG, "c2r: bad sizes");
+    hipLaunchKernelGGL(c2r_kernel, dim3(unsigned(ceil_div(nfft, kSpNT)), unsigned(nspec)), dim3(kSpNT), 0, s, nfft,
+                       nx, X, y);
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
+}  // namespace cir

@@ -1,0 +1,563 @@
+// pipeline.hip -- the calcIR sample x frame pipeline on one GPU
+// (calcIR.cu:360-919), as a context bound to one HIP stream.
+//
+// Two control flows produce the same outputs:
+//   CIR_MODE_BATCHED  every distinct trajectory frame needed by any
+//     (sample, tcf point) is processed once: field, sites, Hamiltonian and
+//     eigensolve run batched over chunks of frames; the IPR histogram and the
+//     spectral density are weighted by how many (sample, point) pairs /
+//     samples use the frame; all samples active at a frame are propagated in
+//     one product in the algebraically identical vector form Y = F M(0)
+//     (9 complex columns per sample).
+//   CIR_MODE_PLAIN    the reference control flow: sample by sample, frame by
+//     frame, one eigensolve per (sample, point) and the literal F-matrix
+//     propagation F <- V P V^T F (calcIR.cu:597-617) with the 15 bilinear
+//     forms of calcIR.cu:628-745.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "spectra.hpp"
+
+namespace cir {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t n) {
+        release();
+        if (n == 0) n = 256;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+        }
+        bytes = n;
+        return CIR_OK;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+}  // namespace cir
+
+struct cir_ctx {
+    cir_params p;
+    int imodel = 0, ispecies = 0;
+    int natoms = 0, nmol = 0, N = 0, ld = 0, nomega = 0, nfft = 0;
+    int real_prec = CIR_PREC_FP32;  // storage precision of real arrays
+    int prop_prec = CIR_PREC_FP32;  // product precision of the propagator
+    size_t esz = 4;
+    hipStream_t stream = nullptr;
+    std::vector<double> tcf[3], ftcf[3], sw, pw, rw, rmw;
+    int64_t nsteps = 0, nframes_used = 0, hist_skipped = 0;
+    double seconds = 0.0;
+    bool have = false;
+};
+
+using namespace cir;
+
+namespace {
+
+struct Schedule {
+    std::vector<int> fidx;        // ns*nt trajectory frame per (sample, point)
+    std::vector<int> uniq;        // distinct frames, ascending
+    std::vector<double> mult;     // (sample, point) pairs per distinct frame
+    std::vector<double> orig;     // samples starting at the distinct frame
+    int nslot = 0;
+    std::vector<int> action;      // nuniq*nslot
+    std::vector<int> target;      // nuniq*nslot
+};
+
+int build_schedule(const cir_params& p, int nframes, const float* times, Schedule& S) {
+    const int ns = p.nsamples, nt = p.ntcfpoints;
+    if (nframes < 2) return fail(CIR_ERR_FRAME, "at least two frames are needed to find the frame spacing");
+    // calcIR.cu:342-345 (XTC precision 1000)
+    const float frame_dt = float(std::round((times[1] - times[0]) * 1000.0f) / (1. * 1000.0f));
+    if (!(frame_dt > 0.0f)) return fail(CIR_ERR_FRAME, "trajectory times are not increasing");
+    S.fidx.assign(size_t(ns) * nt, 0);
+    for (int s = 0; s < ns; ++s)
+        for (int k = 0; k < nt; ++k) {
+            float t;
+            const int f = cir_frame_index(&p, s, k, frame_dt, &t);
+            if (f < 0 || f >= nframes)
+                return fail(CIR_ERR_FRAME, "could not find the desired frame at time " + std::to_string(t) + " ps");
+            if (std::fabs(t - times[f]) > frame_dt * 1E-1)  // calcIR.cu:391
+                return fail(CIR_ERR_FRAME, "could not find the desired frame at time " + std::to_string(t) +
+                                               " ps (found " + std::to_string(times[f]) + ")");
+            if (k > 0 && f <= S.fidx[size_t(s) * nt + k - 1])
+                return fail(CIR_ERR_FRAME, "dt must select increasing trajectory frames");
+            S.fidx[size_t(s) * nt + k] = f;
+        }
+    std::vector<int> uof(nframes, -1);
+    for (int f : S.fidx) uof[f] = 0;
+    S.uniq.clear();
+    for (int f = 0; f < nframes; ++f)
+        if (uof[f] == 0) {
+            uof[f] = int(S.uniq.size());
+            S.uniq.push_back(f);
+        }
+    const int nu = int(S.uniq.size());
+    S.mult.assign(nu, 0.0);
+    S.orig.assign(nu, 0.0);
+    for (int s = 0; s < ns; ++s) {
+        for (int k = 0; k < nt; ++k) S.mult[uof[S.fidx[size_t(s) * nt + k]]] += 1.0;
+        S.orig[uof[S.fidx[size_t(s) * nt]]] += 1.0;
+    }
+    // slots: samples in order of their first frame; a slot is held from the
+    // first to the last frame of its sample
+    std::vector<int> order(ns);
+    for (int s = 0; s < ns; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return S.fidx[size_t(a) * nt] < S.fidx[size_t(b) * nt];
+    });
+    std::vector<int> slot_of(ns, -1), slot_end;  // slot_end: last distinct index in use
+    for (int s : order) {
+        const int u0 = uof[S.fidx[size_t(s) * nt]];
+        const int u1 = uof[S.fidx[size_t(s) * nt + nt - 1]];
+        int q = -1;
+        for (int c = 0; c < int(slot_end.size()); ++c)
+            if (slot_end[c] < u0) {
+                q = c;
+                break;
+            }
+        if (q < 0) {
+            q = int(slot_end.size());
+            slot_end.push_back(-1);
+        }
+        slot_end[q] = u1;
+        slot_of[s] = q;
+    }
+    S.nslot = std::max<int>(1, int(slot_end.size()));
+    S.action.assign(size_t(nu) * S.nslot, 0);
+    S.target.assign(size_t(nu) * S.nslot, 0);
+    for (int s = 0; s < ns; ++s) {
+        const int q = slot_of[s];
+        const int u0 = uof[S.fidx[size_t(s) * nt]];
+        const int u1 = uof[S.fidx[size_t(s) * nt + nt - 1]];
+        for (int u = u0; u <= u1; ++u) S.action[size_t(u) * S.nslot + q] = 3;
+        for (int k = 0; k < nt; ++k) {
+            const int u = uof[S.fidx[size_t(s) * nt + k]];
+            S.action[size_t(u) * S.nslot + q] = (k == 0) ? 1 : 2;
+            S.target[size_t(u) * S.nslot + q] = s * nt + k;
+        }
+    }
+    return CIR_OK;
+}
+
+int finalize(cir_ctx* c, const std::vector<double>& out /* ns*nt*6 */, const std::vector<double>& acc /* 4*nomega */,
+             long long skipped) {
+    const cir_params& p = c->p;
+    const int ns = p.nsamples, nt = p.ntcfpoints;
+    // calcIR.cu:646-648, 739-745 (accumulated over samples), 777-781 (decay)
+    std::vector<double> X(3 * 2 * size_t(nt), 0.0);
+    for (int sp = 0; sp < 3; ++sp) {
+        c->tcf[sp].assign(2 * size_t(nt), 0.0);
+        for (int k = 0; k < nt; ++k) {
+            double re = 0.0, im = 0.0;
+            for (int s = 0; s < ns; ++s) {
+                re += out[(size_t(s) * nt + k) * 6 + 2 * sp];
+                im += out[(size_t(s) * nt + k) * 6 + 2 * sp + 1];
+            }
+            const double dcy = std::exp(-1.0 * k * p.dt / (2.0 * p.t1)) / (1. * ns);
+            c->tcf[sp][2 * k] = re * dcy;
+            c->tcf[sp][2 * k + 1] = im * dcy;
+            X[(size_t(sp) * nt + k) * 2] = re * dcy;
+            X[(size_t(sp) * nt + k) * 2 + 1] = im * dcy;
+        }
+    }
+    // calcIR.cu:788-827: C2R of the zero-padded TCFs
+    DevBuf dX, dy;
+    CIR_TRY(dX.alloc(X.size() * sizeof(double)));
+    CIR_TRY(dy.alloc(3 * size_t(c->nfft) * sizeof(double)));
+    CIR_HIP(hipMemcpyAsync(dX.p, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CIR_TRY(spectrum_c2r(3, c->nfft, nt, dX.as<double>(), dy.as<double>(), c->stream));
+    std::vector<double> y(3 * size_t(c->nfft));
+    CIR_HIP(hipMemcpyAsync(y.data(), dy.p, y.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CIR_HIP(hipStreamSynchronize(c->stream));
+    for (int sp = 0; sp < 3; ++sp) c->ftcf[sp].assign(y.begin() + size_t(sp) * c->nfft, y.begin() + size_t(sp + 1) * c->nfft);
+    // calcIR.cu:831-839
+    const int nom = c->nomega;
+    const double norm = double(c->N) * ns * nt;
+    c->sw.assign(nom, 0.0);
+    c->pw.assign(nom, 0.0);
+    c->rw.assign(nom, 0.0);
+    c->rmw.assign(nom, 0.0);
+    for (int i = 0; i < nom; ++i) {
+        c->sw[i] = acc[i] / double(ns);
+        const double pwi = acc[nom + i] / norm;
+        c->pw[i] = pwi;
+        c->rw[i] = (acc[2 * nom + i] / norm) / pwi;
+        c->rmw[i] = (acc[3 * nom + i] / norm) / pwi;
+    }
+    c->hist_skipped = skipped;
+    c->nsteps = int64_t(ns) * nt;
+    c->have = true;
+    return CIR_OK;
+}
+
+// Gathers trajectory frames frames[0..nb) into contiguous device buffers.
+int gather_frames(cir_ctx* c, const float* xyz_d, const float* box_d, const int* frames, int nb, float* xyzc, float* boxc) {
+    const size_t fb = size_t(c->natoms) * 3 * sizeof(float);
+    for (int q = 0; q < nb; ++q) {
+        CIR_HIP(hipMemcpyAsync(reinterpret_cast<char*>(xyzc) + q * fb,
+                               reinterpret_cast<const char*>(xyz_d) + size_t(frames[q]) * fb, fb, hipMemcpyDeviceToDevice,
+                               c->stream));
+        CIR_HIP(hipMemcpyAsync(boxc + 3 * q, box_d + 3 * size_t(frames[q]), 3 * sizeof(float), hipMemcpyDeviceToDevice,
+                               c->stream));
+    }
+    return CIR_OK;
+}
+
+int auto_chunk(const cir_ctx* c, int nuniq) {
+    if (c->p.chunk > 0) return std::min(c->p.chunk, nuniq);
+    const double per = 2.0 * double(c->N) * c->ld * c->esz;  // H + V
+    int B = int(std::max(1.0, (24.0 * (1 << 30)) / per));
+    B = std::min(B, 512);
+    return std::max(1, std::min(B, nuniq));
+}
+
+int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
+    const cir_params& p = c->p;
+    Schedule S;
+    CIR_TRY(build_schedule(p, nframes, times, S));
+    const int nu = int(S.uniq.size());
+    const int N = c->N, ld = c->ld, ns = p.nsamples, nt = p.ntcfpoints, nom = c->nomega;
+    const int B = auto_chunk(c, nu);
+    const size_t esz = c->esz;
+    const int rp = c->real_prec;
+    hipStream_t st = c->stream;
+
+    DevBuf xyzc, boxc, sites, H, V, w, work, wmult, worig, m2, Y0, Y1, G, act, tgt, out, acc, skip;
+    CIR_TRY(xyzc.alloc(size_t(B) * c->natoms * 3 * sizeof(float)));
+    CIR_TRY(boxc.alloc(size_t(B) * 3 * sizeof(float)));
+    CIR_TRY(sites.alloc(size_t(B) * CIR_NSITE * N * esz));
+    CIR_TRY(H.alloc(size_t(B) * N * ld * esz));
+    CIR_TRY(V.alloc(size_t(B) * N * ld * esz));
+    CIR_TRY(w.alloc(size_t(B) * N * sizeof(double)));
+    CIR_TRY(work.alloc(eigh_workspace(rp, B, N, ld)));
+    CIR_TRY(wmult.alloc(size_t(nu) * sizeof(double)));
+    CIR_TRY(worig.alloc(size_t(nu) * sizeof(double)));
+    CIR_TRY(m2.alloc(size_t(B) * N * sizeof(double)));
+    const int nslot = S.nslot;
+    const int ldy = 18 * nslot;
+    CIR_TRY(Y0.alloc(size_t(N) * ldy * esz));
+    CIR_TRY(Y1.alloc(size_t(N) * ldy * esz));
+    CIR_TRY(G.alloc(size_t(N) * ldy * esz));
+    CIR_TRY(act.alloc(S.action.size() * sizeof(int)));
+    CIR_TRY(tgt.alloc(S.target.size() * sizeof(int)));
+    CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
+    CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
+    CIR_TRY(skip.alloc(sizeof(long long)));
+    CIR_HIP(hipMemcpyAsync(wmult.p, S.mult.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
+    CIR_HIP(hipMemcpyAsync(worig.p, S.orig.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
+    CIR_HIP(hipMemcpyAsync(act.p, S.action.data(), S.action.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    CIR_HIP(hipMemcpyAsync(tgt.p, S.target.data(), S.target.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    CIR_HIP(hipMemsetAsync(out.p, 0, out.bytes, st));
+    CIR_HIP(hipMemsetAsync(acc.p, 0, acc.bytes, st));
+    CIR_HIP(hipMemsetAsync(skip.p, 0, skip.bytes, st));
+    CIR_HIP(hipMemsetAsync(Y0.p, 0, Y0.bytes, st));
+    CIR_HIP(hipMemsetAsync(Y1.p, 0, Y1.bytes, st));
+
+    double* accp = acc.as<double>();
+    void* Ycur = Y0.p;
+    void* Ynext = Y1.p;
+    for (int c0 = 0; c0 < nu; c0 += B) {
+        const int nb = std::min(B, nu - c0);
+        CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxc.as<float>()));
+        CIR_TRY(cir_eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
+                                p.cutoff, xyzc.as<float>(), boxc.as<float>(), sites.p, st));
+        CIR_TRY(cir_hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, boxc.as<float>(), sites.p,
+                                H.p, st));
+        CIR_TRY(eigh_batched(rp, nb, N, ld, H.p, V.p, w.as<double>(), work.p, st));
+        CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
+                         wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom, skip.as<long long>(), st));
+        CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
+                                 sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st));
+        for (int u = c0; u < c0 + nb; ++u) {
+            const int bl = u - c0;
+            const char* Vu = V.as<char>() + size_t(bl) * N * ld * esz;
+            const double* wu = w.as<double>() + size_t(bl) * N;
+            const char* su = sites.as<char>() + size_t(bl) * CIR_NSITE * N * esz;
+            bool has_prop = false;
+            for (int q = 0; q < nslot; ++q) has_prop |= S.action[size_t(u) * nslot + q] == 2;
+            if (has_prop)
+                CIR_TRY(propagate(rp, c->prop_prec, N, ld, 9 * nslot, ldy, double(p.dt), Vu, wu, Ycur, Ynext, G.p, st));
+            CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, st));
+            CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, tgt.as<int>() + size_t(u) * nslot, su,
+                              Ynext, out.as<double>(), st));
+            std::swap(Ycur, Ynext);
+        }
+    }
+    std::vector<double> hout(size_t(ns) * nt * 6), hacc(4 * size_t(nom));
+    long long hskip = 0;
+    CIR_HIP(hipMemcpyAsync(hout.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
+    CIR_HIP(hipMemcpyAsync(hacc.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
+    CIR_HIP(hipMemcpyAsync(&hskip, skip.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+    CIR_HIP(hipStreamSynchronize(st));
+    c->nframes_used = nu;
+    return finalize(c, hout, hacc, hskip);
+}
+
+int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
+    const cir_params& p = c->p;
+    Schedule S;
+    CIR_TRY(build_schedule(p, nframes, times, S));
+    const int N = c->N, ld = c->ld, ns = p.nsamples, nt = p.ntcfpoints, nom = c->nomega;
+    const size_t esz = c->esz;
+    const int rp = c->real_prec;
+    hipStream_t st = c->stream;
+    const int ldf = 2 * N;
+    DevBuf xyz1, box1, sites1, sites0, H, V, w, work, m2, F, G, Y, out, acc, skip;
+    CIR_TRY(xyz1.alloc(size_t(c->natoms) * 3 * sizeof(float)));
+    CIR_TRY(box1.alloc(3 * sizeof(float)));
+    CIR_TRY(sites1.alloc(size_t(CIR_NSITE) * N * esz));
+    CIR_TRY(sites0.alloc(size_t(CIR_NSITE) * N * esz));
+    CIR_TRY(H.alloc(size_t(N) * ld * esz));
+    CIR_TRY(V.alloc(size_t(N) * ld * esz));
+    CIR_TRY(w.alloc(size_t(N) * sizeof(double)));
+    CIR_TRY(work.alloc(eigh_workspace(rp, 1, N, ld)));
+    CIR_TRY(m2.alloc(size_t(N) * sizeof(double)));
+    CIR_TRY(F.alloc(size_t(N) * ldf * esz));
+    CIR_TRY(G.alloc(size_t(N) * ldf * esz));
+    CIR_TRY(Y.alloc(size_t(N) * 18 * esz));
+    CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
+    CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
+    CIR_TRY(skip.alloc(sizeof(long long)));
+    CIR_HIP(hipMemsetAsync(out.p, 0, out.bytes, st));
+    CIR_HIP(hipMemsetAsync(acc.p, 0, acc.bytes, st));
+    CIR_HIP(hipMemsetAsync(skip.p, 0, skip.bytes, st));
+    double* accp = acc.as<double>();
+    for (int s = 0; s < ns; ++s) {
+        for (int k = 0; k < nt; ++k) {
+            const int f = S.fidx[size_t(s) * nt + k];
+            CIR_TRY(gather_frames(c, xyz_d, box_d, &f, 1, xyz1.as<float>(), box1.as<float>()));
+            CIR_TRY(cir_eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
+                                    p.cutoff, xyz1.as<float>(), box1.as<float>(), sites1.p, st));
+            CIR_TRY(cir_hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, box1.as<float>(),
+                                    sites1.p, H.p, st));
+            CIR_TRY(eigh_batched(rp, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st));
+            CIR_TRY(ipr_hist(rp, 1, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
+                             nullptr, accp + nom, accp + 2 * nom, accp + 3 * nom, skip.as<long long>(), st));
+            if (k == 0) {
+                CIR_TRY(spectral_density(rp, 1, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
+                                         w.as<double>(), sites1.p, nullptr, accp, m2.as<double>(), st));
+                CIR_TRY(set_identity(rp, N, ldf, F.p, st));
+                CIR_HIP(hipMemcpyAsync(sites0.p, sites1.p, sites1.bytes, hipMemcpyDeviceToDevice, st));
+            } else {
+                CIR_TRY(propagate(rp, c->prop_prec, N, ld, N, ldf, double(p.dt), V.p, w.as<double>(), F.p, F.p, G.p, st));
+            }
+            // Y = F M(0): Re and Im blocks
+            for (int part = 0; part < 2; ++part) {
+                GemmArgs g;
+                g.M = N;
+                g.N = 9;
+                g.K = N;
+                g.A = F.as<char>() + size_t(part) * N * esz;
+                g.a_m = ldf;
+                g.a_k = 1;
+                g.B = sites0.as<char>() + size_t(CIR_SITE_MU) * N * esz;
+                g.b_k = 1;
+                g.b_n = N;
+                g.C = Y.as<char>() + size_t(part) * 9 * esz;
+                g.c_m = 18;
+                g.c_n = 1;
+                CIR_TRY(gemm(c->prop_prec, g, st));
+            }
+            CIR_TRY(correlate(rp, N, 1, 18, nullptr, nullptr, sites1.p, Y.p,
+                              out.as<double>() + 6 * (size_t(s) * nt + k), st));
+        }
+    }
+    std::vector<double> hout(size_t(ns) * nt * 6), hacc(4 * size_t(nom));
+    long long hskip = 0;
+    CIR_HIP(hipMemcpyAsync(hout.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
+    CIR_HIP(hipMemcpyAsync(hacc.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
+    CIR_HIP(hipMemcpyAsync(&hskip, skip.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+    CIR_HIP(hipStreamSynchronize(st));
+    c->nframes_used = int64_t(ns) * nt;
+    return finalize(c, hout, hacc, hskip);
+}
+
+}  // namespace
+
+extern "C" int cir_device_count(int* n) {
+    if (!n) return fail(CIR_ERR_ARG, "null argument");
+    int cnt = 0;
+    hipError_t e = hipGetDeviceCount(&cnt);
+    if (e != hipSuccess) {
+        *n = 0;
+        return hip_fail(e, "hipGetDeviceCount", __FILE__, __LINE__);
+    }
+    *n = cnt;
+    return CIR_OK;
+}
+
+extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx** out) {
+    if (!p || !out) return fail(CIR_ERR_ARG, "cir_create: null argument");
+    *out = nullptr;
+    int imodel = 0, ispecies = 0;
+    CIR_TRY(cir_resolve_model(p->model, &imodel));
+    CIR_TRY(cir_resolve_species(p->species, &ispecies));
+    MapCoef mc;
+    if (!map_coef(p->imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map must be 0 or 1");
+    if (p->natom_mol < 3 || p->nchrom_mol < 1 || natoms <= 0 || natoms % p->natom_mol)
+        return fail(CIR_ERR_ARG, "natoms must be a positive multiple of natom_mol (>= 3)");
+    if (p->ntcfpoints <= 0 || p->nsamples <= 0 || p->omegaStep <= 0 || p->nzeros < 0 || !(p->t1 > 0) || !(p->dt > 0))
+        return fail(CIR_ERR_ARG, "ntcfpoints, nsamples, omegaStep, t1 and dt must be positive, nzeros >= 0");
+    if (cir_nomega(p) <= 0) return fail(CIR_ERR_ARG, "omegaStop must not be below omegaStart");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CIR_ERR_NODEVICE, "no HIP device available");
+    cir_ctx* c = new cir_ctx();
+    c->p = *p;
+    c->imodel = imodel;
+    c->ispecies = ispecies;
+    c->natoms = natoms;
+    c->nmol = natoms / p->natom_mol;
+    c->N = c->nmol * p->nchrom_mol;
+    c->ld = (c->N + 3) & ~3;
+    c->nomega = cir_nomega(p);
+    c->nfft = p->ntcfpoints + p->nzeros;
+    c->real_prec = (p->precision == CIR_PREC_FP64) ? CIR_PREC_FP64 : CIR_PREC_FP32;
+    c->prop_prec = p->precision;
+    c->esz = (c->real_prec == CIR_PREC_FP64) ? sizeof(double) : sizeof(float);
+    c->stream = as_stream(stream);
+    *out = c;
+    return CIR_OK;
+}
+
+extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
+    if (!c || !xyz_d || !box_d || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_run_device: bad arguments");
+    c->have = false;
+    hipEvent_t e0, e1;
+    CIR_HIP(hipEventCreate(&e0));
+    CIR_HIP(hipEventCreate(&e1));
+    CIR_HIP(hipEventRecord(e0, c->stream));
+    int st = (c->p.mode == CIR_MODE_PLAIN) ? run_plain(c, nframes, xyz_d, box_d, times)
+                                           : run_batched(c, nframes, xyz_d, box_d, times);
+    if (hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) c->seconds = ms * 1e-3;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* box, const float* times) {
+    if (!c || !xyz || !box || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_run: bad arguments");
+    const size_t nx = size_t(nframes) * c->natoms * 3;
+    for (size_t i = 0; i < nx; ++i)
+        if (!std::isfinite(xyz[i])) return fail(CIR_ERR_NONFINITE, "non-finite coordinate in the trajectory");
+    for (size_t i = 0; i < size_t(nframes) * 3; ++i)
+        if (!std::isfinite(box[i]) || !(box[i] > 0)) return fail(CIR_ERR_NONFINITE, "non-finite or non-positive box");
+    DevBuf dx, db;
+    CIR_TRY(dx.alloc(nx * sizeof(float)));
+    CIR_TRY(db.alloc(size_t(nframes) * 3 * sizeof(float)));
+    CIR_HIP(hipMemcpyAsync(dx.p, xyz, nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CIR_HIP(hipMemcpyAsync(db.p, box, size_t(nframes) * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    return cir_run_device(c, nframes, dx.as<float>(), db.as<float>(), times);
+}
+
+extern "C" int cir_get_results(cir_ctx* c, cir_results* r) {
+    if (!c || !r) return fail(CIR_ERR_ARG, "cir_get_results: null argument");
+    if (!c->have) return fail(CIR_ERR_ARG, "cir_get_results: no completed run");
+    const int nt = c->p.ntcfpoints;
+    double* tc[3] = {r->tcf_ir, r->tcf_vv, r->tcf_vh};
+    double* ft[3] = {r->ftcf_ir, r->ftcf_vv, r->ftcf_vh};
+    for (int s = 0; s < 3; ++s) {
+        if (tc[s]) std::memcpy(tc[s], c->tcf[s].data(), 2 * size_t(nt) * sizeof(double));
+        if (ft[s]) std::memcpy(ft[s], c->ftcf[s].data(), size_t(c->nfft) * sizeof(double));
+    }
+    const size_t nb = size_t(c->nomega) * sizeof(double);
+    if (r->sw) std::memcpy(r->sw, c->sw.data(), nb);
+    if (r->pw) std::memcpy(r->pw, c->pw.data(), nb);
+    if (r->rw) std::memcpy(r->rw, c->rw.data(), nb);
+    if (r->rmw) std::memcpy(r->rmw, c->rmw.data(), nb);
+    r->nsteps = c->nsteps;
+    r->nframes_used = c->nframes_used;
+    r->hist_skipped = c->hist_skipped;
+    r->seconds = c->seconds;
+    return CIR_OK;
+}
+
+extern "C" void cir_destroy(cir_ctx* c) { delete c; }
+
+// Per-stage C entry points over the kernels above -----------------------------
+
+extern "C" int cir_eigh_batched(int prec, int batch, int n, int ld, void* A_d, void* V_d, double* w_d, void* work_d,
+                                void* stream) {
+    return eigh_batched(prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32, batch, n, ld, A_d, V_d, w_d, work_d,
+                        as_stream(stream));
+}
+
+extern "C" int cir_ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart,
+                            int omegaStep, int nomega, const void* V_d, const double* w_d, const double* weight_d,
+                            double* pw_d, double* rw_d, double* rmw_d, long long* skipped_d, void* stream) {
+    return ipr_hist(prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32, batch, n, ld, nchrom_mol, avef, omegaStart,
+                    omegaStep, nomega, V_d, w_d, weight_d, pw_d, rw_d, rmw_d, skipped_d, as_stream(stream));
+}
+
+extern "C" int cir_spectral_density(int prec, int batch, int n, int ld, float t1, float avef, float omegaStart,
+                                    int omegaStep, int nomega, const void* V_d, const double* w_d, const void* sites_d,
+                                    const double* weight_d, double* sw_d, double* work_d, void* stream) {
+    return spectral_density(prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32, batch, n, ld, t1, avef, omegaStart,
+                            omegaStep, nomega, V_d, w_d, sites_d, weight_d, sw_d, work_d, as_stream(stream));
+}
+
+extern "C" int cir_propagate(int prec, int n, int ld, int ncols, int ldy, float dt, const void* V_d,
+                             const double* w_d, const void* Yin_d, void* Yout_d, void* work_d, void* stream) {
+    const int rp = prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32;
+    return propagate(rp, prec, n, ld, ncols, ldy, double(dt), V_d, w_d, Yin_d, Yout_d, work_d, as_stream(stream));
+}
+
+extern "C" int cir_correlate(int prec, int n, int nslot, int ldy, const int* action_d, const int* target_d,
+                             const void* sites_d, const void* Y_d, double* out_d, void* stream) {
+    return correlate(prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32, n, nslot, ldy, action_d, target_d, sites_d,
+                     Y_d, out_d, as_stream(stream));
+}
+
+extern "C" int cir_spectrum_c2r(int nspec, int nfft, int nx, const double* X_d, double* y_d, void* stream) {
+    return spectrum_c2r(nspec, nfft, nx, X_d, y_d, as_stream(stream));
+}
+
+extern "C" int cir_gemm(int prec, int M, int N, int K, int batch, const void* A, long long a_m, long long a_k,
+                        long long a_b, const void* B, long long b_k, long long b_n, long long b_b, void* C,
+                        long long c_m, long long c_n, long long c_b, double alpha, double beta, void* stream) {
+    GemmArgs g;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.batch = batch;
+    g.A = A;
+    g.a_m = a_m;
+    g.a_k = a_k;
+    g.a_b = a_b;
+    g.B = B;
+    g.b_k = b_k;
+    g.b_n = b_n;
+    g.b_b = b_b;
+    g.C = C;
+    g.c_m = c_m;
+    g.c_n = c_n;
+    g.c_b = c_b;
+    g.alpha = alpha;
+    g.beta = beta;
+    return gemm(prec, g, as_stream(stream));
+}

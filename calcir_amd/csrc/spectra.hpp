@@ -1,0 +1,30 @@
+// spectra.hpp -- host launchers of the eigensolver, spectra and propagation
+// kernels (k_eigh.hip, k_spectra.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace cir {
+
+size_t eigh_workspace(int prec, int batch, int n, int ld);
+int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w, void* work, hipStream_t s);
+
+int ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart, int omegaStep,
+             int nomega, const void* V, const double* w, const double* weight, double* pw, double* rw,
+             double* rmw, long long* skipped, hipStream_t s);
+int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, float omegaStart, int omegaStep,
+                     int nomega, const void* V, const double* w, const void* sites, const double* weight,
+                     double* sw, double* m2work, hipStream_t s);
+// Yout = V exp(+i diag(w) dt/hbar) V^T Yin, Y row-major n x ldy with Re in
+// columns [0, ncols) and Im in [ncols, 2 ncols); G: n*2*ncols work.  Yout
+// may alias Yin.
+int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
+              const double* w, const void* Yin, void* Yout, void* G, hipStream_t s);
+int set_identity(int prec, int n, int ldf, void* F, hipStream_t s);
+int slot_fixup(int prec, int n, int nslot, int ldy, const int* action, const void* sites, const void* Yold,
+               void* Ynew, hipStream_t s);
+int correlate(int prec, int n, int nslot, int ldy, const int* action, const int* target, const void* sites,
+              const void* Y, double* out, hipStream_t s);
+int spectrum_c2r(int nspec, int nfft, int nx, const double* X, double* y, hipStream_t s);
+
+}  // namespace cir

@@ -1,0 +1,228 @@
+/*
+ * calcir.h -- C ABI of libcalcir, the MI355X (gfx950) spectroscopy hot path
+ * of calcIR (nicholashestand/calcIR): electric-field projection, Skinner
+ * maps, exciton Hamiltonian, batched symmetric eigensolver, propagation,
+ * IR/Raman correlation functions and lineshapes.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (cir_status); none exits the
+ *     process.  cir_strerror() names a status; cir_last_error() returns the
+ *     detail text of the most recent failure on the calling thread.
+ *   - Pointers named *_d are device pointers (hipMalloc'd, on the current
+ *     HIP device); "stream" is a hipStream_t passed as void* (NULL = the
+ *     null stream).  Per-stage calls enqueue work and return without
+ *     synchronising.
+ *   - A cir_ctx owns its device buffers, is bound to one stream and is not
+ *     thread-safe; separate contexts may run concurrently.
+ *   - Real arrays of the per-stage calls hold "real_t" = float for
+ *     CIR_PREC_FP32/CIR_PREC_BF16X3 and double for CIR_PREC_FP64.
+ *
+ * Reference boundary replaced (file:line in calcIR.cu / calcIR.h):
+ *   calcIR.exe <input>                calcIR.cu:9-18     -> calcir CLI (cir_main)
+ *   ir_init                            calcIR.cu:1539-1643 -> cir_params_read
+ *   defaults                           calcIR.cu:36-59   -> cir_params_default
+ *   model/species resolution           calcIR.cu:324-339 -> cir_resolve_model/_species
+ *   sample x frame loop + finalize     calcIR.cu:360-919 -> cir_create/cir_run/cir_get_results
+ *   get_eproj_GPU                      calcIR.cu:1020-1168, calcIR.h:51-53 -> cir_eproj_sites
+ *   get_kappa_GPU                      calcIR.cu:1176-1406, calcIR.h:56-60 -> cir_eproj_sites + cir_hamiltonian
+ *   magma_ssyevd_gpu                   calcIR.cu:420-448 -> cir_eigh_batched
+ *   get_spectral_density + 3 sgemv     calcIR.cu:458-482, 1414-1444, calcIR.h:62-64 -> cir_spectral_density
+ *   IPR / Pw,Rw,Rmw host loop          calcIR.cu:494-538 -> cir_ipr_hist
+ *   cast_to_complex/makeI/Pinit + cgemm x3 + ccopy   calcIR.cu:549-617, 1481-1535,
+ *                                      calcIR.h:66-76    -> cir_propagate (+ cir_gemm)
+ *   15 x (cgemv + cdotu)               calcIR.cu:628-745 -> cir_correlate
+ *   cufftExecC2R x 3                   calcIR.cu:775-827 -> cir_spectrum_c2r
+ *   output writers                     calcIR.cu:843-919 -> cir_write_outputs
+ *   minImage/mag3/dot3                 calcIR.cu:1457-1476, calcIR.h:79-88 -> cir_minimage
+ */
+#ifndef CALCIR_H
+#define CALCIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cir_status {
+    CIR_OK = 0,
+    CIR_ERR_ARG = 1,        /* invalid argument or size                        */
+    CIR_ERR_IO = 2,         /* input file / output file could not be used       */
+    CIR_ERR_MODEL = 3,      /* model string not recognised (calcIR.cu:327-330)  */
+    CIR_ERR_SPECIES = 4,    /* species not recognised (calcIR.cu:336-339)       */
+    CIR_ERR_MAP = 5,        /* map is neither 0 nor 1 (calcIR.cu:1207-1242)     */
+    CIR_ERR_FRAME = 6,      /* frame missing or time check failed (calcIR.cu:379-395) */
+    CIR_ERR_HIP = 7,        /* HIP runtime error                                */
+    CIR_ERR_NOMEM = 8,      /* device or host allocation failed                 */
+    CIR_ERR_EIGH = 9,       /* eigensolver failure (calcIR.cu:445)              */
+    CIR_ERR_NONFINITE = 10, /* non-finite coordinate or box                     */
+    CIR_ERR_NODEVICE = 11   /* no HIP device available                          */
+} cir_status;
+
+enum { CIR_PREC_FP32 = 0, CIR_PREC_FP64 = 1, CIR_PREC_BF16X3 = 2 };
+enum { CIR_MODE_PLAIN = 0, CIR_MODE_BATCHED = 1 };
+
+#define CIR_STRLEN 256
+#define CIR_NSITE 20 /* real_t fields per chromophore in a site record */
+
+/* Site-record field offsets (each field is an array of N values). */
+enum {
+    CIR_SITE_E = 0, CIR_SITE_W = 1, CIR_SITE_X = 2, CIR_SITE_P = 3, CIR_SITE_MUP = 4,
+    CIR_SITE_U = 5,   /* 3 fields: unit OH vector                       */
+    CIR_SITE_R = 8,   /* 3 fields: min-imaged TDM position (nm)         */
+    CIR_SITE_MU = 11, /* 3 fields: transition dipole                    */
+    CIR_SITE_A = 14   /* 6 fields: alpha xx, yy, zz, xy, yz, zx         */
+};
+
+/* Input keys of calcIR.cu:1556-1638 plus extensions. */
+typedef struct cir_params {
+    char  xtcf[CIR_STRLEN];    /* trajectory; "synth:<nwater>:<nframes>:<seed>[:<L_nm>]" selects the
+                                  seeded synthetic source                            */
+    char  outf[CIR_STRLEN];
+    char  cptf[CIR_STRLEN];    /* parsed and unused, as in the reference           */
+    char  model[CIR_STRLEN];
+    char  species[CIR_STRLEN];
+    int   ntcfpoints;
+    int   nsamples;
+    float sampleEvery;
+    int   omegaStep;
+    int   natom_mol;
+    int   nchrom_mol;
+    int   nzeros;
+    int   imap;                /* key "map"                                        */
+    float dt;
+    float t1;
+    float avef;
+    float beginTime;
+    float omegaStart;
+    float omegaStop;
+    /* extensions */
+    float cutoff;              /* key "cutoff" (nm), default 0.7831 (calcIR.cu:1035) */
+    int   precision;           /* key "precision": fp32 | fp64 | bf16x3 (default fp32) */
+    int   mode;                /* key "mode": batched (default) | plain             */
+    int   chunk;               /* key "chunk": frames per eigensolver batch, 0 = auto */
+} cir_params;
+
+/* Results of a run, all arrays caller-allocated (host).  NULL members are
+ * skipped. */
+typedef struct cir_results {
+    double* tcf_ir;   /* 2*ntcfpoints: decayed, 1/nsamples-scaled TCF (re, im) pairs */
+    double* tcf_vv;
+    double* tcf_vh;
+    double* ftcf_ir;  /* ntcfpoints+nzeros: C2R of the padded TCF              */
+    double* ftcf_vv;
+    double* ftcf_vh;
+    double* sw;       /* nomega each                                            */
+    double* pw;
+    double* rw;
+    double* rmw;
+    int64_t nsteps;        /* (sample, tcf point) steps                        */
+    int64_t nframes_used;  /* distinct trajectory frames diagonalised          */
+    int64_t hist_skipped;  /* modes outside the omega window (not binned)      */
+    double  seconds;       /* device time of the last cir_run                  */
+} cir_results;
+
+typedef struct cir_ctx cir_ctx;
+
+/* --- host-side boundary (no GPU needed) --------------------------------- */
+const char* cir_strerror(int status);
+const char* cir_last_error(void);
+void  cir_params_default(cir_params* p);                        /* calcIR.cu:36-59 */
+int   cir_params_read(const char* path, cir_params* p, int verbose);  /* calcIR.cu:1539-1643 */
+void  cir_params_echo(const cir_params* p);                     /* calcIR.cu:70-87 */
+int   cir_resolve_model(const char* model, int* imodel);        /* calcIR.cu:325-330 */
+int   cir_resolve_species(const char* species, int* ispecies);  /* calcIR.cu:332-339 */
+int   cir_nomega(const cir_params* p);                          /* calcIR.cu:94 */
+float cir_minimage(float dx, float boxl);                       /* calcIR.cu:1457-1460 */
+/* Frame selected for (sample, point): calcIR.cu:378-379. */
+int   cir_frame_index(const cir_params* p, int sample, int point, float frame_dt, float* desired_time);
+/* Writes the 13 output files of calcIR.cu:843-919 from r. */
+int   cir_write_outputs(const cir_params* p, const cir_results* r);
+/* Synthetic trajectories (replacing the XTC reader, calcIR.cu:212-227, 341-398). */
+float cir_synth_box_nm(int nwater);
+int   cir_synth_trajectory(int nwater, int nframes, uint64_t seed, float box_nm, float* xyz,
+                           float* times);
+/* The drop-in CLI: argv as calcIR.exe (calcIR.cu:9-18). */
+int   cir_main(int argc, char** argv);
+
+/* --- whole pipeline ------------------------------------------------------ */
+int   cir_device_count(int* n);
+int   cir_create(const cir_params* p, int natoms, void* stream, cir_ctx** out);
+/* Runs every sample of p over an in-memory trajectory (host arrays:
+ * xyz nframes*natoms*3 nm, box nframes*3, times nframes ps). */
+int   cir_run(cir_ctx* ctx, int nframes, const float* xyz, const float* box, const float* times);
+/* Same with the trajectory already on the device. */
+int   cir_run_device(cir_ctx* ctx, int nframes, const float* xyz_d, const float* box_d,
+                     const float* times);
+int   cir_get_results(cir_ctx* ctx, cir_results* r);
+void  cir_destroy(cir_ctx* ctx);
+
+/* --- per-stage device entry points --------------------------------------- */
+/* Field projection (calcIR.cu:1020-1168) fused with the per-row site record
+ * of get_kappa_GPU (calcIR.cu:1245-1304).  xyz_d nframes*natoms*3 float,
+ * box_d nframes*3 float; sites_d nframes*CIR_NSITE*N real_t. */
+int cir_eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel,
+                    int ispecies, int imap, float cutoff, const float* xyz_d, const float* box_d,
+                    void* sites_d, void* stream);
+/* Exciton Hamiltonian (calcIR.cu:1306-1404) from site records: H_d
+ * nframes*N*ld real_t, symmetric (both triangles written), ld >= N. */
+int cir_hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies, int imap,
+                    float avef, const float* box_d, const void* sites_d, void* H_d, void* stream);
+/* Batched symmetric eigensolver replacing magma_ssyevd_gpu (calcIR.cu:443):
+ * A_d holds batch matrices (n x n, leading dimension ld, batch stride n*ld,
+ * symmetric, both triangles) and is destroyed (it receives the Householder
+ * reflectors); V_d receives the eigenvectors row-major, V[b][i*ld + k] =
+ * component i of eigenvector k, and w_d (batch*n doubles) the eigenvalues in
+ * ascending order.  Real arrays are float unless prec == CIR_PREC_FP64.
+ * work_d must hold cir_eigh_workspace() bytes. */
+size_t cir_eigh_workspace(int prec, int batch, int n, int ld);
+int cir_eigh_batched(int prec, int batch, int n, int ld, void* A_d, void* V_d, double* w_d, void* work_d,
+                     void* stream);
+/* IPR / frequency distribution (calcIR.cu:494-538): for each matrix b adds
+ * weight[b]/omegaStep per mode (weight_d NULL = 1) into pw/rw/rmw (nomega
+ * doubles each); skipped_d (one int64) counts modes outside the window,
+ * which the reference writes out of bounds (calcIR.cu:530-537). */
+int cir_ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart,
+                 int omegaStep, int nomega, const void* V_d, const double* w_d,
+                 const double* weight_d, double* pw_d, double* rw_d, double* rmw_d,
+                 long long* skipped_d, void* stream);
+/* Spectral density (calcIR.cu:458-482, 1414-1444): sw_d += weight[b] *
+ * sum_k |V^T mu|_k^2 Lorentz(omega - w_k - avef), mu from the site records;
+ * work_d batch*n doubles. */
+int cir_spectral_density(int prec, int batch, int n, int ld, float t1, float avef,
+                         float omegaStart, int omegaStep, int nomega, const void* V_d,
+                         const double* w_d, const void* sites_d, const double* weight_d,
+                         double* sw_d, double* work_d, void* stream);
+/* One propagation step (calcIR.cu:597-617, Pinit 1498-1516):
+ * Yout = V exp(+i diag(w) dt/hbar) V^T Yin for ncols complex columns stored
+ * row-major n x ldy, real parts in columns [0, ncols), imaginary parts in
+ * [ncols, 2 ncols).  With ncols = n and Yin = Yout = [Re F | Im F] this is the
+ * literal F-matrix step; with the 9 columns F M(0) of each time origin it is
+ * the vector form.  prec CIR_PREC_BF16X3 runs the two products as split
+ * bf16 x 3.  work_d: n*2*ncols reals. */
+int cir_propagate(int prec, int n, int ld, int ncols, int ldy, float dt, const void* V_d,
+                  const double* w_d, const void* Yin_d, void* Yout_d, void* work_d, void* stream);
+/* Correlations (calcIR.cu:628-745): for each of nslot time origins s with
+ * action_d[s] in {1, 2} (NULL = all), the 15 bilinear forms M_b(t)^T Y_a of
+ * Y = F M(0) (slot s: columns 9s..9s+8 of Y_d as in cir_propagate) with
+ * M(t) from sites_d, combined into IR, VV, VH and stored as 6 doubles at
+ * out_d + 6*target_d[s] (target_d NULL = 0). */
+int cir_correlate(int prec, int n, int nslot, int ldy, const int* action_d, const int* target_d,
+                  const void* sites_d, const void* Y_d, double* out_d, void* stream);
+/* cuFFT C2R contract (calcIR.cu:790-791) for nspec spectra: X_d nspec*nx
+ * complex doubles (re, im), y_d nspec*nfft doubles. */
+int cir_spectrum_c2r(int nspec, int nfft, int nx, const double* X_d, double* y_d, void* stream);
+/* Batched dense product C = alpha A B + beta C (the library GEMMs of
+ * calcIR.cu:462-467, 601-613): element (m,k) of A at A[m*a_m + k*a_k] (one of
+ * the two strides 1), B (k,n) at B[k*b_k + n*b_n], C (m,n) at C[m*c_m + n*c_n],
+ * batch strides a_b, b_b, c_b.  prec: FP32 MFMA, BF16X3 MFMA or FP64. */
+int cir_gemm(int prec, int M, int N, int K, int batch, const void* A, long long a_m, long long a_k,
+             long long a_b, const void* B, long long b_k, long long b_n, long long b_b, void* C,
+             long long c_m, long long c_n, long long c_b, double alpha, double beta, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CALCIR_H */

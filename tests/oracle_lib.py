@@ -1,0 +1,206 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module, as the checker or as the timed CPU
+baseline.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+F32, F64 = 0, 1
+PROP_LITERAL, PROP_VECTOR = 0, 1
+SITE_E, SITE_W, SITE_X, SITE_P, SITE_MUP, SITE_U, SITE_R, SITE_MU, SITE_A, SITE_STRIDE = (
+    0, 1, 2, 3, 4, 5, 8, 11, 14, 20)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("natom_mol", ctypes.c_int), ("nchrom_mol", ctypes.c_int), ("imodel", ctypes.c_int),
+        ("ispecies", ctypes.c_int), ("imap", ctypes.c_int), ("cutoff", ctypes.c_float),
+        ("avef", ctypes.c_float), ("dt", ctypes.c_float), ("t1", ctypes.c_float),
+        ("ntcfpoints", ctypes.c_int), ("nsamples", ctypes.c_int), ("nzeros", ctypes.c_int),
+        ("sampleEvery", ctypes.c_float), ("beginTime", ctypes.c_float),
+        ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float), ("omegaStep", ctypes.c_int),
+        ("propagation", ctypes.c_int), ("nthreads", ctypes.c_int),
+    ]
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("tcf_ir", _D), ("tcf_vv", _D), ("tcf_vh", _D),
+        ("ftcf_ir", _D), ("ftcf_vv", _D), ("ftcf_vh", _D),
+        ("sw", _D), ("pw", _D), ("rw", _D), ("rmw", _D),
+        ("nsteps", ctypes.c_int64), ("hist_skipped", ctypes.c_int64),
+        ("t_hamiltonian", ctypes.c_double), ("t_eigh", ctypes.c_double),
+        ("t_hist", ctypes.c_double), ("t_propagate", ctypes.c_double),
+        ("t_finalize", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        L = ctypes.CDLL(ORACLE_LIB)
+        vp = ctypes.c_void_p
+        i = ctypes.c_int
+        cp = ctypes.POINTER(Config)
+        L.orc_minimage.restype = ctypes.c_double
+        L.orc_minimage.argtypes = [i, ctypes.c_double, ctypes.c_double]
+        L.orc_nomega.restype = i
+        L.orc_nomega.argtypes = [cp]
+        L.orc_frame_index.restype = i
+        L.orc_frame_index.argtypes = [cp, i, i, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
+        L.orc_eproj.argtypes = [i, cp, vp, i, vp, vp]
+        L.orc_sites.argtypes = [i, cp, vp, i, vp, vp, vp]
+        L.orc_hamiltonian.argtypes = [i, cp, vp, i, vp, vp, vp]
+        L.orc_eigh.argtypes = [i, i, vp, vp, vp, i]
+        L.orc_spectral_density.argtypes = [i, cp, i, vp, vp, vp, vp]
+        L.orc_ipr_hist.restype = ctypes.c_int64
+        L.orc_ipr_hist.argtypes = [i, cp, i, vp, vp, ctypes.c_double, vp, vp, vp]
+        L.orc_c2r.argtypes = [i, i, vp, vp]
+        L.orc_run.argtypes = [i, cp, i, i, vp, vp, vp, ctypes.POINTER(Result)]
+        for name in ("orc_eproj", "orc_sites", "orc_hamiltonian", "orc_eigh", "orc_spectral_density",
+                     "orc_c2r", "orc_run"):
+            getattr(L, name).restype = i
+        _lib = L
+    return _lib
+
+
+def config(imodel=1, ispecies=0, imap=0, cutoff=0.7831, avef=3415.2, dt=0.010, t1=0.260,
+           ntcfpoints=150, nsamples=1, nzeros=25600, sampleEvery=10.0, beginTime=0.0,
+           omegaStart=2000.0, omegaStop=5000.0, omegaStep=5, propagation=PROP_VECTOR, nthreads=0,
+           natom_mol=4, nchrom_mol=2):
+    return Config(natom_mol, nchrom_mol, imodel, ispecies, imap, cutoff, avef, dt, t1, ntcfpoints,
+                  nsamples, nzeros, sampleEvery, beginTime, omegaStart, omegaStop, omegaStep,
+                  propagation, nthreads)
+
+
+def _c(a, dt=np.float32):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ok(st, what):
+    if st != 0:
+        raise RuntimeError("%s failed with status %d" % (what, st))
+
+
+def eproj(prec, cfg, x, box):
+    x = _c(x)
+    box = _c(box)
+    n = (x.shape[0] // cfg.natom_mol) * cfg.nchrom_mol
+    out = np.zeros(n)
+    _ok(lib().orc_eproj(prec, ctypes.byref(cfg), x.ctypes.data, x.shape[0], box.ctypes.data, out.ctypes.data),
+        "orc_eproj")
+    return out
+
+
+def sites(prec, cfg, x, box, e):
+    x = _c(x)
+    box = _c(box)
+    e = _c(e, np.float64)
+    out = np.zeros((e.size, SITE_STRIDE))
+    _ok(lib().orc_sites(prec, ctypes.byref(cfg), x.ctypes.data, x.shape[0], box.ctypes.data, e.ctypes.data,
+                        out.ctypes.data), "orc_sites")
+    return out
+
+
+def hamiltonian(prec, cfg, x, box, e):
+    x = _c(x)
+    box = _c(box)
+    e = _c(e, np.float64)
+    n = e.size
+    H = np.zeros((n, n))
+    _ok(lib().orc_hamiltonian(prec, ctypes.byref(cfg), x.ctypes.data, x.shape[0], box.ctypes.data,
+                              e.ctypes.data, H.ctypes.data), "orc_hamiltonian")
+    return H
+
+
+def eigh(prec, H, nthreads=0):
+    H = _c(H, np.float64)
+    n = H.shape[0]
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    _ok(lib().orc_eigh(prec, n, H.ctypes.data, w.ctypes.data, V.ctypes.data, nthreads), "orc_eigh")
+    return w, V
+
+
+def spectral_density(prec, cfg, w, V, mu):
+    w, V, mu = _c(w, np.float64), _c(V, np.float64), _c(mu, np.float64)
+    sw = np.zeros(lib().orc_nomega(ctypes.byref(cfg)))
+    _ok(lib().orc_spectral_density(prec, ctypes.byref(cfg), w.size, w.ctypes.data, V.ctypes.data,
+                                   mu.ctypes.data, sw.ctypes.data), "orc_spectral_density")
+    return sw
+
+
+def ipr_hist(prec, cfg, w, V, weight=1.0):
+    w, V = _c(w, np.float64), _c(V, np.float64)
+    nom = lib().orc_nomega(ctypes.byref(cfg))
+    pw, rw, rmw = np.zeros(nom), np.zeros(nom), np.zeros(nom)
+    skipped = lib().orc_ipr_hist(prec, ctypes.byref(cfg), w.size, w.ctypes.data, V.ctypes.data, weight,
+                                 pw.ctypes.data, rw.ctypes.data, rmw.ctypes.data)
+    return pw, rw, rmw, skipped
+
+
+def c2r(n, X):
+    """X: complex array (nx); returns the n real outputs of the C2R contract."""
+    X = np.asarray(X, np.complex128)
+    buf = np.empty(2 * X.size)
+    buf[0::2], buf[1::2] = X.real, X.imag
+    y = np.zeros(n)
+    _ok(lib().orc_c2r(n, X.size, buf.ctypes.data, y.ctypes.data), "orc_c2r")
+    return y
+
+
+def run(prec, cfg, xyz, box, times):
+    xyz, box, times = _c(xyz), _c(box), _c(times)
+    nframes, natoms = xyz.shape[0], xyz.shape[1]
+    nt = cfg.ntcfpoints
+    nfft = nt + cfg.nzeros
+    nom = lib().orc_nomega(ctypes.byref(cfg))
+    bufs = {k: np.zeros(2 * nt) for k in ("tcf_ir", "tcf_vv", "tcf_vh")}
+    bufs.update({k: np.zeros(nfft) for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh")})
+    bufs.update({k: np.zeros(nom) for k in ("sw", "pw", "rw", "rmw")})
+    r = Result()
+    for k, v in bufs.items():
+        setattr(r, k, v.ctypes.data_as(_D))
+    _ok(lib().orc_run(prec, ctypes.byref(cfg), natoms, nframes, xyz.ctypes.data, box.ctypes.data,
+                      times.ctypes.data, ctypes.byref(r)), "orc_run")
+    out = {}
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        out[k] = bufs[k][0::2] + 1j * bufs[k][1::2]
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw", "rw", "rmw"):
+        out[k] = bufs[k]
+    out.update(nsteps=r.nsteps, hist_skipped=r.hist_skipped,
+               t_hamiltonian=r.t_hamiltonian, t_eigh=r.t_eigh, t_hist=r.t_hist,
+               t_propagate=r.t_propagate, t_finalize=r.t_finalize)
+    return out
+
+
+def config_from_params(p):
+    """orc_config from a calcir_amd.Params (resolving model/species strings as
+    calcIR.cu:324-339 does)."""
+    model = p.model.decode()
+    species = p.species.decode()
+    imodel = 1 if model in ("tip4p2005", "e3b3") else 0 if model in ("tip4p", "e3b2") else -1
+    ispecies = {"H2O": 0, "HOD/D2O": 1, "HOD/H2O": 2, "D2O": 3}.get(species, -1)
+    return config(imodel=imodel, ispecies=ispecies, imap=p.imap, cutoff=p.cutoff, avef=p.avef, dt=p.dt,
+                  t1=p.t1, ntcfpoints=p.ntcfpoints, nsamples=p.nsamples, nzeros=p.nzeros,
+                  sampleEvery=p.sampleEvery, beginTime=p.beginTime, omegaStart=p.omegaStart,
+                  omegaStop=p.omegaStop, omegaStep=p.omegaStep, natom_mol=p.natom_mol,
+                  nchrom_mol=p.nchrom_mol)

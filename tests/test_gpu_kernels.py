@@ -1,0 +1,257 @@
+"""Per-kernel parity of the HIP path (through the C ABI) against the CPU oracle,
+numpy and a plain torch float64 reference."""
+
+import numpy as np
+import pytest
+import torch
+
+import calcir_amd as C
+import oracle_lib as O
+from calcir_amd import stages as S
+from cases import frames, rel
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+# Elementwise tolerances relative to (|A| |B|)_mn, the scale of the rounding
+# bound of a K-term dot product: FP32 MFMA (unit roundoff 6e-8, K <= 512),
+# split-BF16x3 (2^-17 per product), FP64.
+GEMM_TOL = {C.PREC_FP32: 1e-6, C.PREC_BF16X3: 1e-5, C.PREC_FP64: 1e-14}
+
+
+@pytest.mark.parametrize("prec", [C.PREC_FP32, C.PREC_BF16X3, C.PREC_FP64])
+@pytest.mark.parametrize("shape", [(1, 64, 64, 64), (3, 130, 77, 45), (2, 256, 144, 512), (1, 33, 1, 200),
+                                   (1, 300, 260, 7)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_vs_torch(prec, shape, ta, tb):
+    bt, M, N, K = shape
+    announce("gemm prec=%d shape=%s ta=%s tb=%s" % (prec, shape, ta, tb))
+    g = torch.Generator().manual_seed(7)
+    dtype = torch.float64 if prec == C.PREC_FP64 else torch.float32
+    A = torch.randn((bt, K, M) if ta else (bt, M, K), generator=g, dtype=dtype)
+    B = torch.randn((bt, N, K) if tb else (bt, K, N), generator=g, dtype=dtype)
+    Ao = (A.transpose(1, 2) if ta else A).double()
+    Bo = (B.transpose(1, 2) if tb else B).double()
+    ref = Ao @ Bo
+    bound = Ao.abs() @ Bo.abs()
+    out = S.gemm(A.to(DEV), B.to(DEV), prec=prec, transA=ta, transB=tb).cpu().double()
+    err = ((out - ref).abs() / bound.clamp_min(1e-30)).max().item()
+    assert err < GEMM_TOL[prec], err
+
+
+FIELD_CASES = [(1, 0, 0), (0, 3, 1), (1, 1, 0), (0, 2, 1)]
+
+
+@pytest.mark.parametrize("prec", [O.F32, O.F64])
+@pytest.mark.parametrize("imodel,ispecies,imap", FIELD_CASES)
+def test_eproj_sites_vs_oracle(prec, imodel, ispecies, imap):
+    xyz, box, _ = frames(128, 3, 1001)
+    announce("eproj_sites prec=%d model=%d species=%d map=%d" % (prec, imodel, ispecies, imap))
+    cfg = O.config(imodel=imodel, ispecies=ispecies, imap=imap, cutoff=0.7831)
+    dtype = torch.float64 if prec == O.F64 else torch.float32
+    sg = S.eproj_sites(torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV), imodel=imodel,
+                       ispecies=ispecies, imap=imap, cutoff=0.7831, dtype=dtype).cpu().double().numpy()
+    for f in range(xyz.shape[0]):
+        e = O.eproj(prec, cfg, xyz[f], box[f])
+        so = O.sites(prec, cfg, xyz[f], box[f], e)
+        # E in a.u. (|E| <~ 0.1): FP32 sums of ~200 Coulomb terms, compensated on the GPU
+        tol_e = 1e-12 if prec == O.F64 else 2e-6
+        assert np.abs(sg[f, C.SITE_E] - e).max() < tol_e
+        tol = 1e-10 if prec == O.F64 else 2e-5
+        for fld in range(O.SITE_A + 6):
+            a, b = sg[f, fld], so[:, fld]
+            assert np.abs(a - b).max() <= tol * max(1.0, np.abs(b).max()), (f, fld)
+
+
+@pytest.mark.parametrize("prec", [O.F32, O.F64])
+@pytest.mark.parametrize("imodel,ispecies,imap", FIELD_CASES)
+def test_hamiltonian_vs_oracle(prec, imodel, ispecies, imap):
+    xyz, box, _ = frames(128, 2, 1001)
+    announce("hamiltonian prec=%d model=%d species=%d map=%d" % (prec, imodel, ispecies, imap))
+    cfg = O.config(imodel=imodel, ispecies=ispecies, imap=imap, avef=3400.0)
+    dtype = torch.float64 if prec == O.F64 else torch.float32
+    x_d, b_d = torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV)
+    sg = S.eproj_sites(x_d, b_d, imodel=imodel, ispecies=ispecies, imap=imap, dtype=dtype)
+    Hg = S.hamiltonian(sg, b_d, ispecies=ispecies, imap=imap, avef=3400.0, ld=260).cpu().double().numpy()
+    n = sg.shape[2]
+    assert np.all(Hg[:, :, n:] == 0.0)  # padding rows of the leading dimension
+    for f in range(xyz.shape[0]):
+        e = O.eproj(prec, cfg, xyz[f], box[f])
+        Ho = O.hamiltonian(prec, cfg, xyz[f], box[f], e)
+        Hf = Hg[f, :, :n].T  # column-major -> row-major element (i, j)
+        assert np.array_equal(Hf, Hf.T)  # symmetric bit for bit
+        tol = 1e-9 if prec == O.F64 else 3e-3  # cm^-1 (|H| ~ 1e3 on the diagonal)
+        assert np.abs(Hf - Ho).max() < tol
+        if ispecies in (1, 2):
+            assert np.count_nonzero(Hf - np.diag(np.diag(Hf))) == 0
+
+
+def _sym_cases(rng):
+    cases = []
+    for n in (1, 2, 3, 37, 100, 255):
+        a = rng.standard_normal((n, n))
+        cases.append(("random%d" % n, (a + a.T) * 50.0))
+    d = rng.uniform(-500, 500, 64)
+    cases.append(("diagonal64", np.diag(d)))
+    q, _ = np.linalg.qr(rng.standard_normal((48, 48)))
+    lam = np.repeat([-100.0, 3.0, 250.0], 16)
+    cases.append(("degenerate48", (q * lam) @ q.T))
+    d2 = np.repeat(rng.uniform(-300, 300, 16), 2)  # exactly repeated diagonal values
+    cases.append(("diag_repeated32", np.diag(d2)))
+    return cases
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_eigh_small_cases(dtype):
+    rng = np.random.default_rng(11)
+    for name, Hn in _sym_cases(rng):
+        n = Hn.shape[0]
+        ld = n + 3
+        announce("eigh %s n=%d ld=%d dtype=%s" % (name, n, ld, dtype))
+        Hd = np.zeros((1, n, ld))
+        Hd[0, :, :n] = Hn.T
+        Hc = Hn.astype(np.float32).astype(np.float64) if dtype == torch.float32 else Hn
+        w, V = S.eigh(torch.from_numpy(Hd).to(dtype).to(DEV).contiguous(), n)
+        w = w.cpu().numpy()[0]
+        V = V.cpu().double().numpy()[0][:, :n]
+        wn = np.linalg.eigvalsh(Hc)
+        scale = max(np.abs(wn).max(), 1.0)
+        eps = 6e-8 if dtype == torch.float32 else 2.2e-16
+        assert np.all(np.diff(w) >= 0), name
+        assert np.abs(w - wn).max() <= 50 * eps * scale * max(1, np.sqrt(n)), name
+        res = np.abs(Hc @ V - V * w).max() / scale
+        orth = np.abs(V.T @ V - np.eye(n)).max()
+        assert res < 50 * eps * max(1, np.sqrt(n)), (name, res)
+        assert orth < 50 * eps * max(1, np.sqrt(n)), (name, orth)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_eigh_hamiltonians_vs_oracle(dtype):
+    xyz, box, _ = frames(128, 4, 1001)
+    announce("eigh of 4 c1 Hamiltonians dtype=%s" % dtype)
+    x_d, b_d = torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV)
+    sg = S.eproj_sites(x_d, b_d, dtype=dtype)
+    H = S.hamiltonian(sg, b_d, avef=3400.0)
+    Hh = H.cpu().double().numpy()
+    w, V = S.eigh(H)
+    w = w.cpu().numpy()
+    V = V.cpu().double().numpy()
+    dt = 0.010 / 5.308837367
+    for f in range(4):
+        Hf = Hh[f].T
+        wn, Vn = np.linalg.eigh(Hf)
+        eps = 6e-8 if dtype == torch.float32 else 2.2e-16
+        assert np.abs(w[f] - wn).max() < 200 * eps * np.abs(wn).max()
+        # basis-invariant: the propagator V e^{i w dt} V^T
+        U = (V[f] * np.exp(1j * w[f] * dt)) @ V[f].T
+        Un = (Vn * np.exp(1j * wn * dt)) @ Vn.T
+        assert np.abs(U - Un).max() < (2e-5 if dtype == torch.float32 else 1e-11)
+
+
+def test_eigh_c2_size_invariants():
+    xyz, box, _ = frames(1024, 1, 1002)
+    announce("eigh N=2048 single Hamiltonian")
+    x_d, b_d = torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV)
+    sg = S.eproj_sites(x_d, b_d)
+    H = S.hamiltonian(sg, b_d, avef=3400.0)
+    Hh = H[0].cpu().double().numpy().T
+    w, V = S.eigh(H)
+    w = w[0].cpu().numpy()
+    V = V[0].cpu().double().numpy()
+    wn = np.linalg.eigvalsh(Hh)
+    assert np.abs(w - wn).max() < 5e-3
+    assert np.abs(V.T @ V - np.eye(2048)).max() < 5e-5
+    assert np.abs(Hh @ V - V * w).max() < 5e-3
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_ipr_hist_and_sw_vs_oracle(dtype):
+    xyz, box, _ = frames(128, 3, 1001)
+    announce("ipr_hist + spectral density dtype=%s" % dtype)
+    cfg = O.config(avef=3400.0, omegaStart=2800.0, omegaStop=4000.0, omegaStep=5, t1=0.26)
+    nom = O.lib().orc_nomega(cfg)
+    x_d, b_d = torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV)
+    sg = S.eproj_sites(x_d, b_d, dtype=dtype)
+    w, V = S.eigh(S.hamiltonian(sg, b_d, avef=3400.0))
+    weight = torch.tensor([1.0, 3.0, 0.0], dtype=torch.float64, device=DEV)
+    acc, skipped = S.ipr_hist(V, w, nom, 3400.0, 2800.0, 5, weight=weight)
+    sw = S.spectral_density(V, w, sg, nom, 0.26, 3400.0, 2800.0, 5, weight=weight)
+    acc, sw = acc.cpu().numpy(), sw.cpu().numpy()
+    wh, Vh, sh = w.cpu().numpy(), V.cpu().double().numpy(), sg.cpu().double().numpy()
+    prec = O.F64 if dtype == torch.float64 else O.F32
+    pw = np.zeros(nom)
+    rw = np.zeros(nom)
+    rmw = np.zeros(nom)
+    swo = np.zeros(nom)
+    sk = 0
+    for f, wt in enumerate([1.0, 3.0, 0.0]):
+        if wt == 0:
+            continue
+        p_, r_, m_, s_ = O.ipr_hist(prec, cfg, wh[f], Vh[f], wt)
+        pw += p_
+        rw += r_
+        rmw += m_
+        sk += s_ * wt
+        swo += wt * O.spectral_density(prec, cfg, wh[f], Vh[f], sh[f, C.SITE_MU:C.SITE_MU + 3].ravel())
+    assert skipped.item() == sk
+    assert np.array_equal(acc[0], pw)
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    assert rel(acc[1], rw) < tol and rel(acc[2], rmw) < tol
+    assert rel(sw, swo) < tol
+
+
+@pytest.mark.parametrize("prec", [C.PREC_FP32, C.PREC_BF16X3, C.PREC_FP64])
+def test_propagate_and_correlate(prec):
+    xyz, box, _ = frames(128, 2, 1001)
+    announce("propagate + correlate prec=%d" % prec)
+    dtype = torch.float64 if prec == C.PREC_FP64 else torch.float32
+    x_d, b_d = torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV)
+    sg = S.eproj_sites(x_d, b_d, dtype=dtype)
+    w, V = S.eigh(S.hamiltonian(sg, b_d, avef=3400.0))
+    n = V.shape[1]
+    rng = np.random.default_rng(5)
+    nslot = 3
+    ncols = 9 * nslot
+    Y0 = rng.standard_normal((n, 2 * ncols))
+    Y = torch.from_numpy(Y0).to(dtype).to(DEV).contiguous()
+    S.propagate(V[1], w[1], Y, ncols, 0.010, prec=prec)
+    Vh, wh = V[1].cpu().double().numpy(), w[1].cpu().numpy()
+    Yc = Y0[:, :ncols] + 1j * Y0[:, ncols:]
+    dt = float(np.float32(0.010))  # dt is a float parameter, as in the reference
+    ref = (Vh * np.exp(1j * wh * dt / 5.308837367)) @ (Vh.T @ Yc)
+    got = Y.cpu().double().numpy()
+    gotc = got[:, :ncols] + 1j * got[:, ncols:]
+    tol = {C.PREC_FP32: 2e-5, C.PREC_BF16X3: 1e-4, C.PREC_FP64: 1e-12}[prec]
+    assert rel(gotc, ref) < tol
+    # correlations M_b(t)^T Y_a for the 15 pairs of calcIR.cu:628-733
+    corr = S.correlate(sg[1], Y, nslot=nslot).numpy()
+    M = sg[1, C.SITE_MU:C.SITE_MU + 9].cpu().double().numpy()
+    pairs_ir = [(0, 0), (1, 1), (2, 2)]
+    iiii = [(3, 3), (4, 4), (5, 5)]
+    ijij = [(6, 6), (7, 7), (8, 8)]
+    iijj = [(4, 3), (5, 3), (3, 4), (5, 4), (3, 5), (4, 5)]
+    for s in range(nslot):
+        Ys = gotc[:, 9 * s:9 * s + 9]
+        f = lambda pr: sum(M[b] @ Ys[:, a] for b, a in pr)
+        ir, a1, a2, a3 = f(pairs_ir), f(iiii), f(ijij), f(iijj)
+        vv = (3 * a1 + a3 + 4 * a2) / 15
+        vh = (2 * a1 - a3 + 6 * a2) / 30
+        np.testing.assert_allclose(corr[s], [ir, vv, vh], rtol=1e-9 if prec == C.PREC_FP64 else 1e-5,
+                                   atol=1e-12)
+
+
+@pytest.mark.parametrize("nfft,nx", [(2048, 150), (25750, 150), (9, 5), (10, 6), (8, 8), (1, 1)])
+def test_c2r_vs_oracle_and_numpy(nfft, nx):
+    announce("c2r nfft=%d nx=%d" % (nfft, nx))
+    rng = np.random.default_rng(nfft + nx)
+    X = rng.standard_normal((3, nx)) + 1j * rng.standard_normal((3, nx))
+    y = S.c2r(torch.from_numpy(X).to(DEV), nfft).cpu().numpy()
+    for s in range(3):
+        yo = O.c2r(nfft, X[s])
+        assert np.abs(y[s] - yo).max() < 1e-9 * max(1.0, np.abs(yo).max())
+        Xf = np.zeros(nfft // 2 + 1, np.complex128)
+        m = min(nx, nfft // 2 + 1)
+        Xf[:m] = X[s, :m]
+        yn = np.fft.irfft(Xf, nfft, norm="forward")
+        assert np.abs(y[s] - yn).max() < 1e-9 * max(1.0, np.abs(yn).max())

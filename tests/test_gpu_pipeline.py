@@ -1,0 +1,106 @@
+"""End-to-end parity of the whole calcIR job on the GPU (cir_run through the
+C ABI) against the CPU oracle's reference control flow."""
+
+import numpy as np
+import pytest
+
+import calcir_amd as C
+import oracle_lib as O
+from cases import frames, rel, rel_l2
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+# Small case with overlapping time origins (3 samples of 30 points, 10 frames
+# apart), 128 waters, tip4p2005 model, H2O, Gruenbaum map.
+KW = dict(species="H2O", model="tip4p2005", ntcfpoints=30, nsamples=3, sampleEvery=0.1, avef=3400.0,
+          omegaStart=2800.0, omegaStop=4000.0, omegaStep=5, nzeros=2018, t1=0.26, dt=0.010)
+
+
+def _run_pair(prec, mode, oracle_prec=O.F64, nwater=128, nframes=60, seed=1001, **over):
+    kw = dict(KW)
+    kw.update(over)
+    p = C.default_params(precision=prec, mode=mode, **kw)
+    xyz, box, times = frames(nwater, nframes, seed)
+    announce("pipeline prec=%s mode=%s nwater=%d nframes=%d %s" % (prec, mode, nwater, nframes, over))
+    g = C.run(p, xyz, box, times)
+    o = O.run(oracle_prec, O.config_from_params(p), xyz, box, times)
+    return p, g, o
+
+
+def _check(g, o, tol_tcf, tol_spec, tol_dist):
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        assert rel(g[k], o[k]) < tol_tcf, (k, rel(g[k], o[k]))
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
+        assert rel_l2(g[k], o[k]) < tol_spec, (k, rel_l2(g[k], o[k]))
+    assert rel(g["sw"], o["sw"]) < tol_dist
+    assert g["hist_skipped"] == o["hist_skipped"]
+    # Pw is a count; a mode whose eigenvalue sits within rounding of a bin
+    # edge may move by one bin between precisions
+    assert np.abs(g["pw"] - o["pw"]).sum() <= 2 * 0.2 / (256 * 3 * 30) * 2 + 1e-12
+    m = o["pw"] > 0
+    assert np.all(np.isnan(g["rw"][~m]))
+    assert rel(g["rw"][m], o["rw"][m]) < max(tol_dist, 1e-9) * 1e2
+    assert rel(g["rmw"][m], o["rmw"][m]) < max(tol_dist, 1e-9) * 1e2
+
+
+def test_pipeline_fp64_batched_matches_oracle_fp64():
+    p, g, o = _run_pair("fp64", "batched")
+    assert g["nsteps"] == 90 and g["nframes_used"] == 50
+    _check(g, o, 1e-10, 1e-10, 1e-10)
+
+
+def test_pipeline_fp64_plain_matches_oracle_fp64():
+    p, g, o = _run_pair("fp64", "plain")
+    assert g["nsteps"] == 90 and g["nframes_used"] == 90
+    _check(g, o, 1e-10, 1e-10, 1e-10)
+
+
+def test_pipeline_fp32_batched_within_1e4_of_oracle():
+    # FP32 path against the reference algorithm: 1e-4 relative (TCFs, spectra)
+    p, g, o = _run_pair("fp32", "batched")
+    _check(g, o, 1e-4, 1e-4, 1e-4)
+    _, _, o32 = _run_pair("fp32", "batched", oracle_prec=O.F32)
+    _check(g, o32, 1e-4, 1e-4, 1e-4)
+
+
+def test_pipeline_fp32_plain_within_1e4_of_oracle():
+    p, g, o = _run_pair("fp32", "plain")
+    _check(g, o, 1e-4, 1e-4, 1e-4)
+
+
+def test_pipeline_bf16x3_within_1e3_of_fp64():
+    p, g, o = _run_pair("bf16x3", "batched")
+    _check(g, o, 1e-3, 1e-3, 1e-4)
+
+
+@pytest.mark.parametrize("species,model,imap", [("HOD/D2O", "tip4p", 0), ("D2O", "e3b3", 1),
+                                                ("HOD/H2O", "e3b2", 1)])
+def test_pipeline_models_species_maps(species, model, imap):
+    p, g, o = _run_pair("fp64", "batched", species=species, model=model, map=imap, omegaStart=2000.0,
+                        omegaStop=4200.0)
+    _check(g, o, 1e-10, 1e-10, 1e-10)
+
+
+def test_pipeline_uncoupled_closed_form():
+    # HOD species: H is diagonal, F(t_k) = diag(exp(+i sum_{j=1..k} (w_i(t_j)-avef) dt/hbar))
+    # (right endpoint), so C_IR(t_k) = sum_i mu_i(t_k).mu_i(0) exp(i phi_i(k)).
+    p, g, _ = _run_pair("fp64", "batched", species="HOD/D2O", nsamples=1, ntcfpoints=12)
+    xyz, box, _ = frames(128, 60, 1001)
+    cfg = O.config_from_params(p)
+    phase = 0.0
+    ref = []
+    mu0 = None
+    for k in range(12):
+        e = O.eproj(O.F64, cfg, xyz[k], box[k])
+        s = O.sites(O.F64, cfg, xyz[k], box[k], e)
+        mu = s[:, O.SITE_MU:O.SITE_MU + 3]
+        if k == 0:
+            mu0 = mu
+        else:
+            phase = phase + (s[:, O.SITE_W] - p.avef) * p.dt / 5.308837367
+        ref.append(np.sum(np.sum(mu * mu0, axis=1) * np.exp(1j * phase)))
+    ref = np.array(ref) * np.exp(-np.arange(12) * p.dt / (2 * p.t1))
+    assert rel(g["tcf_ir"], ref) < 1e-10
+    # C_IR(0) = sum |mu|^2 (real)
+    assert abs(g["tcf_ir"][0].imag) < 1e-12
