@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Frames/s of the calcIR spectroscopy hot path on one MI355X.
+
+Workload (the 2048-OH one-box configuration): 1024 rigid TIP4P/2005 waters
+(2048 OH chromophores) in a cubic box L = 3.128 nm, 2000 seeded OU frames
+10 fs apart, O-H field cutoff 0.7831 nm, 150-point correlation functions
+(1.5 ps), time origins every 20 frames (93 samples), T1 = 260 fs, IR + VV/VH
+Raman, omega 2800-4000 cm^-1, C2R to 2048 points.
+
+One bench step is the whole job over that trajectory (every sample, every
+(origin, tcf point) step, lineshapes) with the trajectory resident in HBM.
+"frames/s" is (origin, tcf point) steps per second (nsamples * ntcfpoints /
+wall time), the trip count of the reference's sample x frame loop.
+
+Timed: the batched mixed-precision path (each trajectory frame's field,
+Hamiltonian and eigensolve computed once; all active origins propagated
+together with split-BF16x3 MFMA products).  Reported beside it: the batched
+FP32 path, the plain FP32 path with the reference control flow (timed on a
+bounded subset), the spectrum error of each form against the FP64 pipeline,
+the roofline of the dominant kernel and the CPU oracle baseline.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s at 2048 OH: mixed-prec batched vs plain FP32, ≤1e-3 rel spectrum err"
+NWATER, NFRAMES, SEED, BOX = 1024, 2000, 1002, 3.128
+WORK = dict(species="H2O", model="tip4p2005", map=0, ntcfpoints=150, sampleEvery=0.2, avef=3400.0,
+            omegaStart=2800.0, omegaStop=4000.0, omegaStep=5, nzeros=1898, t1=0.26, dt=0.010,
+            cutoff=0.7831)
+HBM_PEAK_GBS = 8000.0
+
+
+def nsamples_for(nframes):
+    # origins every 20 frames, each needing 150 frames
+    return (nframes - 150) // 20 + 1
+
+
+def tridiag_bytes(n, esz=4, nb=32):
+    """HBM bytes per matrix of the one-stage panel-blocked tridiagonalisation:
+    each reflector's symmetric product streams the trailing (n-j-1)^2 block
+    once; each panel's trailing update reads and writes the remaining block."""
+    m = np.arange(n - 1, 0, -1, dtype=np.float64)
+    sym = float(np.sum(m * m)) * esz
+    upd = 0.0
+    for j0 in range(0, n - 1, nb):
+        r = n - min(j0 + nb, n - 1)
+        upd += 2.0 * r * r * esz
+    return sym + upd
+
+
+def lineshape(p, ft):
+    """(freq, value) rows as written to <outf>_irls.dat (calcIR.cu:893-919)."""
+    nt, nz = p.ntcfpoints, p.nzeros
+    n = nt + nz
+    factor = np.float32(2 * 3.14159265359 * 5.308837367 / (np.float32(p.dt) * np.float32(n)))
+    fr, va = [], []
+    for i in range(n // 2, n):
+        f = np.float32(-(i - nt - nz)) * factor + np.float32(p.avef)
+        if f <= p.omegaStop:
+            fr.append(f)
+            va.append(ft[i] / (factor * n))
+    for i in range(0, nt + nz // 2):
+        f = np.float32(-i) * factor + np.float32(p.avef)
+        if f >= p.omegaStart:
+            fr.append(f)
+            va.append(ft[i] / (factor * n))
+    return np.array(fr, np.float64), np.array(va, np.float64)
+
+
+def spectrum_error(p, res, ref):
+    out = {}
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
+        f, a = lineshape(p, res[k])
+        _, b = lineshape(p, ref[k])
+        l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        m1a = float(np.sum(f * a) / np.sum(a))
+        m1b = float(np.sum(f * b) / np.sum(b))
+        out[k[5:]] = {"rel_l2": l2, "first_moment_shift_cm-1": m1a - m1b}
+    return out
+
+
+def cpu_baseline(xyz, box, times, nthreads):
+    """The CPU oracle (FP32, reference-literal three-complex-product
+    propagation) on a bounded sample of the workload: one origin x 3 tcf
+    points (one F = I step and two propagated steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    cfg = O.config(imodel=1, ispecies=0, imap=0, cutoff=WORK["cutoff"], avef=WORK["avef"], dt=WORK["dt"],
+                   t1=WORK["t1"], ntcfpoints=3, nsamples=1, nzeros=WORK["nzeros"], sampleEvery=0.2,
+                   omegaStart=WORK["omegaStart"], omegaStop=WORK["omegaStop"], omegaStep=5,
+                   propagation=O.PROP_LITERAL, nthreads=nthreads)
+    t0 = time.perf_counter()
+    r = O.run(O.F32, cfg, xyz[:3], box[:3], times[:3])
+    dt = time.perf_counter() - t0
+    return {"value": r["nsteps"] / dt, "unit": "frames/s", "cores": nthreads, "kind": "port",
+            "sample": "2048-OH box, 1 origin x 3 tcf points (1 F=I step + 2 propagated steps), FP32 oracle, "
+                      "literal F-matrix propagation (3 complex products per step), %.1f s" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=NFRAMES, help="trajectory frames (default 2000)")
+    ap.add_argument("--plain-points", type=int, default=4, help="tcf points of the plain-FP32 subset")
+    ap.add_argument("--no-extras", action="store_true", help="only the timed path (for profiler runs)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+    import torch
+
+    import calcir_amd as C
+
+    dev = torch.device("cuda:0")
+    nframes = args.frames
+    ns = nsamples_for(nframes)
+    xyz, box, times = C.synth_trajectory(NWATER, nframes, SEED, box_nm=BOX)
+    xyz_d = torch.from_numpy(xyz).to(dev)
+    box_d = torch.from_numpy(box).to(dev)
+    torch.cuda.synchronize()
+
+    p = C.default_params(precision="bf16x3", mode="batched", nsamples=ns, **WORK)
+    print("[bench] warmup %d" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        C.run_device(p, xyz_d, box_d, times)
+    torch.cuda.synchronize()
+    phases = {k: [0.0, 0] for k in C.PHASES}
+    t0 = time.perf_counter()
+    res = None
+    for i in range(args.steps):
+        res = C.run_device(p, xyz_d, box_d, times, profile=True)
+        for k, (s, c) in res["phases"].items():
+            phases[k][0] += s
+            phases[k][1] += c
+        print("[bench] step %d done" % i, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    steps_per_run = res["nsteps"]
+    value = steps_per_run * args.steps / elapsed
+
+    n = 2 * NWATER
+    nmat = res["nframes_used"] * args.steps
+    tri_s, tri_n = phases["tridiag"]
+    per_launch = tri_s / max(tri_n, 1)
+    bytes_launch = tridiag_bytes(n) * nmat / max(tri_n, 1)
+    achieved = bytes_launch / per_launch / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2048-OH box: 1024 TIP4P/2005 waters, L=3.128 nm, %d seeded OU frames at 10 fs, "
+                               "cutoff 0.7831 nm, 150-point TCF, origins every 20 frames (%d samples)" % (nframes, ns),
+                   "path": "batched", "precision": "f32 field/H/tridiagonalisation, f64 bisection + inverse "
+                                                   "iteration, split-bf16x3 MFMA propagator",
+                   "nsteps_per_run": steps_per_run, "distinct_frames": res["nframes_used"]},
+        "roofline": {"kernel": "tridiag_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": 1e3 * per_launch, "matrices_per_launch": nmat / max(tri_n, 1),
+                     "bytes_per_matrix": tridiag_bytes(n)},
+        "phases_s_per_run": {k: v[0] / args.steps for k, v in phases.items()},
+    }
+    if not args.no_extras:
+        print("[bench] fp32 batched", file=sys.stderr, flush=True)
+        p32 = C.default_params(precision="fp32", mode="batched", nsamples=ns, **WORK)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r32 = C.run_device(p32, xyz_d, box_d, times)
+        torch.cuda.synchronize()
+        t32 = time.perf_counter() - t1
+        print("[bench] fp64 batched (error reference)", file=sys.stderr, flush=True)
+        p64 = C.default_params(precision="fp64", mode="batched", nsamples=ns, **WORK)
+        t1 = time.perf_counter()
+        r64 = C.run_device(p64, xyz_d, box_d, times)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t1
+        print("[bench] plain fp32 subset", file=sys.stderr, flush=True)
+        pp = C.default_params(precision="fp32", mode="plain", **dict(WORK, ntcfpoints=args.plain_points, nsamples=1))
+        C.run_device(pp, xyz_d, box_d, times)  # warm
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rp = C.run_device(pp, xyz_d, box_d, times)
+        torch.cuda.synchronize()
+        tplain = time.perf_counter() - t1
+        plain_fs = rp["nsteps"] / tplain
+        out["batched_fp32"] = {"value": steps_per_run / t32, "unit": "frames/s",
+                               "error_vs_fp64": spectrum_error(p32, r32, r64)}
+        out["batched_bf16x3_error_vs_fp64"] = spectrum_error(p, res, r64)
+        out["batched_fp64"] = {"value": steps_per_run / t64, "unit": "frames/s"}
+        out["plain_fp32"] = {"value": plain_fs, "unit": "frames/s",
+                             "sample": "1 origin x %d tcf points, reference control flow" % args.plain_points}
+        out["speedup_vs_plain_fp32"] = value / plain_fs
+        out["speedup_vs_batched_fp32"] = value * t32 / steps_per_run
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(xyz, box, times, min(16, os.cpu_count() or 1))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -18,7 +18,7 @@ import numpy as np
 
 __all__ = [
     "CalcirError", "Params", "Results", "lib", "default_params", "read_params", "write_outputs",
-    "synth_box_nm", "synth_trajectory", "run", "nomega", "resolve_model", "resolve_species",
+    "synth_box_nm", "synth_trajectory", "run", "run_device", "nomega", "resolve_model", "resolve_species",
     "frame_index", "PREC_FP32", "PREC_FP64", "PREC_BF16X3", "MODE_PLAIN", "MODE_BATCHED", "NSITE",
     "SITE_E", "SITE_W", "SITE_X", "SITE_P", "SITE_MUP", "SITE_U", "SITE_R", "SITE_MU", "SITE_A",
 ]
@@ -28,6 +28,8 @@ LIB_PATH = os.path.join(HERE, "libcalcir.so")
 CLI_PATH = os.path.join(HERE, "bin", "calcir")
 
 PREC_FP32, PREC_FP64, PREC_BF16X3 = 0, 1, 2
+PHASES = ("sites", "hamiltonian", "tridiag", "bisect", "invit", "backtransform", "ipr", "sw", "propagate",
+          "correlate", "finalize")
 MODE_PLAIN, MODE_BATCHED = 0, 1
 NSITE = 20
 SITE_E, SITE_W, SITE_X, SITE_P, SITE_MUP, SITE_U, SITE_R, SITE_MU, SITE_A = 0, 1, 2, 3, 4, 5, 8, 11, 14
@@ -101,6 +103,8 @@ SIGNATURES = {
     "cir_run": (_i, [_vp, _i, _vp, _vp, _vp]),
     "cir_run_device": (_i, [_vp, _i, _vp, _vp, _vp]),
     "cir_get_results": (_i, [_vp, ctypes.POINTER(Results)]),
+    "cir_set_profiling": (_i, [_vp, _i]),
+    "cir_phase_times": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll), _i]),
     "cir_destroy": (None, [_vp]),
     "cir_eproj_sites": (_i, [_i, _i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "cir_hamiltonian": (_i, [_i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
@@ -222,7 +226,7 @@ def _results_buffers(p):
     return r, bufs
 
 
-def run(p, xyz, box, times, stream=None):
+def run(p, xyz, box, times, stream=None, profile=False):
     """The whole calcIR job (every sample, calcIR.cu:360-919) on one GPU.
 
     xyz: [nframes, natoms, 3] float32 nm (host numpy); box [nframes, 3];
@@ -235,10 +239,18 @@ def run(p, xyz, box, times, stream=None):
     L = lib()
     ctx = ctypes.c_void_p()
     check(L.cir_create(ctypes.byref(p), natoms, stream, ctypes.byref(ctx)))
+    phases = None
     try:
+        if profile:
+            check(L.cir_set_profiling(ctx, 1))
         check(L.cir_run(ctx, nframes, xyz.ctypes.data, box.ctypes.data, times.ctypes.data))
         r, bufs = _results_buffers(p)
         check(L.cir_get_results(ctx, ctypes.byref(r)))
+        if profile:
+            secs = (ctypes.c_double * len(PHASES))()
+            cnts = (ctypes.c_longlong * len(PHASES))()
+            check(L.cir_phase_times(ctx, secs, cnts, len(PHASES)))
+            phases = {name: (secs[i], cnts[i]) for i, name in enumerate(PHASES)}
     finally:
         L.cir_destroy(ctx)
     out = {}
@@ -247,7 +259,50 @@ def run(p, xyz, box, times, stream=None):
     for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw", "rw", "rmw"):
         out[k] = bufs[k]
     out.update(nsteps=r.nsteps, nframes_used=r.nframes_used, hist_skipped=r.hist_skipped,
-               seconds=r.seconds)
+               seconds=r.seconds, phases=phases)
+    return out
+
+
+def run_device(p, xyz, box, times, profile=False):
+    """run() with the trajectory already resident on the device: xyz, box are
+    cuda float32 torch tensors [nframes, natoms, 3] / [nframes, 3]; times a
+    host array.  Runs on torch's current stream."""
+    import torch
+
+    if not (xyz.is_cuda and box.is_cuda and xyz.dtype == torch.float32 and box.dtype == torch.float32):
+        raise ValueError("xyz and box must be cuda float32 tensors")
+    if xyz.dim() != 3 or xyz.shape[2] != 3 or tuple(box.shape) != (xyz.shape[0], 3):
+        raise ValueError("xyz must be [nframes, natoms, 3] and box [nframes, 3]")
+    xyz, box = xyz.contiguous(), box.contiguous()
+    times = np.ascontiguousarray(times, np.float32)
+    if times.size != xyz.shape[0]:
+        raise ValueError("times must have one entry per frame")
+    L = lib()
+    ctx = ctypes.c_void_p()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(L.cir_create(ctypes.byref(p), xyz.shape[1], stream, ctypes.byref(ctx)))
+    phases = None
+    try:
+        if profile:
+            check(L.cir_set_profiling(ctx, 1))
+        check(L.cir_run_device(ctx, xyz.shape[0], ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(box.data_ptr()),
+                               times.ctypes.data))
+        r, bufs = _results_buffers(p)
+        check(L.cir_get_results(ctx, ctypes.byref(r)))
+        if profile:
+            secs = (ctypes.c_double * len(PHASES))()
+            cnts = (ctypes.c_longlong * len(PHASES))()
+            check(L.cir_phase_times(ctx, secs, cnts, len(PHASES)))
+            phases = {name: (secs[i], cnts[i]) for i, name in enumerate(PHASES)}
+    finally:
+        L.cir_destroy(ctx)
+    out = {}
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        out[k] = bufs[k][0::2] + 1j * bufs[k][1::2]
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw", "rw", "rmw"):
+        out[k] = bufs[k]
+    out.update(nsteps=r.nsteps, nframes_used=r.nframes_used, hist_skipped=r.hist_skipped,
+               seconds=r.seconds, phases=phases)
     return out
 
 

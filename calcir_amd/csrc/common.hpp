@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "calcir.h"
 
@@ -52,6 +53,52 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     } while (0)
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Per-phase device timing with HIP events recorded on the stream the phase's
+// kernels are launched on.  Inactive (on == false) it records nothing.
+struct Prof {
+    bool on = false;
+    struct Rec {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    int begin(int phase, hipStream_t s) {
+        if (!on) return -1;
+        Rec r{phase, nullptr, nullptr};
+        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+        (void)hipEventRecord(r.a, s);
+        recs.push_back(r);
+        return int(recs.size()) - 1;
+    }
+    void end(int idx, hipStream_t s) {
+        if (idx >= 0) (void)hipEventRecord(recs[idx].b, s);
+    }
+    // Adds the elapsed time of every record into secs[phase] / counts[phase].
+    void collect(double* secs, long long* counts, int nphase) {
+        for (Rec& r : recs) {
+            float ms = 0.f;
+            if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess &&
+                r.phase >= 0 && r.phase < nphase) {
+                secs[r.phase] += ms * 1e-3;
+                counts[r.phase] += 1;
+            }
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        recs.clear();
+    }
+};
+
+struct ProfScope {
+    Prof* p;
+    int idx;
+    hipStream_t s;
+    ProfScope(Prof* prof, int phase, hipStream_t st) : p(prof), idx(prof ? prof->begin(phase, st) : -1), s(st) {}
+    ~ProfScope() {
+        if (p) p->end(idx, s);
+    }
+};
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

@@ -12,9 +12,13 @@
 //   2. bisect_kernel: every eigenvalue of T by FP64 Sturm-sequence
 //      bisection, one lane per eigenvalue (eigenvalues come out ascending).
 //   3. invit_kernel: eigenvectors of T by FP64 inverse iteration with
-//      partial pivoting, one lane per eigenvalue; eigenvalues closer than
-//      1e-7*||T|| form a cluster that its first lane solves sequentially with
-//      modified Gram-Schmidt against the cluster's earlier vectors.
+//      partial pivoting, one lane per eigenvalue; neighbours closer than
+//      1e-7 * max(|lambda|, 1) form a cluster (at most kMaxCluster members)
+//      that its first lane solves sequentially with modified Gram-Schmidt
+//      against the cluster's earlier vectors.  Both this tolerance and the
+//      bisection tolerance are relative to the eigenvalue itself, so a few
+//      outlying diagonal entries (strong-field chromophores) neither merge the
+//      bulk of the spectrum into one cluster nor coarsen its eigenvalues.
 //   4. back-transform V = Q Z in compact-WY blocks of kBtNB reflectors on
 //      the MFMA product (gemm.hpp): Z <- Z - Y (T (Y^T Z)), last block first.
 
@@ -31,6 +35,42 @@ constexpr int kTriRQ = 8;    // rows per thread per pass of the symmetric produc
 constexpr int kBtNB = 128;   // reflectors per compact-WY block in the back-transform
 constexpr int kEigNT = 256;  // threads per block of bisection / inverse iteration
 constexpr double kClusterRel = 1e-7;
+constexpr int kMaxCluster = 128;
+
+// Eigenvalue k joins the cluster of k-1 (ascending w).
+__device__ __forceinline__ bool joins_cluster(const double* w, int k) {
+    const double scale = fmax(fmax(fabs(w[k]), fabs(w[k - 1])), 1.0);
+    return w[k] - w[k - 1] <= kClusterRel * scale;
+}
+
+// 16-byte vector of T for the streaming loads of the symmetric product.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+template <typename V, typename T>
+__device__ __forceinline__ void vfma(T (&acc)[4], const V& a, T s);
+template <>
+__device__ __forceinline__ void vfma<float4, float>(float (&acc)[4], const float4& a, float s) {
+    acc[0] += a.x * s;
+    acc[1] += a.y * s;
+    acc[2] += a.z * s;
+    acc[3] += a.w * s;
+}
+template <>
+__device__ __forceinline__ void vfma<double2, double>(double (&acc)[4], const double2& a, double s) {
+    acc[0] += a.x * s;
+    acc[1] += a.y * s;
+}
+constexpr int kTriCU = 8;  // columns of the symmetric product in flight per lane
 
 template <typename T>
 __device__ T block_sum1(T v, T* sred) {
@@ -122,26 +162,33 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 for (int r = r0 + tid; r < n; r += kTriNT) Wp[size_t(i) * ld + r] = T(0);
                 continue;
             }
-            // p = A_panel(r0:, r0:) v
+            // p = A_panel(r0:, r0:) v: each lane streams 16-byte row vectors of
+            // kTriCU columns at a time (rows below r0 are read and discarded
+            // so that every load is aligned; ld is a multiple of 4)
             if (m >= kTriNT) {
-                for (int rb = r0; rb < n; rb += kTriNT * kTriRQ) {
-                    T acc[kTriRQ];
+                using VT = typename Vec16<T>::type;
+                constexpr int VW = Vec16<T>::n;
+                const int rstart = r0 & ~(VW - 1);
+                for (int rb = rstart; rb < n; rb += kTriNT * VW) {
+                    const int r = rb + tid * VW;
+                    const bool act = r < n;
+                    T acc[4] = {T(0), T(0), T(0), T(0)};
+                    int c = r0;
+                    for (; c + kTriCU <= n; c += kTriCU) {
+                        VT a[kTriCU];
 #pragma unroll
-                    for (int q = 0; q < kTriRQ; ++q) acc[q] = T(0);
-                    for (int c = r0; c < n; ++c) {
-                        const T vc = sv[c];
-                        const T* col = A + size_t(c) * ld;
+                        for (int u = 0; u < kTriCU; ++u)
+                            a[u] = act ? *reinterpret_cast<const VT*>(A + size_t(c + u) * ld + r) : VT{};
 #pragma unroll
-                        for (int q = 0; q < kTriRQ; ++q) {
-                            const int r = rb + tid + q * kTriNT;
-                            if (r < n) acc[q] += col[r] * vc;
-                        }
+                        for (int u = 0; u < kTriCU; ++u) vfma(acc, a[u], sv[c + u]);
+                    }
+                    for (; c < n; ++c) {
+                        const VT a = act ? *reinterpret_cast<const VT*>(A + size_t(c) * ld + r) : VT{};
+                        vfma(acc, a, sv[c]);
                     }
 #pragma unroll
-                    for (int q = 0; q < kTriRQ; ++q) {
-                        const int r = rb + tid + q * kTriNT;
-                        if (r < n) sp[r] = acc[q];
-                    }
+                    for (int q = 0; q < VW; ++q)
+                        if (act && r + q >= r0 && r + q < n) sp[r + q] = acc[q];
                 }
             } else {
                 const int rnd = (m + 63) & ~63;
@@ -164,14 +211,16 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 const bool isw = q < i;
                 const int l = isw ? q : q - i;
                 const T* col = (isw ? Wp : Vp) + size_t(l) * ld;
-                T a0 = T(0), a1 = T(0);
+                T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
                 int r = r0 + lane;
-                for (; r + 64 < n; r += 128) {
+                for (; r + 192 < n; r += 256) {
                     a0 += col[r] * sv[r];
                     a1 += col[r + 64] * sv[r + 64];
+                    a2 += col[r + 128] * sv[r + 128];
+                    a3 += col[r + 192] * sv[r + 192];
                 }
-                if (r < n) a0 += col[r] * sv[r];
-                const T tot = wave_sum(a0 + a1);
+                for (; r < n; r += 64) a0 += col[r] * sv[r];
+                const T tot = wave_sum((a0 + a1) + (a2 + a3));
                 if (lane == 0) sdot[isw ? l : kTriNB + l] = tot;
             }
             __syncthreads();
@@ -317,13 +366,14 @@ bisect_kernel(int n, const T* __restrict__ dall, const T* __restrict__ eall, dou
     }
     const double tnorm = fmax(fabs(gl), fabs(gu));
     const double pivmin = DBL_MIN * fmax(1.0, e2);
-    const double tol = 4.0 * DBL_EPSILON * tnorm + 2.0 * pivmin;
-    gl -= 2.0 * tol + 4.0 * pivmin;
-    gu += 2.0 * tol + 4.0 * pivmin;
+    const double widen = 4.0 * DBL_EPSILON * tnorm + 4.0 * pivmin;
+    gl -= widen;
+    gu += widen;
     const int j = blockIdx.x * kEigNT + tid;
     if (j >= n) return;
     double a = gl, c = gu;
-    for (int it = 0; it < 128 && (c - a) > tol; ++it) {
+    // relative tolerance (absolute floor 4 eps) on the bracket of eigenvalue j
+    for (int it = 0; it < 128 && (c - a) > 2.0 * DBL_EPSILON * (fabs(a) + fabs(c)) + 4.0 * DBL_EPSILON; ++it) {
         const double mid = 0.5 * (a + c);
         if (sturm_count(n, sd, se2, mid, pivmin) > j)
             c = mid;
@@ -364,17 +414,20 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
     const int c = blockIdx.x * kEigNT + tid;
     if (c >= n) return;
     const double tnorm = fmax(fmax(fabs(w[0]), fabs(w[n - 1])), DBL_MIN);
-    const double ortol = kClusterRel * tnorm;
     const double pivmin = DBL_EPSILON * tnorm;
-    if (c > 0 && w[c] - w[c - 1] <= ortol) return;  // solved by its cluster's first lane
+    // cluster start and this lane's position in it; every kMaxCluster-th
+    // member leads a group of the cluster
+    int cs = c;
+    while (cs > 0 && joins_cluster(w, cs)) --cs;
+    if ((c - cs) % kMaxCluster != 0) return;  // solved by its group's first lane
     int cend = c;
-    while (cend + 1 < n && w[cend + 1] - w[cend] <= ortol) ++cend;
+    while (cend + 1 < n && cend + 1 - c < kMaxCluster && joins_cluster(w, cend + 1)) ++cend;
 
     const size_t nn = size_t(n) * n;
     double lam_prev = 0.0;
     for (int mb = c; mb <= cend; ++mb) {
         double lam = w[mb];
-        if (mb > c) lam = fmax(lam, lam_prev + 10.0 * DBL_EPSILON * tnorm);
+        if (mb > c) lam = fmax(lam, lam_prev + 10.0 * DBL_EPSILON * fmax(fabs(lam_prev), 1.0));
         lam_prev = lam;
         double* U0 = S + mb;
         double* U1 = S + nn + mb;
@@ -528,7 +581,7 @@ inline EighWork eigh_layout(size_t esz, int batch, int n, int ld) {
 
 template <typename T>
 int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, unsigned char* work,
-              hipStream_t s) {
+              hipStream_t s, Prof* prof) {
     const EighWork L = eigh_layout(sizeof(T), batch, n, ld);
     T* dd = reinterpret_cast<T*>(work + L.off_d);
     T* ee = reinterpret_cast<T*>(work + L.off_e);
@@ -541,30 +594,40 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     double* scr = reinterpret_cast<double*>(work + L.off_scr);
 
     // 1. tridiagonalisation
+    if (ld % 4 != 0) return fail(CIR_ERR_ARG, "eigh: the leading dimension must be a multiple of 4");
     const size_t sm = tridiag_smem_elems(n) * sizeof(T);
     if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "eigh: n too large for the LDS of one workgroup");
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
-    hipLaunchKernelGGL(tridiag_kernel<T>, dim3(batch), dim3(kTriNT), sm, s, n, ld, A, dd, ee, tau, vw);
-    CIR_CHECK_LAUNCH();
+    {
+        ProfScope ps(prof, CIR_PH_TRIDIAG, s);
+        hipLaunchKernelGGL(tridiag_kernel<T>, dim3(batch), dim3(kTriNT), sm, s, n, ld, A, dd, ee, tau, vw);
+        CIR_CHECK_LAUNCH();
+    }
     // 2. eigenvalues
     const size_t smb = (2 * size_t(n) + 16) * sizeof(double);
     if (smb > 160 * 1024) return fail(CIR_ERR_ARG, "eigh: n too large for bisection LDS");
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&bisect_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smb)));
-    hipLaunchKernelGGL(bisect_kernel<T>, dim3(unsigned(ceil_div(n, kEigNT)), batch), dim3(kEigNT), smb, s,
-                       n, dd, ee, w);
-    CIR_CHECK_LAUNCH();
+    {
+        ProfScope ps(prof, CIR_PH_BISECT, s);
+        hipLaunchKernelGGL(bisect_kernel<T>, dim3(unsigned(ceil_div(n, kEigNT)), batch), dim3(kEigNT), smb, s,
+                           n, dd, ee, w);
+        CIR_CHECK_LAUNCH();
+    }
     // 3. eigenvectors of T into V (row-major)
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&invit_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smb)));
+    ProfScope* pinv = new ProfScope(prof, CIR_PH_INVIT, s);
     for (int b0 = 0; b0 < batch; b0 += L.sub) {
         const int nb = std::min(L.sub, batch - b0);
         hipLaunchKernelGGL(invit_kernel<T>, dim3(unsigned(ceil_div(n, kEigNT)), nb), dim3(kEigNT), smb, s, n,
                            ld, dd, ee, w, V, scr, b0);
         CIR_CHECK_LAUNCH();
     }
+    delete pinv;
     // 4. back-transform V <- Q V, last block of reflectors first
+    ProfScope pbt(prof, CIR_PH_BACKTRANSFORM, s);
     const int nref = n - 1;
     const int nblk = int(ceil_div(nref, kBtNB));
     for (int blk = nblk - 1; blk >= 0; --blk) {
@@ -664,16 +727,16 @@ size_t eigh_workspace(int prec, int batch, int n, int ld) {
 }
 
 int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w, void* work,
-                 hipStream_t s) {
+                 hipStream_t s, Prof* prof) {
     if (batch <= 0 || n <= 0 || ld < n || !A || !V || !w || !work)
         return fail(CIR_ERR_ARG, "cir_eigh_batched: bad sizes or null pointer");
     if (batch > 65535) return fail(CIR_ERR_ARG, "cir_eigh_batched: batch > 65535");
     unsigned char* wk = static_cast<unsigned char*>(work);
     if (prec == CIR_PREC_FP64)
         return eigh_impl<double>(CIR_PREC_FP64, batch, n, ld, static_cast<double*>(A), static_cast<double*>(V), w,
-                                 wk, s);
+                                 wk, s, prof);
     return eigh_impl<float>(CIR_PREC_FP32, batch, n, ld, static_cast<float*>(A), static_cast<float*>(V), w, wk,
-                            s);
+                            s, prof);
 }
 
 }  // namespace cir

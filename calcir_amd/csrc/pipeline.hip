@@ -70,6 +70,9 @@ struct cir_ctx {
     int64_t nsteps = 0, nframes_used = 0, hist_skipped = 0;
     double seconds = 0.0;
     bool have = false;
+    cir::Prof prof;
+    double phase_sec[CIR_NPHASE] = {};
+    long long phase_cnt[CIR_NPHASE] = {};
 };
 
 using namespace cir;
@@ -188,7 +191,10 @@ int finalize(cir_ctx* c, const std::vector<double>& out /* ns*nt*6 */, const std
     CIR_TRY(dX.alloc(X.size() * sizeof(double)));
     CIR_TRY(dy.alloc(3 * size_t(c->nfft) * sizeof(double)));
     CIR_HIP(hipMemcpyAsync(dX.p, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    CIR_TRY(spectrum_c2r(3, c->nfft, nt, dX.as<double>(), dy.as<double>(), c->stream));
+    {
+        ProfScope ps(&c->prof, CIR_PH_FINALIZE, c->stream);
+        CIR_TRY(spectrum_c2r(3, c->nfft, nt, dX.as<double>(), dy.as<double>(), c->stream));
+    }
     std::vector<double> y(3 * size_t(c->nfft));
     CIR_HIP(hipMemcpyAsync(y.data(), dy.p, y.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CIR_HIP(hipStreamSynchronize(c->stream));
@@ -282,15 +288,29 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     for (int c0 = 0; c0 < nu; c0 += B) {
         const int nb = std::min(B, nu - c0);
         CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxc.as<float>()));
-        CIR_TRY(cir_eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
-                                p.cutoff, xyzc.as<float>(), boxc.as<float>(), sites.p, st));
-        CIR_TRY(cir_hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, boxc.as<float>(), sites.p,
-                                H.p, st));
-        CIR_TRY(eigh_batched(rp, nb, N, ld, H.p, V.p, w.as<double>(), work.p, st));
-        CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
-                         wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom, skip.as<long long>(), st));
-        CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
-                                 sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st));
+        Prof* pf = &c->prof;
+        {
+            ProfScope ps(pf, CIR_PH_SITES, st);
+            CIR_TRY(cir_eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
+                                    p.cutoff, xyzc.as<float>(), boxc.as<float>(), sites.p, st));
+        }
+        {
+            ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
+            CIR_TRY(cir_hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, boxc.as<float>(),
+                                    sites.p, H.p, st));
+        }
+        CIR_TRY(eigh_batched(rp, nb, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
+        {
+            ProfScope ps(pf, CIR_PH_IPR, st);
+            CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
+                             w.as<double>(), wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom,
+                             skip.as<long long>(), st));
+        }
+        {
+            ProfScope ps(pf, CIR_PH_SW, st);
+            CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
+                                     w.as<double>(), sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st));
+        }
         for (int u = c0; u < c0 + nb; ++u) {
             const int bl = u - c0;
             const char* Vu = V.as<char>() + size_t(bl) * N * ld * esz;
@@ -298,11 +318,17 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
             const char* su = sites.as<char>() + size_t(bl) * CIR_NSITE * N * esz;
             bool has_prop = false;
             for (int q = 0; q < nslot; ++q) has_prop |= S.action[size_t(u) * nslot + q] == 2;
-            if (has_prop)
-                CIR_TRY(propagate(rp, c->prop_prec, N, ld, 9 * nslot, ldy, double(p.dt), Vu, wu, Ycur, Ynext, G.p, st));
-            CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, st));
-            CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, tgt.as<int>() + size_t(u) * nslot, su,
-                              Ynext, out.as<double>(), st));
+            if (has_prop) {
+                ProfScope ps(pf, CIR_PH_PROPAGATE, st);
+                CIR_TRY(propagate(rp, c->prop_prec, N, ld, 9 * nslot, ldy, double(p.dt), Vu, wu, Ycur, Ynext, G.p,
+                                  st));
+            }
+            {
+                ProfScope ps(pf, CIR_PH_CORRELATE, st);
+                CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, st));
+                CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot,
+                                  tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), st));
+            }
             std::swap(Ycur, Ynext);
         }
     }
@@ -349,21 +375,36 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
         for (int k = 0; k < nt; ++k) {
             const int f = S.fidx[size_t(s) * nt + k];
             CIR_TRY(gather_frames(c, xyz_d, box_d, &f, 1, xyz1.as<float>(), box1.as<float>()));
-            CIR_TRY(cir_eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
-                                    p.cutoff, xyz1.as<float>(), box1.as<float>(), sites1.p, st));
-            CIR_TRY(cir_hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, box1.as<float>(),
-                                    sites1.p, H.p, st));
-            CIR_TRY(eigh_batched(rp, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st));
-            CIR_TRY(ipr_hist(rp, 1, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
-                             nullptr, accp + nom, accp + 2 * nom, accp + 3 * nom, skip.as<long long>(), st));
+            Prof* pf = &c->prof;
+            {
+                ProfScope ps(pf, CIR_PH_SITES, st);
+                CIR_TRY(cir_eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
+                                        p.cutoff, xyz1.as<float>(), box1.as<float>(), sites1.p, st));
+            }
+            {
+                ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
+                CIR_TRY(cir_hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, box1.as<float>(),
+                                        sites1.p, H.p, st));
+            }
+            CIR_TRY(eigh_batched(rp, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
+            {
+                ProfScope ps(pf, CIR_PH_IPR, st);
+                CIR_TRY(ipr_hist(rp, 1, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
+                                 w.as<double>(), nullptr, accp + nom, accp + 2 * nom, accp + 3 * nom,
+                                 skip.as<long long>(), st));
+            }
             if (k == 0) {
+                ProfScope ps(pf, CIR_PH_SW, st);
                 CIR_TRY(spectral_density(rp, 1, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
                                          w.as<double>(), sites1.p, nullptr, accp, m2.as<double>(), st));
                 CIR_TRY(set_identity(rp, N, ldf, F.p, st));
                 CIR_HIP(hipMemcpyAsync(sites0.p, sites1.p, sites1.bytes, hipMemcpyDeviceToDevice, st));
             } else {
-                CIR_TRY(propagate(rp, c->prop_prec, N, ld, N, ldf, double(p.dt), V.p, w.as<double>(), F.p, F.p, G.p, st));
+                ProfScope ps(pf, CIR_PH_PROPAGATE, st);
+                CIR_TRY(propagate(rp, c->prop_prec, N, ld, N, ldf, double(p.dt), V.p, w.as<double>(), F.p, F.p, G.p,
+                                  st));
             }
+            ProfScope pcorr(pf, CIR_PH_CORRELATE, st);
             // Y = F M(0): Re and Im blocks
             for (int part = 0; part < 2; ++part) {
                 GemmArgs g;
@@ -445,6 +486,10 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
 extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
     if (!c || !xyz_d || !box_d || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_run_device: bad arguments");
     c->have = false;
+    for (int i = 0; i < CIR_NPHASE; ++i) {
+        c->phase_sec[i] = 0.0;
+        c->phase_cnt[i] = 0;
+    }
     hipEvent_t e0, e1;
     CIR_HIP(hipEventCreate(&e0));
     CIR_HIP(hipEventCreate(&e1));
@@ -457,7 +502,23 @@ extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    c->prof.collect(c->phase_sec, c->phase_cnt, CIR_NPHASE);
     return st;
+}
+
+extern "C" int cir_set_profiling(cir_ctx* c, int on) {
+    if (!c) return fail(CIR_ERR_ARG, "null context");
+    c->prof.on = on != 0;
+    return CIR_OK;
+}
+
+extern "C" int cir_phase_times(cir_ctx* c, double* seconds, long long* groups, int nphase) {
+    if (!c || nphase < 0 || nphase > CIR_NPHASE) return fail(CIR_ERR_ARG, "cir_phase_times: bad arguments");
+    for (int i = 0; i < nphase; ++i) {
+        if (seconds) seconds[i] = c->phase_sec[i];
+        if (groups) groups[i] = c->phase_cnt[i];
+    }
+    return CIR_OK;
 }
 
 extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* box, const float* times) {
@@ -504,7 +565,7 @@ extern "C" void cir_destroy(cir_ctx* c) { delete c; }
 extern "C" int cir_eigh_batched(int prec, int batch, int n, int ld, void* A_d, void* V_d, double* w_d, void* work_d,
                                 void* stream) {
     return eigh_batched(prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32, batch, n, ld, A_d, V_d, w_d, work_d,
-                        as_stream(stream));
+                        as_stream(stream), nullptr);
 }
 
 extern "C" int cir_ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart,

@@ -7,7 +7,8 @@
 namespace cir {
 
 size_t eigh_workspace(int prec, int batch, int n, int ld);
-int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w, void* work, hipStream_t s);
+int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w, void* work, hipStream_t s,
+                 Prof* prof = nullptr);
 
 int ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart, int omegaStep,
              int nomega, const void* V, const double* w, const double* weight, double* pw, double* rw,

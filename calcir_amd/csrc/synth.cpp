@@ -6,13 +6,19 @@
 // seeded Ornstein-Uhlenbeck rigid-body model, deterministic across machines:
 //   * oxygen sites on the smallest simple-cubic lattice with >= n sites,
 //     occupied by a seeded shuffle, uniform jitter of +-0.02 nm;
-//   * uniform random orientations (Shoemake quaternions), redrawn while any
-//     intermolecular H...(H, M) distance is below 0.15 nm;
+//   * uniform random orientations (Shoemake quaternions), redrawn while an
+//     intermolecular H...H distance is below 0.18 nm or an H...M distance
+//     below 0.21 nm (hard-core contacts);
 //   * rigid TIP4P/2005 geometry: r_OH 0.09572 nm, HOH 104.52 deg, O-M
 //     0.01546 nm on the bisector;
 //   * mean-reverting OU motion of the displacement from the lattice site and
 //     of the rotation vector from the initial orientation, tau = 100 fs,
-//     frame step 10 fs, stationary amplitudes 0.02 nm and 0.3 rad.
+//     frame step 10 fs, stationary amplitudes 0.02 nm and 0.3 rad; a
+//     molecule's proposed move is rejected (it keeps its previous pose) when
+//     it would create a hard-core contact, so no frame holds the unphysical
+//     H...H or H...M overlaps whose fields lie far outside the maps' range.
+//     Six normals are drawn per molecule and frame whether or not the move is
+//     accepted, so the random stream does not depend on the outcome.
 // Random numbers: SplitMix64 with Box-Muller normals.
 
 #include <cmath>
@@ -143,68 +149,102 @@ int cir_synth_trajectory(int nwater, int nframes, uint64_t seed, float box_nm, f
     }
     double body[3][3];
     body_sites(body);
+    // neighbour lists from the lattice sites (displacements stay << 0.1 nm)
+    std::vector<std::vector<int>> nbr(nwater);
+    const double rn2 = 0.62 * 0.62;
+    for (int m = 0; m < nwater; ++m)
+        for (int o = m + 1; o < nwater; ++o) {
+            double d2 = 0;
+            for (int c = 0; c < 3; ++c) {
+                const double d = mi(site[3 * m + c] - site[3 * o + c], L);
+                d2 += d * d;
+            }
+            if (d2 < rn2) {
+                nbr[m].push_back(o);
+                nbr[o].push_back(m);
+            }
+        }
+    // lab positions of H1, H2, M per molecule
+    std::vector<double> pos(9 * size_t(nwater), 0.0);
+    auto pose = [&](int m, const Mat3& R, const double* o, double* out9) {
+        for (int s2 = 0; s2 < 3; ++s2) {
+            double v[3];
+            apply(R, body[s2], v);
+            for (int c = 0; c < 3; ++c) out9[3 * s2 + c] = o[c] + v[c];
+        }
+        (void)m;
+    };
+    const double hh2 = 0.18 * 0.18, hm2 = 0.21 * 0.21;
+    auto clashes = [&](int m, const double* p9, const std::vector<char>& placed) {
+        for (int o : nbr[m]) {
+            if (!placed[o]) continue;
+            const double* q9 = &pos[9 * size_t(o)];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    if (a == 2 && b == 2) continue;  // M...M carries no H
+                    double d2 = 0;
+                    for (int c = 0; c < 3; ++c) {
+                        const double d = mi(p9[3 * a + c] - q9[3 * b + c], L);
+                        d2 += d * d;
+                    }
+                    const double lim = (a == 2 || b == 2) ? hm2 : hh2;
+                    if (d2 < lim) return true;
+                }
+        }
+        return false;
+    };
     std::vector<Mat3> R0(nwater);
-    std::vector<double> placed(9 * size_t(nwater));  // H1, H2, M lab positions
-    const double clash2 = 0.15 * 0.15;
+    std::vector<char> placed(nwater, 0);
+    const double tp = 6.283185307179586;
     for (int m = 0; m < nwater; ++m) {
-        for (int attempt = 0; attempt < 200; ++attempt) {
+        double p9[9];
+        for (int attempt = 0; attempt < 400; ++attempt) {
             const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
-            const double tp = 6.283185307179586;
             const double qx = std::sqrt(1 - u1) * std::sin(tp * u2), qy = std::sqrt(1 - u1) * std::cos(tp * u2);
             const double qz = std::sqrt(u1) * std::sin(tp * u3), qw = std::sqrt(u1) * std::cos(tp * u3);
             R0[m] = quat_to_mat(qw, qx, qy, qz);
-            for (int s = 0; s < 3; ++s) {
-                double v[3];
-                apply(R0[m], body[s], v);
-                for (int c = 0; c < 3; ++c) placed[9 * m + 3 * s + c] = site[3 * m + c] + v[c];
-            }
-            bool ok = true;
-            for (int o = 0; o < m && ok; ++o) {
-                double doo = 0;
-                for (int c = 0; c < 3; ++c) {
-                    const double d = mi(site[3 * m + c] - site[3 * o + c], L);
-                    doo += d * d;
-                }
-                if (doo > 0.5 * 0.5) continue;
-                for (int s1 = 0; s1 < 3 && ok; ++s1)
-                    for (int s2 = 0; s2 < 3 && ok; ++s2) {
-                        if (s1 == 2 && s2 == 2) continue;  // M...M is not an H contact
-                        double dd = 0;
-                        for (int c = 0; c < 3; ++c) {
-                            const double d = mi(placed[9 * m + 3 * s1 + c] - placed[9 * o + 3 * s2 + c], L);
-                            dd += d * d;
-                        }
-                        if (dd < clash2) ok = false;
-                    }
-            }
-            if (ok) break;
+            pose(m, R0[m], &site[3 * m], p9);
+            if (!clashes(m, p9, placed)) break;
         }
+        for (int q = 0; q < 9; ++q) pos[9 * size_t(m) + q] = p9[q];
+        placed[m] = 1;
     }
-    // OU motion
+    // OU motion with hard-core rejection
     const double tau = 0.100, dt = 0.010, sx = 0.02, sphi = 0.3;
     const double dec = std::exp(-dt / tau), kick = std::sqrt(1.0 - dec * dec);
     std::vector<double> disp(3 * size_t(nwater), 0.0), phi(3 * size_t(nwater), 0.0);
+    std::vector<double> opos(3 * size_t(nwater));  // oxygen positions (unwrapped)
+    for (int m = 0; m < nwater; ++m)
+        for (int c = 0; c < 3; ++c) opos[3 * m + c] = site[3 * m + c];
     const int natoms = 4 * nwater;
     for (int f = 0; f < nframes; ++f) {
         if (f > 0) {
-            for (size_t i = 0; i < disp.size(); ++i) disp[i] = disp[i] * dec + sx * kick * rng.normal();
-            for (size_t i = 0; i < phi.size(); ++i) phi[i] = phi[i] * dec + sphi * kick * rng.normal();
+            for (int m = 0; m < nwater; ++m) {
+                double nd[3], np[3], noxy[3], p9[9];
+                for (int c = 0; c < 3; ++c) nd[c] = disp[3 * m + c] * dec + sx * kick * rng.normal();
+                for (int c = 0; c < 3; ++c) np[c] = phi[3 * m + c] * dec + sphi * kick * rng.normal();
+                for (int c = 0; c < 3; ++c) noxy[c] = site[3 * m + c] + nd[c];
+                const Mat3 R = mul(rotvec_to_mat(np), R0[m]);
+                pose(m, R, noxy, p9);
+                if (clashes(m, p9, placed)) continue;  // rejected: keep the previous pose
+                for (int c = 0; c < 3; ++c) {
+                    disp[3 * m + c] = nd[c];
+                    phi[3 * m + c] = np[c];
+                    opos[3 * m + c] = noxy[c];
+                }
+                for (int q = 0; q < 9; ++q) pos[9 * size_t(m) + q] = p9[q];
+            }
         }
         float* fr = xyz + size_t(f) * natoms * 3;
         for (int m = 0; m < nwater; ++m) {
-            const Mat3 R = mul(rotvec_to_mat(&phi[3 * m]), R0[m]);
-            double o[3];
+            double shift[3];
             for (int c = 0; c < 3; ++c) {
-                double p = site[3 * m + c] + disp[3 * m + c];
-                p -= L * std::floor(p / L);
-                o[c] = p;
-                fr[3 * (4 * m) + c] = float(p);
+                const double o = opos[3 * m + c];
+                shift[c] = -L * std::floor(o / L);  // wrap the oxygen into [0, L), keep the molecule whole
+                fr[3 * (4 * m) + c] = float(o + shift[c]);
             }
-            for (int s = 0; s < 3; ++s) {
-                double v[3];
-                apply(R, body[s], v);
-                for (int c = 0; c < 3; ++c) fr[3 * (4 * m + 1 + s) + c] = float(o[c] + v[c]);
-            }
+            for (int s2 = 0; s2 < 3; ++s2)
+                for (int c = 0; c < 3; ++c) fr[3 * (4 * m + 1 + s2) + c] = float(pos[9 * size_t(m) + 3 * s2 + c] + shift[c]);
         }
         if (times) times[f] = float(f * 0.010);
     }

@@ -56,7 +56,7 @@ def hamiltonian(sites, box, nchrom_mol=2, ispecies=0, imap=0, avef=3415.2, ld=No
     """Exciton Hamiltonian (calcIR.cu:1306-1404) from site records."""
     _need(sites.is_cuda and sites.dim() == 3 and sites.shape[1] == NSITE, "sites must be [nframes, NSITE, N]")
     nframes, n = sites.shape[0], sites.shape[2]
-    ld = ld or n
+    ld = ld or (n + 3) // 4 * 4
     _need(ld >= n, "ld must be >= N")
     _need(box.shape == (nframes, 3), "box must be [nframes, 3]")
     H = torch.empty((nframes, n, ld), dtype=sites.dtype, device=sites.device)
@@ -70,7 +70,7 @@ def eigh(H, n=None):
     _need(H.is_cuda and H.dim() == 3 and H.is_contiguous(), "H must be a contiguous cuda [batch, n, ld] tensor")
     batch, nn, ld = H.shape
     n = n or nn
-    _need(nn == n and ld >= n, "H must be [batch, n, ld>=n]")
+    _need(nn == n and ld >= n and ld % 4 == 0, "H must be [batch, n, ld>=n], ld a multiple of 4")
     prec = _prec_of(H.dtype)
     V = torch.zeros_like(H)
     w = torch.empty((batch, n), dtype=torch.float64, device=H.device)

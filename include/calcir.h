@@ -125,6 +125,23 @@ typedef struct cir_results {
 
 typedef struct cir_ctx cir_ctx;
 
+/* Device-time phases reported by cir_phase_times (HIP events on the
+ * context's stream around each phase's launches). */
+enum {
+    CIR_PH_SITES = 0,         /* field projection + site records            */
+    CIR_PH_HAMILTONIAN = 1,   /* exciton Hamiltonian                        */
+    CIR_PH_TRIDIAG = 2,       /* Householder tridiagonalisation             */
+    CIR_PH_BISECT = 3,        /* tridiagonal eigenvalues                    */
+    CIR_PH_INVIT = 4,         /* tridiagonal eigenvectors                   */
+    CIR_PH_BACKTRANSFORM = 5, /* eigenvectors V = Q Z                       */
+    CIR_PH_IPR = 6,           /* IPR / frequency distribution               */
+    CIR_PH_SW = 7,            /* spectral density                           */
+    CIR_PH_PROPAGATE = 8,     /* propagation products + phase               */
+    CIR_PH_CORRELATE = 9,     /* origin bookkeeping + correlation functions */
+    CIR_PH_FINALIZE = 10,     /* C2R lineshapes                             */
+    CIR_NPHASE = 11
+};
+
 /* --- host-side boundary (no GPU needed) --------------------------------- */
 const char* cir_strerror(int status);
 const char* cir_last_error(void);
@@ -156,6 +173,11 @@ int   cir_run(cir_ctx* ctx, int nframes, const float* xyz, const float* box, con
 int   cir_run_device(cir_ctx* ctx, int nframes, const float* xyz_d, const float* box_d,
                      const float* times);
 int   cir_get_results(cir_ctx* ctx, cir_results* r);
+/* Enables (on != 0) per-phase device timing of subsequent runs. */
+int   cir_set_profiling(cir_ctx* ctx, int on);
+/* Device seconds and timed launch groups per phase of the last run
+ * (arrays of nphase <= CIR_NPHASE entries). */
+int   cir_phase_times(cir_ctx* ctx, double* seconds, long long* groups, int nphase);
 void  cir_destroy(cir_ctx* ctx);
 
 /* --- per-stage device entry points --------------------------------------- */
@@ -175,7 +197,8 @@ int cir_hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int is
  * reflectors); V_d receives the eigenvectors row-major, V[b][i*ld + k] =
  * component i of eigenvector k, and w_d (batch*n doubles) the eigenvalues in
  * ascending order.  Real arrays are float unless prec == CIR_PREC_FP64.
- * work_d must hold cir_eigh_workspace() bytes. */
+ * ld must be a multiple of 4 and A_d 16-byte aligned (rows are streamed as
+ * 16-byte vectors).  work_d must hold cir_eigh_workspace() bytes. */
 size_t cir_eigh_workspace(int prec, int batch, int n, int ld);
 int cir_eigh_batched(int prec, int batch, int n, int ld, void* A_d, void* V_d, double* w_d, void* work_d,
                      void* stream);

@@ -28,32 +28,39 @@ def _run_pair(prec, mode, oracle_prec=O.F64, nwater=128, nframes=60, seed=1001, 
     return p, g, o
 
 
-def _check(g, o, tol_tcf, tol_spec, tol_dist):
+def _check(g, o, tol_tcf, tol_spec, tol_dist, dlam=1e-3):
     for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
         assert rel(g[k], o[k]) < tol_tcf, (k, rel(g[k], o[k]))
     for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
         assert rel_l2(g[k], o[k]) < tol_spec, (k, rel_l2(g[k], o[k]))
     assert rel(g["sw"], o["sw"]) < tol_dist
-    assert g["hist_skipped"] == o["hist_skipped"]
-    # Pw is a count; a mode whose eigenvalue sits within rounding of a bin
-    # edge may move by one bin between precisions
-    assert np.abs(g["pw"] - o["pw"]).sum() <= 2 * 0.2 / (256 * 3 * 30) * 2 + 1e-12
+    # Pw is a count: a mode whose eigenvalue lies within the precision's
+    # eigenvalue error dlam (cm^-1) of a bin edge may change bin.  Expected
+    # moves ~ mode-steps * 2 dlam / omegaStep; allow 4x that, plus 2.
+    steps = g["nsteps"] * 256  # mode-steps (N = 256 chromophores)
+    unit = 1.0 / KW["omegaStep"] / steps
+    moves = 0.5 * np.abs(g["pw"] - o["pw"]).sum() / unit
+    allowed = 4 * steps * 2 * dlam / KW["omegaStep"] + 2
+    assert moves <= allowed, (moves, allowed)
+    assert abs(int(g["hist_skipped"]) - int(o["hist_skipped"])) <= allowed
+    # Rw, Rmw (ratios) on bins whose counts agree; NaN where Pw = 0
     m = o["pw"] > 0
     assert np.all(np.isnan(g["rw"][~m]))
-    assert rel(g["rw"][m], o["rw"][m]) < max(tol_dist, 1e-9) * 1e2
-    assert rel(g["rmw"][m], o["rmw"][m]) < max(tol_dist, 1e-9) * 1e2
+    same = m & (np.abs(g["pw"] - o["pw"]) < 0.5 * unit)
+    assert rel(g["rw"][same], o["rw"][same]) < max(tol_dist, 1e-9) * 1e2
+    assert rel(g["rmw"][same], o["rmw"][same]) < max(tol_dist, 1e-9) * 1e2
 
 
 def test_pipeline_fp64_batched_matches_oracle_fp64():
     p, g, o = _run_pair("fp64", "batched")
     assert g["nsteps"] == 90 and g["nframes_used"] == 50
-    _check(g, o, 1e-10, 1e-10, 1e-10)
+    _check(g, o, 1e-10, 1e-10, 1e-10, dlam=1e-9)
 
 
 def test_pipeline_fp64_plain_matches_oracle_fp64():
     p, g, o = _run_pair("fp64", "plain")
     assert g["nsteps"] == 90 and g["nframes_used"] == 90
-    _check(g, o, 1e-10, 1e-10, 1e-10)
+    _check(g, o, 1e-10, 1e-10, 1e-10, dlam=1e-9)
 
 
 def test_pipeline_fp32_batched_within_1e4_of_oracle():
@@ -79,7 +86,7 @@ def test_pipeline_bf16x3_within_1e3_of_fp64():
 def test_pipeline_models_species_maps(species, model, imap):
     p, g, o = _run_pair("fp64", "batched", species=species, model=model, map=imap, omegaStart=2000.0,
                         omegaStop=4200.0)
-    _check(g, o, 1e-10, 1e-10, 1e-10)
+    _check(g, o, 1e-10, 1e-10, 1e-10, dlam=1e-9)
 
 
 def test_pipeline_uncoupled_closed_form():
