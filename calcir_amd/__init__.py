@@ -113,6 +113,7 @@ SIGNATURES = {
     "cir_ipr_hist": (_i, [_i, _i, _i, _i, _i, _f, _f, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cir_spectral_density": (_i, [_i, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cir_propagate": (_i, [_i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cir_propagate_workspace": (ctypes.c_size_t, [_i, _i, _i]),
     "cir_correlate": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cir_spectrum_c2r": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "cir_gemm": (_i, [_i, _i, _i, _i, _i, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll,

@@ -2,6 +2,7 @@
 // propagation helpers, correlation functions and the C2R lineshape
 // transform (gfx950).
 
+#include <algorithm>
 #include <cfloat>
 
 #include "gemm.hpp"
@@ -10,6 +11,7 @@
 namespace cir {
 
 constexpr int kSpNT = 256;
+constexpr int kPropMaxSplit = 8;
 
 // ---------------------------------------------------------------------------
 // IPR and frequency distribution, calcIR.cu:494-538.  One lane per mode; V is
@@ -138,6 +140,41 @@ phase_kernel(int n, int ncols, int ldg, double dt, const double* __restrict__ w,
     const T gr = row[c], gi = row[ncols + c];
     row[c] = T(co) * gr - T(s) * gi;
     row[ncols + c] = T(s) * gr + T(co) * gi;
+}
+
+// Split-K reduction fused with the row phase: G'_k = exp(+i w_k dt/hbar)
+// sum_s P[s]_k, P holding nsplit partial n x (2 ncols) products.
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+phase_reduce_kernel(int n, int ncols, int nsplit, double dt, const double* __restrict__ w,
+                    const T* __restrict__ P, T* __restrict__ G) {
+    const int64_t idx = int64_t(blockIdx.x) * kSpNT + threadIdx.x;
+    if (idx >= int64_t(n) * ncols) return;
+    const int k = int(idx / ncols), c = int(idx % ncols);
+    const size_t stride = size_t(n) * 2 * ncols;
+    const size_t o = size_t(k) * 2 * ncols + c;
+    T gr = T(0), gi = T(0);
+    for (int sp = 0; sp < nsplit; ++sp) {
+        gr += P[sp * stride + o];
+        gi += P[sp * stride + o + ncols];
+    }
+    double s, co;
+    sincos(w[k] * dt / kHbar, &s, &co);
+    G[o] = T(co) * gr - T(s) * gi;
+    G[o + ncols] = T(s) * gr + T(co) * gi;
+}
+
+// Y[i][c] = sum_s P[s][i][c] for the 2 ncols columns of a split-K product.
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+sum_partials_kernel(int n, int width, int nsplit, int ldy, const T* __restrict__ P, T* __restrict__ Y) {
+    const int64_t idx = int64_t(blockIdx.x) * kSpNT + threadIdx.x;
+    if (idx >= int64_t(n) * width) return;
+    const int i = int(idx / width), c = int(idx % width);
+    const size_t stride = size_t(n) * width;
+    T acc = T(0);
+    for (int sp = 0; sp < nsplit; ++sp) acc += P[sp * stride + size_t(i) * width + c];
+    Y[size_t(i) * ldy + c] = acc;
 }
 
 // F = I as [Re | Im] (calcIR.cu:1521-1535).
@@ -307,47 +344,125 @@ int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, f
     return CIR_OK;
 }
 
+// Number of K slices of the propagation products: enough workgroups to
+// cover the chip (>= ~512) for the skinny origin products, 1 for the
+// n x 2n F-matrix step.
+static int prop_splits(int n, int ncols, int* kslice) {
+    const int64_t tiles = ceil_div(n, 64) * ceil_div(2 * ncols, 64);
+    int ns = int(std::min<int64_t>(kPropMaxSplit, std::max<int64_t>(1, ceil_div(512, tiles))));
+    ns = std::max(1, std::min(ns, n / 128));
+    int ks = int(ceil_div(ceil_div(n, ns), 32) * 32);
+    ns = int(ceil_div(n, ks));
+    *kslice = ks;
+    return ns;
+}
+
+size_t propagate_work_elems(int n, int ncols) {
+    int ks;
+    const int ns = prop_splits(n, ncols, &ks);
+    return size_t(n) * 2 * ncols * (ns > 1 ? ns + 1 : 1);
+}
+
+// One K-sliced product P[s] = A[:, slice s] B[slice s, :] (s < ns).
+static int split_gemm(int gemm_prec, int M, int N, int K, int ks, int ns, size_t esz, const void* A, int64_t a_m,
+                      int64_t a_k, const void* B, int64_t b_k, int64_t b_n, void* P, hipStream_t s) {
+    const int nfull = K / ks, rem = K - nfull * ks;
+    GemmArgs g;
+    g.M = M;
+    g.N = N;
+    g.K = ks;
+    g.batch = nfull;
+    g.A = A;
+    g.a_m = a_m;
+    g.a_k = a_k;
+    g.a_b = int64_t(ks) * a_k;
+    g.B = B;
+    g.b_k = b_k;
+    g.b_n = b_n;
+    g.b_b = int64_t(ks) * b_k;
+    g.C = P;
+    g.c_m = N;
+    g.c_n = 1;
+    g.c_b = int64_t(M) * N;
+    if (nfull > 0) CIR_TRY(gemm(gemm_prec, g, s));
+    if (rem > 0) {
+        g.K = rem;
+        g.batch = 1;
+        g.A = static_cast<const char*>(A) + size_t(nfull) * ks * a_k * esz;
+        g.B = static_cast<const char*>(B) + size_t(nfull) * ks * b_k * esz;
+        g.C = static_cast<char*>(P) + size_t(nfull) * M * N * esz;
+        CIR_TRY(gemm(gemm_prec, g, s));
+    }
+    (void)ns;
+    return CIR_OK;
+}
+
 int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
               const double* w, const void* Yin, void* Yout, void* G, hipStream_t s) {
     if (n <= 0 || ncols <= 0 || ldy < 2 * ncols || ld < n) return fail(CIR_ERR_ARG, "propagate: bad sizes");
-    GemmArgs g;
-    // G = V^T Y   (V row-major: V(i,k) at V[i*ld+k]; A(m=k, kk=i))
-    g.M = n;
-    g.N = 2 * ncols;
-    g.K = n;
-    g.A = V;
-    g.a_m = 1;
-    g.a_k = ld;
-    g.B = Yin;
-    g.b_k = ldy;
-    g.b_n = 1;
-    g.C = G;
-    g.c_m = 2 * ncols;
-    g.c_n = 1;
-    CIR_TRY(gemm(gemm_prec, g, s));
+    const size_t esz = (prec == CIR_PREC_FP64) ? sizeof(double) : sizeof(float);
+    const int w2 = 2 * ncols;
+    int ks;
+    const int ns = prop_splits(n, ncols, &ks);
     const int64_t tot = int64_t(n) * ncols;
+    if (ns == 1) {
+        // G = V^T Yin (V row-major: V(i,k) at V[i*ld+k]; A(m=k, kk=i)), phase, Yout = V G'
+        GemmArgs g;
+        g.M = n;
+        g.N = w2;
+        g.K = n;
+        g.A = V;
+        g.a_m = 1;
+        g.a_k = ld;
+        g.B = Yin;
+        g.b_k = ldy;
+        g.b_n = 1;
+        g.C = G;
+        g.c_m = w2;
+        g.c_n = 1;
+        CIR_TRY(gemm(gemm_prec, g, s));
+        if (prec == CIR_PREC_FP64)
+            hipLaunchKernelGGL(phase_kernel<double>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                               ncols, w2, dt, w, static_cast<double*>(G));
+        else
+            hipLaunchKernelGGL(phase_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n, ncols,
+                               w2, dt, w, static_cast<float*>(G));
+        CIR_CHECK_LAUNCH();
+        g = GemmArgs{};
+        g.M = n;
+        g.N = w2;
+        g.K = n;
+        g.A = V;
+        g.a_m = ld;
+        g.a_k = 1;
+        g.B = G;
+        g.b_k = w2;
+        g.b_n = 1;
+        g.C = Yout;
+        g.c_m = ldy;
+        g.c_n = 1;
+        return gemm(gemm_prec, g, s);
+    }
+    // split-K: G holds G' (n x w2), P the ns partial products behind it
+    void* P = static_cast<char*>(G) + size_t(n) * w2 * esz;
+    CIR_TRY(split_gemm(gemm_prec, n, w2, n, ks, ns, esz, V, 1, ld, Yin, ldy, 1, P, s));
     if (prec == CIR_PREC_FP64)
-        hipLaunchKernelGGL(phase_kernel<double>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
-                           ncols, 2 * ncols, dt, w, static_cast<double*>(G));
+        hipLaunchKernelGGL(phase_reduce_kernel<double>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           ncols, ns, dt, w, static_cast<const double*>(P), static_cast<double*>(G));
     else
-        hipLaunchKernelGGL(phase_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n, ncols,
-                           2 * ncols, dt, w, static_cast<float*>(G));
+        hipLaunchKernelGGL(phase_reduce_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           ncols, ns, dt, w, static_cast<const float*>(P), static_cast<float*>(G));
     CIR_CHECK_LAUNCH();
-    // Y = V G'
-    g = GemmArgs{};
-    g.M = n;
-    g.N = 2 * ncols;
-    g.K = n;
-    g.A = V;
-    g.a_m = ld;
-    g.a_k = 1;
-    g.B = G;
-    g.b_k = 2 * ncols;
-    g.b_n = 1;
-    g.C = Yout;
-    g.c_m = ldy;
-    g.c_n = 1;
-    return gemm(gemm_prec, g, s);
+    CIR_TRY(split_gemm(gemm_prec, n, w2, n, ks, ns, esz, V, ld, 1, G, w2, 1, P, s));
+    const int64_t tot2 = int64_t(n) * w2;
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(sum_partials_kernel<double>, dim3(unsigned(ceil_div(tot2, kSpNT))), dim3(kSpNT), 0, s, n, w2,
+                           ns, ldy, static_cast<const double*>(P), static_cast<double*>(Yout));
+    else
+        hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(unsigned(ceil_div(tot2, kSpNT))), dim3(kSpNT), 0, s, n, w2,
+                           ns, ldy, static_cast<const float*>(P), static_cast<float*>(Yout));
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
 }
 
 int set_identity(int prec, int n, int ldf, void* F, hipStream_t s) {

@@ -250,14 +250,19 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     const size_t esz = c->esz;
     const int rp = c->real_prec;
     hipStream_t st = c->stream;
-
-    DevBuf xyzc, boxc, sites, H, V, w, work, wmult, worig, m2, Y0, Y1, G, act, tgt, out, acc, skip;
+    // One stream: the propagation of a chunk follows its eigensolve.  (Run
+    // concurrently on a second stream it only competes with the
+    // tridiagonalisation, which keeps two workgroups resident on every CU.)
+    DevBuf xyzc, boxc, H, work, wmult, worig, m2, Y0, Y1, G, act, tgt, out, acc, skip;
+    DevBuf sites[1], V[1], w[1];
+    const int set = 0;
+    hipStream_t sp = st;
     CIR_TRY(xyzc.alloc(size_t(B) * c->natoms * 3 * sizeof(float)));
     CIR_TRY(boxc.alloc(size_t(B) * 3 * sizeof(float)));
-    CIR_TRY(sites.alloc(size_t(B) * CIR_NSITE * N * esz));
+    CIR_TRY(sites[set].alloc(size_t(B) * CIR_NSITE * N * esz));
+    CIR_TRY(V[set].alloc(size_t(B) * N * ld * esz));
+    CIR_TRY(w[set].alloc(size_t(B) * N * sizeof(double)));
     CIR_TRY(H.alloc(size_t(B) * N * ld * esz));
-    CIR_TRY(V.alloc(size_t(B) * N * ld * esz));
-    CIR_TRY(w.alloc(size_t(B) * N * sizeof(double)));
     CIR_TRY(work.alloc(eigh_workspace(rp, B, N, ld)));
     CIR_TRY(wmult.alloc(size_t(nu) * sizeof(double)));
     CIR_TRY(worig.alloc(size_t(nu) * sizeof(double)));
@@ -266,7 +271,7 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     const int ldy = 18 * nslot;
     CIR_TRY(Y0.alloc(size_t(N) * ldy * esz));
     CIR_TRY(Y1.alloc(size_t(N) * ldy * esz));
-    CIR_TRY(G.alloc(size_t(N) * ldy * esz));
+    CIR_TRY(G.alloc(propagate_work_elems(N, 9 * nslot) * esz));
     CIR_TRY(act.alloc(S.action.size() * sizeof(int)));
     CIR_TRY(tgt.alloc(S.target.size() * sizeof(int)));
     CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
@@ -285,49 +290,50 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     double* accp = acc.as<double>();
     void* Ycur = Y0.p;
     void* Ynext = Y1.p;
+    Prof* pf = &c->prof;
     for (int c0 = 0; c0 < nu; c0 += B) {
         const int nb = std::min(B, nu - c0);
         CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxc.as<float>()));
-        Prof* pf = &c->prof;
         {
             ProfScope ps(pf, CIR_PH_SITES, st);
             CIR_TRY(cir_eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
-                                    p.cutoff, xyzc.as<float>(), boxc.as<float>(), sites.p, st));
+                                    p.cutoff, xyzc.as<float>(), boxc.as<float>(), sites[set].p, st));
         }
         {
             ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
             CIR_TRY(cir_hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, boxc.as<float>(),
-                                    sites.p, H.p, st));
+                                    sites[set].p, H.p, st));
         }
-        CIR_TRY(eigh_batched(rp, nb, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
+        CIR_TRY(eigh_batched(rp, nb, N, ld, H.p, V[set].p, w[set].as<double>(), work.p, st, pf));
         {
             ProfScope ps(pf, CIR_PH_IPR, st);
-            CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
-                             w.as<double>(), wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom,
+            CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V[set].p,
+                             w[set].as<double>(), wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom,
                              skip.as<long long>(), st));
         }
         {
             ProfScope ps(pf, CIR_PH_SW, st);
-            CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
-                                     w.as<double>(), sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st));
+            CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V[set].p,
+                                     w[set].as<double>(), sites[set].p, worig.as<double>() + c0, accp,
+                                     m2.as<double>(), st));
         }
         for (int u = c0; u < c0 + nb; ++u) {
             const int bl = u - c0;
-            const char* Vu = V.as<char>() + size_t(bl) * N * ld * esz;
-            const double* wu = w.as<double>() + size_t(bl) * N;
-            const char* su = sites.as<char>() + size_t(bl) * CIR_NSITE * N * esz;
+            const char* Vu = V[set].as<char>() + size_t(bl) * N * ld * esz;
+            const double* wu = w[set].as<double>() + size_t(bl) * N;
+            const char* su = sites[set].as<char>() + size_t(bl) * CIR_NSITE * N * esz;
             bool has_prop = false;
             for (int q = 0; q < nslot; ++q) has_prop |= S.action[size_t(u) * nslot + q] == 2;
             if (has_prop) {
-                ProfScope ps(pf, CIR_PH_PROPAGATE, st);
+                ProfScope ps(pf, CIR_PH_PROPAGATE, sp);
                 CIR_TRY(propagate(rp, c->prop_prec, N, ld, 9 * nslot, ldy, double(p.dt), Vu, wu, Ycur, Ynext, G.p,
-                                  st));
+                                  sp));
             }
             {
-                ProfScope ps(pf, CIR_PH_CORRELATE, st);
-                CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, st));
+                ProfScope ps(pf, CIR_PH_CORRELATE, sp);
+                CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, sp));
                 CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot,
-                                  tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), st));
+                                  tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), sp));
             }
             std::swap(Ycur, Ynext);
         }
@@ -362,7 +368,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
     CIR_TRY(work.alloc(eigh_workspace(rp, 1, N, ld)));
     CIR_TRY(m2.alloc(size_t(N) * sizeof(double)));
     CIR_TRY(F.alloc(size_t(N) * ldf * esz));
-    CIR_TRY(G.alloc(size_t(N) * ldf * esz));
+    CIR_TRY(G.alloc(propagate_work_elems(N, N) * esz));
     CIR_TRY(Y.alloc(size_t(N) * 18 * esz));
     CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
     CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
@@ -586,6 +592,10 @@ extern "C" int cir_propagate(int prec, int n, int ld, int ncols, int ldy, float 
                              const double* w_d, const void* Yin_d, void* Yout_d, void* work_d, void* stream) {
     const int rp = prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32;
     return propagate(rp, prec, n, ld, ncols, ldy, double(dt), V_d, w_d, Yin_d, Yout_d, work_d, as_stream(stream));
+}
+
+extern "C" size_t cir_propagate_workspace(int prec, int n, int ncols) {
+    return propagate_work_elems(n, ncols) * (prec == CIR_PREC_FP64 ? sizeof(double) : sizeof(float));
 }
 
 extern "C" int cir_correlate(int prec, int n, int nslot, int ldy, const int* action_d, const int* target_d,

@@ -17,8 +17,9 @@ int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, f
                      int nomega, const void* V, const double* w, const void* sites, const double* weight,
                      double* sw, double* m2work, hipStream_t s);
 // Yout = V exp(+i diag(w) dt/hbar) V^T Yin, Y row-major n x ldy with Re in
-// columns [0, ncols) and Im in [ncols, 2 ncols); G: n*2*ncols work.  Yout
-// may alias Yin.
+// columns [0, ncols) and Im in [ncols, 2 ncols); G: propagate_work_elems()
+// reals of work.  Yout may alias Yin.
+size_t propagate_work_elems(int n, int ncols);
 int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
               const double* w, const void* Yin, void* Yout, void* G, hipStream_t s);
 int set_identity(int prec, int n, int ldf, void* F, hipStream_t s);

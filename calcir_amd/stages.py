@@ -116,7 +116,7 @@ def propagate(V, w, Y, ncols, dt, prec=None):
     if prec is None:
         prec = _prec_of(V.dtype)
     _need((prec == PREC_FP64) == (V.dtype == torch.float64), "prec does not match the dtype")
-    work = torch.empty((n, 2 * ncols), dtype=V.dtype, device=V.device)
+    work = torch.empty(lib().cir_propagate_workspace(prec, n, ncols), dtype=torch.uint8, device=V.device)
     check(lib().cir_propagate(prec, n, ld, ncols, Y.shape[1], dt, _p(V.contiguous()), _p(w.contiguous()), _p(Y),
                               _p(Y), _p(work), _stream()))
     return Y

@@ -1,0 +1,220 @@
+"""The C ABI boundary without a GPU: libcalcir loads and exports every symbol
+include/calcir.h declares; defaults, the input-file parser, model/species
+resolution, frame selection, the 13 output writers, the synthetic frame
+source and the calcir command line (calcIR.cu:9-87, 323-352, 843-919,
+1539-1643)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import calcir_amd as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "calcir.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cir_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.lib()
+    names = _declared()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(L, name), name
+        assert name in C.SIGNATURES, name
+    # and nm agrees that they are exported, C-linkage functions
+    out = subprocess.run(["nm", "-D", "--defined-only", C.LIB_PATH], stdout=subprocess.PIPE, text=True).stdout
+    exported = set(re.findall(r" T (cir_[a-z0-9_]+)$", out, flags=re.M))
+    assert set(names) <= exported
+
+
+def test_defaults_are_the_reference_defaults():
+    p = C.default_params()
+    assert p.xtcf == b"traj.xtc" and p.outf == b"spec" and p.cptf == b"spec" and p.model == b"e3b3"
+    assert p.species == b" "
+    assert (p.ntcfpoints, p.nsamples, p.omegaStep, p.natom_mol, p.nchrom_mol, p.nzeros, p.imap) == \
+        (150, 1, 5, 4, 2, 25600, 0)
+    assert (p.sampleEvery, p.omegaStart, p.omegaStop, p.beginTime) == (10.0, 2000.0, 5000.0, 0.0)
+    assert p.dt == pytest.approx(0.010) and p.t1 == pytest.approx(0.260) and p.avef == pytest.approx(3415.2)
+    assert p.cutoff == pytest.approx(0.7831)
+    assert C.nomega(p) == 601
+
+
+def test_parser_keys_comments_and_quirks(tmp_path):
+    f = tmp_path / "input.inp"
+    f.write_text(
+        "xtcf            nvt.xtc                     // the xtc file\n"
+        "outf            test                        // prefix\n"
+        "dt              0.020\n"
+        "ntcfpoints      400\n"
+        "nsamples        2\n"
+        "sampleEvery     4.0\n"
+        "t1              0.300\n"
+        "avef            3400.0\n"
+        "omegaStart      2000\n"
+        "omegaStop       5000\n"
+        "#avef            2415.2                      // commented keys are unknown\n"
+        "omegaStep       5\n"
+        "natom_mol       4\n"
+        "nchrom_mol      2\n"
+        "nzeros          25600\n"
+        "model           tip4p\n"
+        "beginTime       1.5\n"
+        "map             1\n"
+        "species         HOD/D2O\n"
+        "cutoff          0.75\n"
+        "precision       bf16x3\n"
+        "mode            plain\n")
+    p = C.read_params(str(f))
+    assert p.xtcf == b"nvt.xtc" and p.outf == b"test" and p.model == b"tip4p" and p.species == b"HOD/D2O"
+    assert p.ntcfpoints == 400 and p.nsamples == 2 and p.imap == 1
+    assert p.dt == pytest.approx(0.02) and p.t1 == pytest.approx(0.3) and p.avef == pytest.approx(3400.0)
+    assert p.beginTime == pytest.approx(1.5) and p.sampleEvery == pytest.approx(4.0)
+    assert p.cutoff == pytest.approx(0.75) and p.precision == C.PREC_BF16X3 and p.mode == C.MODE_PLAIN
+    # a line holding one token takes the next line's first token as its value
+    g = tmp_path / "quirk.inp"
+    g.write_text("outf\nspec2 extra words\nntcfpoints 7\n")
+    q = C.read_params(str(g))
+    assert q.outf == b"spec2" and q.ntcfpoints == 7
+
+
+def test_parser_warns_on_unknown_key(tmp_path):
+    f = tmp_path / "a.inp"
+    f.write_text("#avef 2415.2\nbogus 3\nntcfpoints 12\n")
+    code = ("import calcir_amd as C; p = C.read_params(%r, verbose=True); print('N', p.ntcfpoints)" % str(f))
+    out = subprocess.run(["python", "-c", code], cwd=ROOT, stdout=subprocess.PIPE, text=True).stdout
+    assert "WARNING: Parameter #avef in input file" in out
+    assert "WARNING: Parameter bogus in input file" in out
+    assert "N 12" in out
+
+
+def test_parser_missing_file_is_an_io_error(tmp_path):
+    with pytest.raises(C.CalcirError) as e:
+        C.read_params(str(tmp_path / "nope.inp"))
+    assert e.value.status == 2
+
+
+@pytest.mark.parametrize("model,val", [("tip4p", 0), ("e3b2", 0), ("tip4p2005", 1), ("e3b3", 1)])
+def test_resolve_model(model, val):
+    assert C.resolve_model(model) == val
+
+
+@pytest.mark.parametrize("species,val", [("H2O", 0), ("HOD/D2O", 1), ("HOD/H2O", 2), ("D2O", 3)])
+def test_resolve_species(species, val):
+    assert C.resolve_species(species) == val
+
+
+def test_resolve_errors():
+    with pytest.raises(C.CalcirError) as e:
+        C.resolve_model("spce")
+    assert e.value.status == 3
+    with pytest.raises(C.CalcirError) as e:
+        C.resolve_species(" ")
+    assert e.value.status == 4
+
+
+def _fake_results(p):
+    nt, nom, nfft = p.ntcfpoints, C.nomega(p), p.ntcfpoints + p.nzeros
+    rng = np.random.default_rng(0)
+    res = {k: rng.standard_normal(nt) + 1j * rng.standard_normal(nt) for k in ("tcf_ir", "tcf_vv", "tcf_vh")}
+    res.update({k: rng.standard_normal(nfft) for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh")})
+    res.update({k: rng.random(nom) for k in ("sw", "pw", "rw", "rmw")})
+    return res
+
+
+def test_writers_produce_the_thirteen_files(tmp_path):
+    p = C.default_params(outf=str(tmp_path / "run"), ntcfpoints=10, nzeros=118, avef=3400.0, omegaStart=2800.0,
+                         omegaStop=4000.0, dt=0.01)
+    res = _fake_results(p)
+    C.write_outputs(p, res)
+    names = sorted(os.listdir(tmp_path))
+    expect = sorted("run_" + s for s in ["irrtcf.dat", "iritcf.dat", "vvrtcf.dat", "vvitcf.dat", "vhrtcf.dat",
+                                         "vhitcf.dat", "spdn.dat", "Pw.dat", "Rw.dat", "Rmw.dat", "irls.dat",
+                                         "vvls.dat", "vhls.dat"])
+    assert names == expect
+    # TCF lines: "%g %g \n" (trailing space), time = i*dt
+    lines = open(tmp_path / "run_irrtcf.dat").read().split("\n")
+    assert lines[0] == "0 %g " % np.float32(res["tcf_ir"][0].real)
+    assert lines[3] == "%g %g " % (np.float32(3 * np.float32(0.01)), np.float32(res["tcf_ir"][3].real))
+    iml = open(tmp_path / "run_vhitcf.dat").read().split("\n")
+    assert iml[2] == "%g %g " % (np.float32(2 * np.float32(0.01)), np.float32(res["tcf_vh"][2].imag))
+    # distributions on the omega grid, "%g %g\n"
+    pw = open(tmp_path / "run_Pw.dat").read().split("\n")
+    assert len(pw) == C.nomega(p) + 1 and pw[-1] == ""
+    assert pw[1] == "%g %g" % (np.float32(2805.0), np.float32(res["pw"][1]))
+    # lineshape: descending frequency inside [omegaStart, omegaStop]
+    ls = np.loadtxt(tmp_path / "run_irls.dat")
+    assert np.all(np.diff(ls[:, 0]) < 0)
+    assert ls[:, 0].max() <= 4000.0 and ls[:, 0].min() >= 2800.0
+    n = p.ntcfpoints + p.nzeros
+    factor = np.float32(2 * 3.14159265359 * 5.308837367 / (np.float32(0.01) * np.float32(n)))
+    # first row of the 'positive' half: i = 0 -> freq = avef, value y0/(factor n)
+    row = [r for r in ls if abs(r[0] - 3400.0) < 1e-3][0]
+    assert row[1] == pytest.approx(np.float32(res["ftcf_ir"][0]) / (factor * n), rel=1e-5)
+
+
+def test_synthetic_trajectory_is_seeded_and_rigid():
+    a = C.synth_trajectory(64, 6, 42)
+    b = C.synth_trajectory(64, 6, 42)
+    c = C.synth_trajectory(64, 6, 43)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert not np.array_equal(a[0], c[0])
+    xyz, box, times = a
+    assert xyz.shape == (6, 256, 3) and np.allclose(times, np.arange(6) * 0.01)
+    assert box[0, 0] == pytest.approx(1.564 * (64 / 128) ** (1 / 3), rel=1e-6)
+    o, h1, h2, m = xyz[:, 0::4], xyz[:, 1::4], xyz[:, 2::4], xyz[:, 3::4]
+    assert np.allclose(np.linalg.norm(h1 - o, axis=2), 0.09572, atol=2e-6)
+    assert np.allclose(np.linalg.norm(h2 - o, axis=2), 0.09572, atol=2e-6)
+    assert np.allclose(np.linalg.norm(m - o, axis=2), 0.01546, atol=2e-6)
+    cosang = np.sum((h1 - o) * (h2 - o), axis=2) / 0.09572 ** 2
+    assert np.allclose(np.degrees(np.arccos(np.clip(cosang, -1, 1))), 104.52, atol=2e-3)
+    assert np.all((o >= 0) & (o < box[0, 0]))
+    # no hard-core contacts: H...H >= 0.18 nm, H...M >= 0.21 nm between molecules
+    L = box[0, 0]
+    for f in range(6):
+        hs = np.concatenate([h1[f], h2[f]])
+        mol = np.concatenate([np.arange(64), np.arange(64)])
+        d = hs[:, None, :] - hs[None, :, :]
+        d -= L * np.round(d / L)
+        r = np.linalg.norm(d, axis=2)
+        r[mol[:, None] == mol[None, :]] = 9
+        assert r.min() >= 0.18 - 1e-6
+        d = hs[:, None, :] - m[f][None, :, :]
+        d -= L * np.round(d / L)
+        r = np.linalg.norm(d, axis=2)
+        r[mol[:, None] == np.arange(64)[None, :]] = 9
+        assert r.min() >= 0.21 - 1e-6
+
+
+def _cli(args, cwd):
+    return subprocess.run([C.CLI_PATH] + args, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+def test_cli_usage_and_input_errors(tmp_path):
+    r = _cli([], tmp_path)
+    assert r.returncode == 1 and "Include as the first argument the name of an input file" in r.stdout
+    r = _cli(["a", "b"], tmp_path)
+    assert r.returncode == 1
+    r = _cli([str(tmp_path / "missing.inp")], tmp_path)
+    assert r.returncode == 1 and "ERROR: Could not open" in r.stdout
+    f = tmp_path / "bad.inp"
+    f.write_text("model spce\nspecies H2O\n")
+    r = _cli([str(f)], tmp_path)
+    assert r.returncode == 1 and "WARNING: model: spce is not recognized" in r.stdout
+    f.write_text("model tip4p\n")
+    r = _cli([str(f)], tmp_path)
+    assert r.returncode == 1 and "WARNING: species:   is not recognized" in r.stdout
+    f.write_text("model tip4p\nspecies H2O\nxtcf nothere.xtc\n")
+    r = _cli([str(f)], tmp_path)
+    assert r.returncode == 1 and "could not be opened" in r.stdout
+    # the echo of calcIR.cu:70-87
+    assert "\tSetting model to tip4p\n" in r.stdout and "\tSetting species to H2O\n" in r.stdout
