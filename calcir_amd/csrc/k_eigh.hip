@@ -6,8 +6,15 @@
 //      kTriNB columns the two-sided update is deferred and folded into each
 //      column and each symmetric matrix-vector product through the panel's
 //      V and W (A_now = A_panel - V W^T - W V^T); the trailing matrix is
-//      updated once per panel.  Full symmetric storage, column-major.  The
-//      reflector v_j (v_j[j+1] = 1) replaces column j of A below the
+//      updated once per panel.  Only the lower triangle (column-major, row
+//      >= column) is read or written: the symmetric product streams each
+//      lower element once and uses it for both A v and A^T v, with 16-byte
+//      raw buffer loads whose column offsets are scalar and whose next tile
+//      is in flight while a tile is consumed.  The panel's V and W live
+//      row-major (one 2*kTriNB row per matrix row), so the panel dot products
+//      V^T v and W^T v take one coalesced row per wave and lane per column,
+//      and the FP32 trailing update runs on MFMA 32x32x2 straight from it.
+//      The reflector v_j (v_j[j+1] = 1) replaces column j of A below the
 //      diagonal; rows 0..j of column j are zeroed.
 //   2. bisect_kernel: every eigenvalue of T by FP64 Sturm-sequence
 //      bisection, one lane per eigenvalue (eigenvalues come out ascending).
@@ -31,7 +38,6 @@ namespace cir {
 constexpr int kTriNT = 512;  // threads per matrix in the tridiagonalisation
 constexpr int kTriNW = kTriNT / 64;
 constexpr int kTriNB = 32;   // panel width
-constexpr int kTriRQ = 8;    // rows per thread per pass of the symmetric product
 constexpr int kBtNB = 128;   // reflectors per compact-WY block in the back-transform
 constexpr int kEigNT = 256;  // threads per block of bisection / inverse iteration
 constexpr double kClusterRel = 1e-7;
@@ -56,21 +62,27 @@ struct Vec16<double> {
     using type = double2;
     static constexpr int n = 2;
 };
-template <typename V, typename T>
-__device__ __forceinline__ void vfma(T (&acc)[4], const V& a, T s);
-template <>
-__device__ __forceinline__ void vfma<float4, float>(float (&acc)[4], const float4& a, float s) {
-    acc[0] += a.x * s;
-    acc[1] += a.y * s;
-    acc[2] += a.z * s;
-    acc[3] += a.w * s;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Buffer resource over one matrix of `bytes` bytes.  A raw buffer load whose
+// per-lane offset lies at or past `bytes` returns zero, which masks rows past
+// the end of a column without a branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t matrix_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, int(bytes), 0x00020000);
 }
-template <>
-__device__ __forceinline__ void vfma<double2, double>(double (&acc)[4], const double2& a, double s) {
-    acc[0] += a.x * s;
-    acc[1] += a.y * s;
+constexpr int kOffNone = int(0x80000000u);  // a lane offset past every matrix
+
+template <typename VT>
+__device__ __forceinline__ VT buffer_load16(__amdgpu_buffer_rsrc_t r, int lane_off, int col_off) {
+    return __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(r, lane_off, col_off, 0));
 }
-constexpr int kTriCU = 8;  // columns of the symmetric product in flight per lane
+
+// columns per wave strip of the lower-triangle symmetric product (16 loads of
+// 16 bytes in flight per lane)
+template <typename T>
+constexpr int strip_cols() {
+    return sizeof(T) == 4 ? 16 : 8;
+}
 
 template <typename T>
 __device__ T block_sum1(T v, T* sred) {
@@ -85,52 +97,157 @@ __device__ T block_sum1(T v, T* sred) {
     return t;
 }
 
+// LDS stride of a transposed 64-row panel tile (padded against bank conflicts)
+constexpr int kTileLd = 65;
+
 inline size_t tridiag_smem_elems(int n) {
-    const size_t uni = std::max<size_t>(2 * size_t(n), size_t(2) * 2 * kTriNB * 64);
-    return uni + size_t(kTriNW) + 2 * kTriNB + 2 * kTriNB + kTriNT;
+    const size_t uni = std::max<size_t>(2 * size_t(n), size_t(2) * 2 * kTriNB * kTileLd);
+    return uni + size_t(kTriNW) + 2 * kTriNB + 2 * kTriNB + size_t(kTriNW) * 2 * kTriNB;
 }
 
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Trailing update of the FP32 tridiagonalisation, A(jn:, jn:) -= V W^T + W V^T
+// on the lower triangle, with MFMA 32x32x2 and no LDS.  Wave w takes the
+// 32-row blocks br = w, w + kTriNW, ... and every block column bc <= br.  The
+// block product is formed transposed (MFMA row index = column of A, MFMA column
+// index = row of A) so that lanes run along rows of the column-major A.  Step s
+// of the K = 64 sum takes k = 32 h + s from lane half h: the b operand is the
+// panel row (V(r, .), W(r, .)) and the a operand the panel row of the column
+// with its halves swapped, (W(c, .), V(c, .)).
+__device__ void trailing_update_f32(int n, int ld, int jn, float* __restrict__ A, const float* __restrict__ P,
+                                    int lane, int wave) {
+    constexpr int PW = 2 * kTriNB;
+    const int nblk = (n - jn + 31) / 32;
+    const int h = lane >> 5, l32 = lane & 31;
+    for (int br = wave; br < nblk; br += kTriNW) {
+        const int r = jn + br * 32 + l32;
+        float bv[32];
+        {
+            const float4* src = reinterpret_cast<const float4*>(P + size_t(r < n ? r : 0) * PW + h * 32);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float4 t = (r < n) ? src[q] : float4{};
+                bv[4 * q] = t.x;
+                bv[4 * q + 1] = t.y;
+                bv[4 * q + 2] = t.z;
+                bv[4 * q + 3] = t.w;
+            }
+        }
+        // a operand of block column bc: panel row of column cb + l32, halves swapped
+        auto load_a = [&](float(&av)[32], int bc) {
+            const int cl = jn + bc * 32 + l32;
+            const float4* src = reinterpret_cast<const float4*>(P + size_t(cl < n ? cl : 0) * PW + (1 - h) * 32);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float4 t = (cl < n) ? src[q] : float4{};
+                av[4 * q] = t.x;
+                av[4 * q + 1] = t.y;
+                av[4 * q + 2] = t.z;
+                av[4 * q + 3] = t.w;
+            }
+        };
+        float av[32], an[32];
+        load_a(av, 0);
+        for (int bc = 0; bc <= br; ++bc) {
+            const int cb = jn + bc * 32;
+            // this block of A and the next block's operand are in flight
+            // while the MFMA chain runs
+            float old[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = cb + 8 * (i >> 2) + 4 * h + (i & 3);
+                old[i] = (c < n && r < n && r >= c) ? A[size_t(c) * ld + r] : 0.f;
+            }
+            if (bc < br) load_a(an, bc + 1);
+            floatx16 acc;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+#pragma unroll
+            for (int q = 0; q < 32; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[q], bv[q], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = cb + 8 * (i >> 2) + 4 * h + (i & 3);
+                if (c < n && r < n && r >= c) A[size_t(c) * ld + r] = old[i] - acc[i];
+            }
+#pragma unroll
+            for (int q = 0; q < 32; ++q) av[q] = an[q];
+        }
+    }
+}
+
+
+// Householder tridiagonalisation of a batch of symmetric matrices, one
+// workgroup per matrix, reading and updating only the lower triangle
+// (column-major, A(r, c) = A[c*ld + r], r >= c).  Panels of kTriNB
+// reflectors (LAPACK latrd form): within a panel the trailing block is not
+// updated; column j and the product A v are corrected with the panel's V and W
+// (A_eff = A - V W^T - W V^T), and the trailing block gets the rank-2kTriNB
+// update once per panel.  The panel lives row-major in P (row r: V(r, 0..NB)
+// then W(r, 0..NB)), so each thread reads a row's panel entries as 16-byte
+// vectors.  Output: d, e (T = tridiag(d, e)), tau, and the reflectors
+// v_j (v_j(j+1) = 1) in column j of A below the diagonal; the rest of A is
+// zeroed.  Same factorisation as LAPACK ssytrd/dsytrd (lower) on the
+// column-major view, which is what MAGMA's ssyevd "Upper" on the row-major
+// Hamiltonian computes (calcIR.cu:616-646).
 template <typename T>
-__global__ void __launch_bounds__(kTriNT)
+__global__ void __launch_bounds__(kTriNT, 2)
 tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __restrict__ eall,
                T* __restrict__ tauall, T* __restrict__ VWall) {
+    using VT = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n;
+    constexpr int PW = 2 * kTriNB;  // panel row width
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* const uni = reinterpret_cast<T*>(smem_raw);
-    const size_t unisz = max(2 * size_t(n), size_t(2) * 2 * kTriNB * 64);
-    T* const sv = uni;          // Householder vector (n)
-    T* const sp = uni + n;      // column / product (n)
-    T* const sL = uni;          // trailing update: row panel [2NB][64]
-    T* const sR = uni + 2 * kTriNB * 64;  //                 col panel [2NB][64]
-    T* const sred = uni + unisz;           // kTriNW
-    T* const srow = sred + kTriNW;          // 2*kTriNB: row j of W then V
-    T* const sdot = srow + 2 * kTriNB;      // 2*kTriNB: W^T v then V^T v
-    T* const spart = sdot + 2 * kTriNB;     // kTriNT
+    const size_t unisz = max(2 * size_t(n), size_t(2) * PW * kTileLd);
+    T* const sv = uni;                   // Householder vector (n)
+    T* const sp = uni + n;               // column / product (n)
+    T* const sL = uni;                   // trailing update: row tile [PW][kTileLd]
+    T* const sR = uni + PW * kTileLd;    //                  column tile [PW][kTileLd]
+    T* const sred = uni + unisz;         // kTriNW
+    T* const srow = sred + kTriNW;       // PW: row j of W then of V
+    T* const sdot = srow + PW;           // PW: V^T v then W^T v
+    T* const sdp = sdot + PW;            // kTriNW * PW: per-wave partial dots
 
     const int b = blockIdx.x;
     T* const A = Aall + size_t(b) * n * ld;
     T* const d = dall + size_t(b) * n;
     T* const e = eall + size_t(b) * n;
     T* const tau = tauall + size_t(b) * n;
-    T* const Vp = VWall + size_t(b) * 2 * kTriNB * ld;
-    T* const Wp = Vp + size_t(kTriNB) * ld;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T* const P = VWall + size_t(b) * PW * ld;
+    // wave index as a wave-uniform (scalar) value, so per-strip column
+    // addresses are scalar and the row offset is the only per-lane part
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     for (int j0 = 0; j0 < n - 1; j0 += kTriNB) {
         const int nbp = min(kTriNB, n - 1 - j0);
+        // clear the panel rows this panel uses (columns not formed yet are
+        // read as zeros)
+        for (size_t q = size_t(j0) * PW + size_t(tid) * VW; q < size_t(n) * PW; q += size_t(kTriNT) * VW)
+            *reinterpret_cast<VT*>(P + q) = VT{};
         for (int i = 0; i < nbp; ++i) {
             const int j = j0 + i;
             __syncthreads();
-            if (tid < i) {
-                srow[tid] = Wp[size_t(tid) * ld + j];
-                srow[kTriNB + tid] = Vp[size_t(tid) * ld + j];
+            if (tid < kTriNB) {
+                srow[tid] = (tid < i) ? P[size_t(j) * PW + kTriNB + tid] : T(0);
+                srow[kTriNB + tid] = (tid < i) ? P[size_t(j) * PW + tid] : T(0);
             }
             __syncthreads();
-            // column j with the panel's pending updates, rows j..n-1
+            // column j with the panel's pending updates, rows j..n-1 (panel
+            // columns l < i, whole 16-byte vectors)
+            const int iv = (i + VW - 1) / VW * VW;
             T sig = T(0);
             for (int r = j + tid; r < n; r += kTriNT) {
                 T a = A[size_t(j) * ld + r];
-                for (int l = 0; l < i; ++l)
-                    a -= Vp[size_t(l) * ld + r] * srow[l] + Wp[size_t(l) * ld + r] * srow[kTriNB + l];
+                const T* pr = P + size_t(r) * PW;
+                for (int l = 0; l < iv; l += VW) {
+                    const VT v4 = *reinterpret_cast<const VT*>(pr + l);
+                    const VT w4 = *reinterpret_cast<const VT*>(pr + kTriNB + l);
+                    const T* vv = reinterpret_cast<const T*>(&v4);
+                    const T* ww = reinterpret_cast<const T*>(&w4);
+#pragma unroll
+                    for (int q = 0; q < VW; ++q) a -= vv[q] * srow[l + q] + ww[q] * srow[kTriNB + l + q];
+                }
                 sp[r] = a;
                 if (r >= j + 2) sig += a * a;
             }
@@ -148,7 +265,7 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 const T vr = (r == j + 1) ? T(1) : sp[r] * scal;
                 sv[r] = vr;
                 A[size_t(j) * ld + r] = vr;
-                Vp[size_t(i) * ld + r] = vr;
+                P[size_t(r) * PW + i] = vr;
             }
             for (int r = tid; r <= j; r += kTriNT) A[size_t(j) * ld + r] = T(0);
             if (tid == 0) {
@@ -157,141 +274,197 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 tau[j] = tj;
             }
             __syncthreads();
-            const int r0 = j + 1, m = n - r0;
-            if (tj == T(0)) {
-                for (int r = r0 + tid; r < n; r += kTriNT) Wp[size_t(i) * ld + r] = T(0);
-                continue;
-            }
-            // p = A_panel(r0:, r0:) v: each lane streams 16-byte row vectors of
-            // kTriCU columns at a time (rows below r0 are read and discarded
-            // so that every load is aligned; ld is a multiple of 4)
-            if (m >= kTriNT) {
-                using VT = typename Vec16<T>::type;
-                constexpr int VW = Vec16<T>::n;
-                const int rstart = r0 & ~(VW - 1);
-                for (int rb = rstart; rb < n; rb += kTriNT * VW) {
-                    const int r = rb + tid * VW;
-                    const bool act = r < n;
-                    T acc[4] = {T(0), T(0), T(0), T(0)};
-                    int c = r0;
-                    for (; c + kTriCU <= n; c += kTriCU) {
-                        VT a[kTriCU];
-#pragma unroll
-                        for (int u = 0; u < kTriCU; ++u)
-                            a[u] = act ? *reinterpret_cast<const VT*>(A + size_t(c + u) * ld + r) : VT{};
-#pragma unroll
-                        for (int u = 0; u < kTriCU; ++u) vfma(acc, a[u], sv[c + u]);
-                    }
-                    for (; c < n; ++c) {
-                        const VT a = act ? *reinterpret_cast<const VT*>(A + size_t(c) * ld + r) : VT{};
-                        vfma(acc, a, sv[c]);
-                    }
-#pragma unroll
-                    for (int q = 0; q < VW; ++q)
-                        if (act && r + q >= r0 && r + q < n) sp[r + q] = acc[q];
-                }
-            } else {
-                const int rnd = (m + 63) & ~63;
-                const int tcn = kTriNT / rnd;
-                const int tr = tid % rnd, tc = tid / rnd;
-                T acc = T(0);
-                if (tc < tcn && tr < m)
-                    for (int c = r0 + tc; c < n; c += tcn) acc += A[size_t(c) * ld + r0 + tr] * sv[c];
-                spart[tid] = acc;
+            const int r0 = j + 1;
+            if (tj == T(0)) continue;  // W(:, i) stays zero
+            // p = A_panel(r0:, r0:) v from the lower triangle only: each wave
+            // takes strips of kStrip columns; a lane holds a 16-byte vector of
+            // rows of a (64*VW) x kStrip tile, adds the row part
+            // sum_c A(r,c) v_c to p_r with LDS atomics and keeps the
+            // transposed part sum_r A(r,c) v_r (r > c) of every column in a
+            // register, reduced over the wave when the strip ends.  Rows above
+            // the diagonal are read (the tile is aligned) and masked.
+            {
+                constexpr int kStrip = strip_cols<T>();
+                const __amdgpu_buffer_rsrc_t arsrc = matrix_rsrc(A, uint32_t(size_t(n) * ld * sizeof(T)));
+                for (int r = r0 + tid; r < n; r += kTriNT) sp[r] = T(0);
                 __syncthreads();
-                if (tid < m) {
-                    T s = T(0);
-                    for (int g = 0; g < tcn; ++g) s += spart[g * rnd + tid];
-                    sp[r0 + tid] = s;
+                for (int c = r0 + wave * kStrip; c < n; c += kTriNW * kStrip) {
+                    const int nc = min(kStrip, n - c);
+                    T dotc[kStrip];
+#pragma unroll
+                    for (int k = 0; k < kStrip; ++k) dotc[k] = T(0);
+                    // column offsets are wave-uniform (scalar); columns past n
+                    // are clamped to n - 1 and enter with zero weight
+                    auto load = [&](VT(&a)[kStrip], int rt) {
+                        const int r = rt + lane * VW;
+                        const int loff = (r < n) ? r * int(sizeof(T)) : kOffNone;
+#pragma unroll
+                        for (int k = 0; k < kStrip; ++k)
+                            a[k] = buffer_load16<VT>(arsrc, loff, min(c + k, n - 1) * ld * int(sizeof(T)));
+                    };
+                    auto tile = [&](const VT(&a)[kStrip], int rt, bool diag) {
+                        const int r = rt + lane * VW;
+                        T vr[VW], rowp[VW], vcs[kStrip];
+#pragma unroll
+                        for (int q = 0; q < VW; ++q) {
+                            vr[q] = (r + q < n) ? sv[r + q] : T(0);
+                            rowp[q] = T(0);
+                        }
+                        // v over the strip's columns (zero past n), re-read
+                        // from LDS for each tile to save registers
+#pragma unroll
+                        for (int k = 0; k < kStrip; ++k) vcs[k] = (k < nc) ? sv[c + k] : T(0);
+                        if (diag) {
+                            // tile on the diagonal: row >= column for the row
+                            // part, row > column for the transposed part
+#pragma unroll
+                            for (int k = 0; k < kStrip; ++k) {
+                                const T* ak = reinterpret_cast<const T*>(&a[k]);
+#pragma unroll
+                                for (int q = 0; q < VW; ++q) {
+                                    const int rr = r + q;
+                                    const T val = (rr >= c + k) ? ak[q] : T(0);
+                                    rowp[q] += val * vcs[k];
+                                    dotc[k] += (rr > c + k) ? val * vr[q] : T(0);
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < kStrip; ++k) {
+                                const T* ak = reinterpret_cast<const T*>(&a[k]);
+#pragma unroll
+                                for (int q = 0; q < VW; ++q) {
+                                    rowp[q] += ak[q] * vcs[k];
+                                    dotc[k] += ak[q] * vr[q];
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < VW; ++q)
+                            if (r + q < n && r + q >= c) atomicAdd(&sp[r + q], rowp[q]);
+                    };
+                    // the first tile holds the diagonal; the loads of the next
+                    // tile are in flight while a tile is consumed
+                    constexpr int kTile = 64 * VW;
+                    VT a0[kStrip], a1[kStrip];
+                    int rt = c & ~(VW - 1);
+                    load(a0, rt);
+                    bool more = rt + kTile < n;
+                    if (more) load(a1, rt + kTile);
+                    tile(a0, rt, true);
+                    while (more) {
+                        rt += kTile;
+                        more = rt + kTile < n;
+                        if (more) load(a0, rt + kTile);
+                        tile(a1, rt, false);
+                        if (!more) break;
+                        rt += kTile;
+                        more = rt + kTile < n;
+                        if (more) load(a1, rt + kTile);
+                        tile(a0, rt, false);
+                    }
+#pragma unroll
+                    for (int k = 0; k < kStrip; ++k) {
+                        const T dk = wave_sum(dotc[k]);
+                        if (lane == k && k < nc) atomicAdd(&sp[c + k], dk);
+                    }
                 }
             }
-            __syncthreads();
-            // x_l = W_l . v and y_l = V_l . v (l < i), one column per wave at a time
-            for (int q = wave; q < 2 * i; q += kTriNW) {
-                const bool isw = q < i;
-                const int l = isw ? q : q - i;
-                const T* col = (isw ? Wp : Vp) + size_t(l) * ld;
-                T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
-                int r = r0 + lane;
-                for (; r + 192 < n; r += 256) {
-                    a0 += col[r] * sv[r];
-                    a1 += col[r + 64] * sv[r + 64];
-                    a2 += col[r + 128] * sv[r + 128];
-                    a3 += col[r + 192] * sv[r + 192];
+            // sdot = (V^T v, W^T v): lane l of each wave sums P(r, l) v_r over
+            // the rows r = r0 + wave + kTriNW t (columns not formed yet are
+            // zero), right after the wave's strips of the product
+            {
+                T acc = T(0);
+                int r = r0 + wave;
+                for (; r + 7 * kTriNW < n; r += 8 * kTriNW) {
+                    T pv[8], vv[8];
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        pv[t] = P[size_t(r + t * kTriNW) * PW + lane];
+                        vv[t] = sv[r + t * kTriNW];
+                    }
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) acc += pv[t] * vv[t];
                 }
-                for (; r < n; r += 64) a0 += col[r] * sv[r];
-                const T tot = wave_sum((a0 + a1) + (a2 + a3));
-                if (lane == 0) sdot[isw ? l : kTriNB + l] = tot;
+                for (; r < n; r += kTriNW) acc += P[size_t(r) * PW + lane] * sv[r];
+                sdp[wave * PW + lane] = acc;
+            }
+            __syncthreads();  // the product's LDS atomics and the dot partials
+            if (tid < PW) {
+                T t = T(0);
+#pragma unroll
+                for (int w = 0; w < kTriNW; ++w) t += sdp[w * PW + tid];
+                sdot[tid] = t;
             }
             __syncthreads();
+            // p <- tau (p - V (W^T v) - W (V^T v)), w = p - (tau/2)(p.v) v
             T wv = T(0);
             for (int r = r0 + tid; r < n; r += kTriNT) {
-                T pr = sp[r];
-                for (int l = 0; l < i; ++l)
-                    pr -= Vp[size_t(l) * ld + r] * sdot[l] + Wp[size_t(l) * ld + r] * sdot[kTriNB + l];
-                pr *= tj;
-                sp[r] = pr;
-                wv += pr * sv[r];
+                T pr_ = sp[r];
+                const T* pr = P + size_t(r) * PW;
+                for (int l = 0; l < iv; l += VW) {
+                    const VT v4 = *reinterpret_cast<const VT*>(pr + l);
+                    const VT w4 = *reinterpret_cast<const VT*>(pr + kTriNB + l);
+                    const T* vv = reinterpret_cast<const T*>(&v4);
+                    const T* ww = reinterpret_cast<const T*>(&w4);
+#pragma unroll
+                    for (int q = 0; q < VW; ++q) pr_ -= vv[q] * sdot[kTriNB + l + q] + ww[q] * sdot[l + q];
+                }
+                pr_ *= tj;
+                sp[r] = pr_;
+                wv += pr_ * sv[r];
             }
             wv = block_sum1(wv, sred);
             const T K = T(-0.5) * tj * wv;
-            for (int r = r0 + tid; r < n; r += kTriNT) Wp[size_t(i) * ld + r] = sp[r] + K * sv[r];
+            for (int r = r0 + tid; r < n; r += kTriNT) P[size_t(r) * PW + kTriNB + i] = sp[r] + K * sv[r];
         }
-        // trailing update: A(jn:, jn:) -= V W^T + W V^T
+        // trailing update: A(jn:, jn:) -= V W^T + W V^T on lower tiles
         const int jn = j0 + nbp;
         __syncthreads();
-        if (jn < n) {
+        if constexpr (sizeof(T) == 4) {
+            if (jn < n) trailing_update_f32(n, ld, jn, A, P, lane, wave);
+        } else if (jn < n) {
             const int mt = n - jn;
             const int ntile = (mt + 63) / 64;
             const int x = tid & 63, yb = tid >> 6;
             for (int tr = 0; tr < ntile; ++tr) {
                 const int rbase = jn + tr * 64;
                 __syncthreads();
-                for (int q = tid; q < 2 * kTriNB * 64; q += kTriNT) {
-                    const int l = q >> 6, xx = q & 63, r = rbase + xx;
-                    T val = T(0);
-                    if (r < n) {
-                        if (l < kTriNB) {
-                            if (l < nbp) val = Vp[size_t(l) * ld + r];
-                        } else if (l - kTriNB < nbp) {
-                            val = Wp[size_t(l - kTriNB) * ld + r];
-                        }
-                    }
-                    sL[q] = val;
+                // sL[l][xx] = (V(r, l), W(r, l - NB)) of rows rbase + xx
+                for (int q = tid; q < PW * 64; q += kTriNT) {
+                    const int xx = q / PW, l = q % PW, r = rbase + xx;
+                    sL[l * kTileLd + xx] = (r < n) ? P[size_t(r) * PW + l] : T(0);
                 }
-                for (int tc = 0; tc < ntile; ++tc) {
+                for (int tc = 0; tc <= tr; ++tc) {
                     const int cbase = jn + tc * 64;
                     __syncthreads();
-                    for (int q = tid; q < 2 * kTriNB * 64; q += kTriNT) {
-                        const int l = q >> 6, yy = q & 63, c = cbase + yy;
-                        T val = T(0);
-                        if (c < n) {
-                            if (l < kTriNB) {
-                                if (l < nbp) val = Wp[size_t(l) * ld + c];
-                            } else if (l - kTriNB < nbp) {
-                                val = Vp[size_t(l - kTriNB) * ld + c];
-                            }
-                        }
-                        sR[q] = val;
+                    // sR[l][yy] = (W(c, l), V(c, l - NB)) of columns cbase + yy
+                    for (int q = tid; q < PW * 64; q += kTriNT) {
+                        const int yy = q / PW, l = q % PW, c = cbase + yy;
+                        const int ls = l < kTriNB ? l + kTriNB : l - kTriNB;
+                        sR[ls * kTileLd + yy] = (c < n) ? P[size_t(c) * PW + l] : T(0);
                     }
                     __syncthreads();
+                    // thread (x, yb): row rbase + x, columns cbase + yb + kTriNW q
+                    constexpr int kCols = 64 / kTriNW;
                     const int r = rbase + x;
                     if (r < n) {
-                        T a[8];
+                        T a[kCols];
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            const int c = cbase + yb + 8 * q;
-                            a[q] = (c < n) ? A[size_t(c) * ld + r] : T(0);
+                        for (int q = 0; q < kCols; ++q) {
+                            const int c = cbase + yb + kTriNW * q;
+                            a[q] = (c < n && r >= c) ? A[size_t(c) * ld + r] : T(0);
                         }
-                        for (int l = 0; l < 2 * kTriNB; ++l) {
-                            const T lv = sL[l * 64 + x];
+                        for (int l = 0; l < 2 * nbp; ++l) {
+                            const int lr = l < nbp ? l : l - nbp + kTriNB;
+                            const T lv = sL[lr * kTileLd + x];
 #pragma unroll
-                            for (int q = 0; q < 8; ++q) a[q] -= lv * sR[l * 64 + yb + 8 * q];
+                            for (int q = 0; q < kCols; ++q) a[q] -= lv * sR[lr * kTileLd + yb + kTriNW * q];
                         }
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            const int c = cbase + yb + 8 * q;
-                            if (c < n) A[size_t(c) * ld + r] = a[q];
+                        for (int q = 0; q < kCols; ++q) {
+                            const int c = cbase + yb + kTriNW * q;
+                            if (c < n && r >= c) A[size_t(c) * ld + r] = a[q];
                         }
                     }
                 }
