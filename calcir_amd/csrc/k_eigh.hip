@@ -908,8 +908,10 @@ int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w
     if (prec == CIR_PREC_FP64)
         return eigh_impl<double>(CIR_PREC_FP64, batch, n, ld, static_cast<double*>(A), static_cast<double*>(V), w,
                                  wk, s, prof);
-    return eigh_impl<float>(CIR_PREC_FP32, batch, n, ld, static_cast<float*>(A), static_cast<float*>(V), w, wk,
-                            s, prof);
+    // float arrays; the back-transform products in FP32 or split-BF16x3
+    const int gemm_prec = (prec == CIR_PREC_BF16X3) ? CIR_PREC_BF16X3 : CIR_PREC_FP32;
+    return eigh_impl<float>(gemm_prec, batch, n, ld, static_cast<float*>(A), static_cast<float*>(V), w, wk, s,
+                            prof);
 }
 
 }  // namespace cir

@@ -39,32 +39,61 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
         for (int j = 0; j < FN; ++j)
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        for (int e = tid; e < BM * BK; e += 256) {
-            int mm, kk;
-            if (a_mc) {
-                mm = e % BM;
-                kk = e / BM;
-            } else {
-                kk = e % BK;
-                mm = e / BK;
-            }
-            const int gm = m0 + mm, gk = k0 + kk;
-            As[kk][mm] = (gm < M && gk < K) ? A[int64_t(gm) * g.a_m + int64_t(gk) * g.a_k] : 0.f;
+    // The next k tile is loaded into registers while the current one is
+    // multiplied from LDS.
+    constexpr int AE = BM * BK / 256, BE = BN * BK / 256;
+    static_assert(AE * 256 == BM * BK && BE * 256 == BN * BK, "tile must split evenly over 256 threads");
+    float ra[AE], rb[BE];
+    auto a_coord = [&](int e, int& mm, int& kk) {
+        if (a_mc) {
+            mm = e % BM;
+            kk = e / BM;
+        } else {
+            kk = e % BK;
+            mm = e / BK;
         }
-        for (int e = tid; e < BN * BK; e += 256) {
+    };
+    auto b_coord = [&](int e, int& nn, int& kk) {
+        if (b_nc) {
+            nn = e % BN;
+            kk = e / BN;
+        } else {
+            kk = e % BK;
+            nn = e / BK;
+        }
+    };
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int t = 0; t < AE; ++t) {
+            int mm, kk;
+            a_coord(tid + 256 * t, mm, kk);
+            const int gm = m0 + mm, gk = k0 + kk;
+            ra[t] = (gm < M && gk < K) ? A[int64_t(gm) * g.a_m + int64_t(gk) * g.a_k] : 0.f;
+        }
+#pragma unroll
+        for (int t = 0; t < BE; ++t) {
             int nn, kk;
-            if (b_nc) {
-                nn = e % BN;
-                kk = e / BN;
-            } else {
-                kk = e % BK;
-                nn = e / BK;
-            }
+            b_coord(tid + 256 * t, nn, kk);
             const int gn = n0 + nn, gk = k0 + kk;
-            Bs[kk][nn] = (gn < N && gk < K) ? B[int64_t(gk) * g.b_k + int64_t(gn) * g.b_n] : 0.f;
+            rb[t] = (gn < N && gk < K) ? B[int64_t(gk) * g.b_k + int64_t(gn) * g.b_n] : 0.f;
+        }
+    };
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int t = 0; t < AE; ++t) {
+            int mm, kk;
+            a_coord(tid + 256 * t, mm, kk);
+            As[kk][mm] = ra[t];
+        }
+#pragma unroll
+        for (int t = 0; t < BE; ++t) {
+            int nn, kk;
+            b_coord(tid + 256 * t, nn, kk);
+            Bs[kk][nn] = rb[t];
         }
         __syncthreads();
+        if (k0 + BK < K) gload(k0 + BK);
         if constexpr (!SPLIT) {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 2) {

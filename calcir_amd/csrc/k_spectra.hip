@@ -347,6 +347,139 @@ int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, f
 // Number of K slices of the propagation products: enough workgroups to
 // cover the chip (>= ~512) for the skinny origin products, 1 for the
 // n x 2n F-matrix step.
+// ---------------------------------------------------------------------------
+// Propagation products of the batched path, Y(t+dt) = V exp(+i w dt/hbar)
+// V^T Y(t): the F update of calcIR.cu:597-617 applied to the dipole and
+// polarizability columns that calcIR.cu:628-733 multiplies by F.  n x n times
+// n x 2 ncols with a few hundred columns at most, which the tiled GEMM splits
+// poorly.  One
+// 32 x 32 output block per workgroup over all of K: rows m0..m0+31, the Re
+// columns 16 t..16 t+15 and the matching Im columns ncols + 16 t.., so that
+// the row phase can be applied to (Re, Im) pairs held by lanes l and l ^ 16.
+// The four waves take a quarter of K each and are summed through LDS.
+// Blocks are numbered row tile fastest over a row-tile count padded to a
+// multiple of 8, so every column tile of a row stripe lands on the same XCD
+// and reads the stripe of V from one L2.
+//   TRANS: op(V)(m, k) = V[k*ld + m]  (the projection G = V^T Y)
+//   else:  op(V)(m, k) = V[m*ld + k]  (the return Y = V G')
+//   PHASE: rotate row m by exp(+i w_m dt / hbar) before the store.
+//   SPLIT3: split-BF16x3 MFMA 32x32x16 (a = hi + lo, a b ~ lo hi + hi lo +
+//   hi hi); else FP32 MFMA 32x32x2.
+// ---------------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kPrNT = 256;
+constexpr int kPrMaxCols = 512;  // wider products (the F-matrix form) use the tiled GEMM
+
+template <bool TRANS, bool PHASE, bool SPLIT3>
+__global__ void __launch_bounds__(kPrNT)
+prop_mfma_kernel(int n, int ncols, int nmt_pad, int ld, const float* __restrict__ V, const float* __restrict__ B,
+                 int ldb, float* __restrict__ C, int ldc, double dt, const double* __restrict__ w) {
+    __shared__ float red[3][16][64];
+    __shared__ float ph[2][32];
+    const int mt = blockIdx.x % nmt_pad, nt = blockIdx.x / nmt_pad;
+    const int m0 = mt * 32;
+    if (m0 >= n) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, l32 = lane & 31;
+    const int cc = 16 * nt + (l32 & 15);  // complex column of this lane's operand / result
+    const bool cvalid = cc < ncols;
+    const int bcol = (l32 < 16) ? cc : ncols + cc;
+    const int m = m0 + l32;
+    const bool mvalid = m < n;
+    const int kq = ((n + 63) / 64) * 16;
+    const int kbeg = wave * kq, kend = min(n, kbeg + kq);
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    // four K steps of 16 per pass: the 64 loads of a pass are in flight together
+    constexpr int kU = 4;
+    for (int k0 = kbeg; k0 < kend; k0 += 16 * kU) {
+        float av[kU][8], bv[kU][8];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int k = k0 + 16 * u + 8 * h + q;
+                const bool kv = k < kend;
+                av[u][q] = (mvalid && kv) ? (TRANS ? V[size_t(k) * ld + m] : V[size_t(m) * ld + k]) : 0.f;
+                bv[u][q] = (cvalid && kv) ? B[size_t(k) * ldb + bcol] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if constexpr (SPLIT3) {
+                bf16x8 ahi, alo, bhi, blo;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const __bf16 a = __bf16(av[u][q]), b = __bf16(bv[u][q]);
+                    ahi[q] = a;
+                    alo[q] = __bf16(av[u][q] - float(a));
+                    bhi[q] = b;
+                    blo[q] = __bf16(bv[u][q] - float(b));
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
+            } else {
+                // instruction q sums k = 16 u + q (lanes 0..31) and 16 u + 8 + q (lanes 32..63)
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u][q], bv[u][q], acc, 0, 0, 0);
+            }
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) red[wave - 1][r][lane] = acc[r];
+    }
+    if (PHASE && tid < 32) {
+        const int mm = m0 + tid;
+        double sn = 0.0, co = 1.0;
+        if (mm < n) sincos(w[mm] * dt / kHbar, &sn, &co);
+        ph[0][tid] = float(co);
+        ph[1][tid] = float(sn);
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+            float v = acc[r] + red[0][r][lane] + red[1][r][lane] + red[2][r][lane];
+            if (PHASE) {
+                const float o = __shfl_xor(v, 16, 64);  // the other part of the same complex entry
+                const float co = ph[0][i], sn = ph[1][i];
+                v = (l32 < 16) ? co * v - sn * o : sn * o + co * v;
+            }
+            const int mm = m0 + i;
+            if (mm < n && cvalid) C[size_t(mm) * ldc + bcol] = v;
+        }
+    }
+}
+
+static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
+                         const float* Yin, float* Yout, float* G, hipStream_t s) {
+    const int nmt = int(ceil_div(n, 32));
+    const int nmt_pad = int(ceil_div(nmt, 8) * 8);
+    const dim3 grid(unsigned(nmt_pad * ceil_div(ncols, 16)));
+    const int w2 = 2 * ncols;
+    if (split3) {
+        hipLaunchKernelGGL((prop_mfma_kernel<true, true, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
+                           Yin, ldy, G, w2, dt, w);
+        CIR_CHECK_LAUNCH();
+        hipLaunchKernelGGL((prop_mfma_kernel<false, false, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
+                           G, w2, Yout, ldy, dt, w);
+    } else {
+        hipLaunchKernelGGL((prop_mfma_kernel<true, true, false>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
+                           Yin, ldy, G, w2, dt, w);
+        CIR_CHECK_LAUNCH();
+        hipLaunchKernelGGL((prop_mfma_kernel<false, false, false>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
+                           G, w2, Yout, ldy, dt, w);
+    }
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
 static int prop_splits(int n, int ncols, int* kslice) {
     const int64_t tiles = ceil_div(n, 64) * ceil_div(2 * ncols, 64);
     int ns = int(std::min<int64_t>(kPropMaxSplit, std::max<int64_t>(1, ceil_div(512, tiles))));
@@ -405,6 +538,9 @@ int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double
     int ks;
     const int ns = prop_splits(n, ncols, &ks);
     const int64_t tot = int64_t(n) * ncols;
+    if (prec != CIR_PREC_FP64 && ncols <= kPrMaxCols)
+        return prop_products(gemm_prec == CIR_PREC_BF16X3, n, ld, ncols, ldy, dt, static_cast<const float*>(V), w,
+                             static_cast<const float*>(Yin), static_cast<float*>(Yout), static_cast<float*>(G), s);
     if (ns == 1) {
         // G = V^T Yin (V row-major: V(i,k) at V[i*ld+k]; A(m=k, kk=i)), phase, Yout = V G'
         GemmArgs g;

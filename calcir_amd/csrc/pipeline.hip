@@ -63,7 +63,7 @@ struct cir_ctx {
     int imodel = 0, ispecies = 0;
     int natoms = 0, nmol = 0, N = 0, ld = 0, nomega = 0, nfft = 0;
     int real_prec = CIR_PREC_FP32;  // storage precision of real arrays
-    int prop_prec = CIR_PREC_FP32;  // product precision of the propagator
+    int prop_prec = CIR_PREC_FP32;  // product precision of the propagator and the eigenvector back-transform
     size_t esz = 4;
     hipStream_t stream = nullptr;
     std::vector<double> tcf[3], ftcf[3], sw, pw, rw, rmw;
@@ -304,7 +304,7 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
             CIR_TRY(cir_hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, boxc.as<float>(),
                                     sites[set].p, H.p, st));
         }
-        CIR_TRY(eigh_batched(rp, nb, N, ld, H.p, V[set].p, w[set].as<double>(), work.p, st, pf));
+        CIR_TRY(eigh_batched(c->prop_prec, nb, N, ld, H.p, V[set].p, w[set].as<double>(), work.p, st, pf));
         {
             ProfScope ps(pf, CIR_PH_IPR, st);
             CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V[set].p,
@@ -392,7 +392,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
                 CIR_TRY(cir_hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, box1.as<float>(),
                                         sites1.p, H.p, st));
             }
-            CIR_TRY(eigh_batched(rp, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
+            CIR_TRY(eigh_batched(c->prop_prec, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
             {
                 ProfScope ps(pf, CIR_PH_IPR, st);
                 CIR_TRY(ipr_hist(rp, 1, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
@@ -570,8 +570,8 @@ extern "C" void cir_destroy(cir_ctx* c) { delete c; }
 
 extern "C" int cir_eigh_batched(int prec, int batch, int n, int ld, void* A_d, void* V_d, double* w_d, void* work_d,
                                 void* stream) {
-    return eigh_batched(prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32, batch, n, ld, A_d, V_d, w_d, work_d,
-                        as_stream(stream), nullptr);
+    const int ep = (prec == CIR_PREC_FP64 || prec == CIR_PREC_BF16X3) ? prec : CIR_PREC_FP32;
+    return eigh_batched(ep, batch, n, ld, A_d, V_d, w_d, work_d, as_stream(stream), nullptr);
 }
 
 extern "C" int cir_ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart,

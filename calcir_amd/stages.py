@@ -65,13 +65,18 @@ def hamiltonian(sites, box, nchrom_mol=2, ispecies=0, imap=0, avef=3415.2, ld=No
     return H
 
 
-def eigh(H, n=None):
-    """Batched eigensolve of H [batch, n, ld] (destroyed).  Returns (w, V)."""
+def eigh(H, n=None, prec=None):
+    """Batched eigensolve of H [batch, n, ld] (destroyed).  Returns (w, V).
+    prec=PREC_BF16X3 (float32 H) forms the back-transform products with
+    split-BF16x3 MFMA."""
     _need(H.is_cuda and H.dim() == 3 and H.is_contiguous(), "H must be a contiguous cuda [batch, n, ld] tensor")
     batch, nn, ld = H.shape
     n = n or nn
     _need(nn == n and ld >= n and ld % 4 == 0, "H must be [batch, n, ld>=n], ld a multiple of 4")
-    prec = _prec_of(H.dtype)
+    if prec is None:
+        prec = _prec_of(H.dtype)
+    _need(prec == _prec_of(H.dtype) or (prec == PREC_BF16X3 and H.dtype == torch.float32),
+          "prec must match H's dtype (or be PREC_BF16X3 for float32)")
     V = torch.zeros_like(H)
     w = torch.empty((batch, n), dtype=torch.float64, device=H.device)
     ws = lib().cir_eigh_workspace(prec, batch, n, ld)

@@ -196,7 +196,9 @@ int cir_hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int is
  * symmetric, both triangles) and is destroyed (it receives the Householder
  * reflectors); V_d receives the eigenvectors row-major, V[b][i*ld + k] =
  * component i of eigenvector k, and w_d (batch*n doubles) the eigenvalues in
- * ascending order.  Real arrays are float unless prec == CIR_PREC_FP64.
+ * ascending order.  Real arrays are float unless prec == CIR_PREC_FP64;
+ * CIR_PREC_BF16X3 keeps float arrays and forms the back-transform products
+ * with split-BF16x3 MFMA (eigenvector error ~1e-5 instead of ~1e-6).
  * ld must be a multiple of 4 and A_d 16-byte aligned (rows are streamed as
  * 16-byte vectors).  work_d must hold cir_eigh_workspace() bytes. */
 size_t cir_eigh_workspace(int prec, int batch, int n, int ld);

@@ -166,6 +166,37 @@ def test_eigh_c2_size_invariants():
     assert np.abs(Hh @ V - V * w).max() < 5e-3
 
 
+@pytest.mark.parametrize("nwater", [128, 1024])
+def test_eigh_bf16x3_backtransform(nwater):
+    # BF16X3 changes only the back-transform products: the eigenvalues are the
+    # FP32 ones (up to the run-to-run rounding of the tridiagonalisation, whose
+    # LDS float atomics sum in no fixed order), the eigenvectors carry
+    # split-BF16 rounding (2^-16 relative per product term) through the
+    # compact-WY blocks
+    xyz, box, _ = frames(nwater, 2, 1001 if nwater == 128 else 1002)
+    announce("eigh bf16x3 back-transform N=%d" % (2 * nwater))
+    x_d, b_d = torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV)
+    sg = S.eproj_sites(x_d, b_d)
+    H = S.hamiltonian(sg, b_d, avef=3400.0)
+    Hh = H.cpu().double().numpy()
+    w32, V32 = S.eigh(H.clone())
+    wbf, Vbf = S.eigh(H.clone(), prec=C.PREC_BF16X3)
+    w = wbf.cpu().numpy()
+    n = 2 * nwater
+    assert np.abs(w - w32.cpu().numpy()).max() < 50 * 6e-8 * np.abs(w).max() * np.sqrt(n)
+    V = Vbf.cpu().double().numpy()
+    V32 = V32.cpu().double().numpy()
+    dt = 0.010 / 5.308837367
+    for f in range(2):
+        Hf = Hh[f].T
+        scale = np.abs(w[f]).max()
+        assert np.abs(V[f].T @ V[f] - np.eye(n)).max() < 3e-4
+        assert np.abs(Hf @ V[f] - V[f] * w[f]).max() < 3e-4 * scale
+        U = (V[f] * np.exp(1j * w[f] * dt)) @ V[f].T
+        U32 = (V32[f] * np.exp(1j * w[f] * dt)) @ V32[f].T
+        assert np.abs(U - U32).max() < 3e-4
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_ipr_hist_and_sw_vs_oracle(dtype):
     xyz, box, _ = frames(128, 3, 1001)
