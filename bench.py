@@ -58,6 +58,21 @@ def tridiag_bytes(n, esz=4, nb=32):
     return sym + upd
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tridiag_kernel_2048oh_traffic.json")
+
+
+def traffic_per_launch(matrices_per_launch):
+    """HBM bytes per tridiag_kernel launch from the committed counter
+    measurement (FETCH_SIZE + WRITE_SIZE per matrix, each scaled by the same
+    counter's reading for a 1 GiB copy); None when that file is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return float(t["hbm_bytes_per_matrix"]) * matrices_per_launch
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def lineshape(p, ft):
     """(freq, value) rows as written to <outf>_irls.dat (calcIR.cu:893-919)."""
     nt, nz = p.ntcfpoints, p.nzeros
@@ -121,6 +136,9 @@ def main():
     ap.add_argument("--plain-points", type=int, default=4, help="tcf points of the plain-FP32 subset")
     ap.add_argument("--no-extras", action="store_true", help="only the timed path (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -130,6 +148,15 @@ def main():
     import calcir_amd as C
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print("[bench] 1 GiB calibration copy done", file=sys.stderr, flush=True)
     nframes = args.frames
     ns = nsamples_for(nframes)
     xyz, box, times = C.synth_trajectory(NWATER, nframes, SEED, box_nm=BOX)
@@ -172,7 +199,7 @@ def main():
                                                    "iteration, split-bf16x3 MFMA propagator",
                    "nsteps_per_run": steps_per_run, "distinct_frames": res["nframes_used"]},
         "roofline": {"kernel": "tridiag_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(nmat / max(tri_n, 1)),
                      "avg_launch_ms": 1e3 * per_launch, "matrices_per_launch": nmat / max(tri_n, 1),
                      "bytes_per_matrix": tridiag_bytes(n)},
         "phases_s_per_run": {k: v[0] / args.steps for k, v in phases.items()},

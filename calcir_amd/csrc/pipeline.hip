@@ -252,7 +252,7 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     hipStream_t st = c->stream;
     // One stream: the propagation of a chunk follows its eigensolve.  (Run
     // concurrently on a second stream it only competes with the
-    // tridiagonalisation, which keeps two workgroups resident on every CU.)
+    // tridiagonalisation, which keeps every CU busy.)
     DevBuf xyzc, boxc, H, work, wmult, worig, m2, Y0, Y1, G, act, tgt, out, acc, skip;
     DevBuf sites[1], V[1], w[1];
     const int set = 0;

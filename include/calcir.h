@@ -225,8 +225,11 @@ int cir_spectral_density(int prec, int batch, int n, int ld, float t1, float ave
  * [ncols, 2 ncols).  With ncols = n and Yin = Yout = [Re F | Im F] this is the
  * literal F-matrix step; with the 9 columns F M(0) of each time origin it is
  * the vector form.  prec CIR_PREC_BF16X3 runs the two products as split
- * bf16 x 3.  The two products are split along K across workgroups when the
- * column count is small; work_d must hold cir_propagate_workspace() bytes. */
+ * bf16 x 3.  Up to 512 complex columns (float arrays) the products run as
+ * one 32 x 32 output block per workgroup over all of K with the row phase in
+ * the first product's epilogue; wider or FP64 products use the tiled GEMM,
+ * split along K when the column count is small.  work_d must hold
+ * cir_propagate_workspace() bytes. */
 size_t cir_propagate_workspace(int prec, int n, int ncols);
 int cir_propagate(int prec, int n, int ld, int ncols, int ldy, float dt, const void* V_d,
                   const double* w_d, const void* Yin_d, void* Yout_d, void* work_d, void* stream);

@@ -12,20 +12,25 @@
  * error reference).
  *
  * Parity pinning (what checks each routine independently of its own text):
- *   orc_minimage        closed form incl. half-box ties (tests/test_oracle_geometry.py)
+ *   orc_minimage, orc_frame_index   closed forms incl. half-box ties
+ *                       (tests/test_oracle_geometry_field.py)
  *   orc_eproj           brute-force Coulomb sum on hand-placed molecules, cutoff
  *                       in/out, own-molecule exclusion, M relocation, periodic
- *                       image invariance (tests/test_oracle_field.py)
+ *                       image invariance (tests/test_oracle_geometry_field.py)
  *   orc_sites           closed-form map polynomials, both maps x 4 species
  *   orc_hamiltonian     symmetry, HOD => diagonal, two-molecule TDC closed form,
- *                       intramolecular closed form (tests/test_oracle_hamiltonian.py)
- *   orc_eigh            numpy.linalg.eigh fixtures (tests/golden/eigh_*.npz),
- *                       residual and orthogonality invariants
- *   orc_c2r             numpy.fft.irfft(norm="forward") fixtures and a damped
- *                       exponential -> Lorentzian closed form
+ *                       intramolecular closed form (tests/test_oracle_maps_hamiltonian.py)
+ *   orc_eigh            numpy.linalg.eigh fixtures (tests/golden/eigh_fixtures.npz),
+ *                       residual and orthogonality invariants, lower-triangle use
+ *   orc_c2r             numpy.fft.irfft(norm="forward") fixtures, DC/Nyquist
+ *                       imaginary parts, damped exponential closed form
+ *                       (tests/test_oracle_eigh_spectrum.py)
+ *   orc_spectral_density  uncoupled chromophores -> Lorentzian sum closed form
+ *   orc_ipr_hist        localised modes -> IPR = 1, histogram normalisation
  *   orc_run             uncoupled (HOD) closed-form TCF, literal F-matrix vs
- *                       vector propagation, C_IR(0) = sum |mu|^2, static-alpha
- *                       isotropic identities, FP32 vs FP64
+ *                       vector propagation, C_IR(0) = sum |mu|^2, VV(0)/VH(0) =
+ *                       orientational averages of alpha, FP32 vs FP64
+ *                       (tests/test_oracle_pipeline.py)
  */
 #ifndef CALCIR_ORACLE_H
 #define CALCIR_ORACLE_H

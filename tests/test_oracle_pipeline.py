@@ -1,7 +1,9 @@
 """Oracle pinning of the whole reference control flow (calcIR.cu:360-919) on
 small synthetic systems: literal F-matrix vs vector propagation, the
-uncoupled (HOD) closed form, C_IR(0) = sum |mu|^2, histogram normalisation,
-IPR closed forms, spectral-density normalisation and FP32 vs FP64."""
+uncoupled (HOD) closed form, C_IR(0) = sum |mu|^2, the Raman TCFs at t = 0 as
+orientational averages, the spectral density of uncoupled chromophores as a
+Lorentzian sum, histogram normalisation, IPR closed forms and FP32 vs
+FP64."""
 
 import numpy as np
 import pytest
@@ -115,3 +117,44 @@ def test_lineshape_of_uncoupled_static_chromophores():
     # and the total intensity matches C(0): sum_k y_k = n * Re c_0
     assert np.sum(ft) == pytest.approx(n * r["tcf_ir"][0].real, rel=1e-9)
     assert r["tcf_ir"][0].real == pytest.approx(np.sum(wgt), rel=1e-12)
+
+
+def test_raman_tcfs_at_zero_are_the_orientational_averages():
+    # F(0) = I, so VV(0) and VH(0) are sums over chromophores of the isotropic
+    # averages <a_zz^2> = (2 tr(a^2) + (tr a)^2) / 15 and
+    # <a_zx^2> = (3 tr(a^2) - (tr a)^2) / 30 (calcIR.cu:736-745 combine the
+    # component products into exactly these); both are rotation invariants, so
+    # a swapped coefficient or a misplaced tensor component fails here
+    xyz, box, times = frames(16, 40, 21)
+    cfg = _cfg(nsamples=1, ntcfpoints=4)
+    r = O.run(O.F64, cfg, xyz, box, times)
+    s = O.sites(O.F64, cfg, xyz[0], box[0], O.eproj(O.F64, cfg, xyz[0], box[0]))
+    a = s[:, O.SITE_A:O.SITE_A + 6]  # xx, yy, zz, xy, yz, zx
+    tr = a[:, 0] + a[:, 1] + a[:, 2]
+    tr2 = np.sum(a[:, :3] ** 2, axis=1) + 2 * np.sum(a[:, 3:] ** 2, axis=1)
+    assert r["tcf_vv"][0].real == pytest.approx(np.sum(2 * tr2 + tr ** 2) / 15, rel=1e-12)
+    assert r["tcf_vh"][0].real == pytest.approx(np.sum(3 * tr2 - tr ** 2) / 30, rel=1e-12)
+    assert r["tcf_vv"][0].imag == 0.0 and r["tcf_vh"][0].imag == 0.0
+    # and VH(0) is positive: the map's 4.6 u u^T term makes every alpha anisotropic
+    assert r["tcf_vh"][0].real > 0
+
+
+def test_spectral_density_of_uncoupled_chromophores_is_a_lorentzian_sum():
+    # HOD: H is diagonal, each eigenvector a unit vector, so
+    # Sw(om) = (1/nsamples) sum_samples sum_i |mu_i|^2 (gamma/pi) / ((om - w_i)^2 + gamma^2)
+    # over frame 0 of each sample, gamma = hbar / (2 T1) (calcIR.cu:458-482, 1414-1444, 832)
+    xyz, box, times = frames(16, 40, 21)
+    cfg = _cfg(ispecies=1, nsamples=2)
+    r = O.run(O.F64, cfg, xyz, box, times)
+    gamma = HBAR / (float(np.float32(cfg.t1)) * 2.0)
+    om = np.array([np.float32(cfg.omegaStart + np.float32(cfg.omegaStep * i)) for i in range(len(r["sw"]))],
+                  np.float64)
+    expect = np.zeros_like(om)
+    for smp in range(cfg.nsamples):
+        f = int(round(smp * 0.05 / 0.01))
+        s = O.sites(O.F64, cfg, xyz[f], box[f], O.eproj(O.F64, cfg, xyz[f], box[f]))
+        m2 = np.sum(s[:, O.SITE_MU:O.SITE_MU + 3] ** 2, axis=1)
+        wi = s[:, O.SITE_W]
+        expect += np.sum(m2[None, :] * gamma / np.pi / ((om[:, None] - wi[None, :]) ** 2 + gamma ** 2), axis=1)
+    expect /= cfg.nsamples
+    np.testing.assert_allclose(r["sw"], expect, rtol=1e-9, atol=1e-12 * expect.max())
