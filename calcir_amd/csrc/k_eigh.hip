@@ -503,7 +503,7 @@ bisect_kernel(int n, const T* __restrict__ dall, const T* __restrict__ eall, dou
     extern __shared__ double shd[];
     double* sd = shd;
     double* se2 = shd + n;
-    double* sr = shd + 2 * n;  // 3 * (kEigNT/64)
+    double* sr = shd + 2 * n;  // 3 * (kEigNT/64) reductions, then kEigNT multisection counts
     const int b = blockIdx.y, tid = threadIdx.x;
     const T* d = dall + size_t(b) * n;
     const T* e = eall + size_t(b) * n;
@@ -542,11 +542,25 @@ bisect_kernel(int n, const T* __restrict__ dall, const T* __restrict__ eall, dou
     const double widen = 4.0 * DBL_EPSILON * tnorm + 4.0 * pivmin;
     gl -= widen;
     gu += widen;
+    // multisection: Sturm counts at kEigNT interior points of [gl, gu] give
+    // every eigenvalue a starting bracket kEigNT + 1 times narrower
+    int* scnt = reinterpret_cast<int*>(sr + 12);
+    const double step = (gu - gl) / double(kEigNT + 1);
+    scnt[tid] = sturm_count(n, sd, se2, gl + step * double(tid + 1), pivmin);
+    __syncthreads();
     const int j = blockIdx.x * kEigNT + tid;
     if (j >= n) return;
-    double a = gl, c = gu;
-    // relative tolerance (absolute floor 4 eps) on the bracket of eigenvalue j
-    for (int it = 0; it < 128 && (c - a) > 2.0 * DBL_EPSILON * (fabs(a) + fabs(c)) + 4.0 * DBL_EPSILON; ++it) {
+    // eigenvalue j lies in (x_t, x_{t+1}] with t = #points whose count <= j
+    int t = 0;
+    for (int h = kEigNT; h >= 1; h >>= 1)
+        if (t + h <= kEigNT && scnt[t + h - 1] <= j) t += h;
+    double a = (t == 0) ? gl : gl + step * double(t);
+    double c = (t == kEigNT) ? gu : gl + step * double(t + 1);
+    // tolerance: relative to the eigenvalue, with an absolute floor scaled by
+    // ||T|| (1024 eps ||T|| for an FP32-derived T, whose entries carry FP32
+    // rounding, 4 eps ||T|| for FP64)
+    const double abstol = (sizeof(T) == 4 ? 1024.0 : 4.0) * DBL_EPSILON * tnorm;
+    for (int it = 0; it < 128 && (c - a) > fmax(2.0 * DBL_EPSILON * (fabs(a) + fabs(c)), abstol); ++it) {
         const double mid = 0.5 * (a + c);
         if (sturm_count(n, sd, se2, mid, pivmin) > j)
             c = mid;
@@ -564,12 +578,19 @@ __device__ __forceinline__ double hash_uniform(uint32_t a, uint32_t b) {
     return double(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// Scratch of one matrix: S[(a*n + k)*n + c], a = 0..2 the three diagonals of
-// U (LU of T - lambda_c I with partial pivoting), a = 3 the vector.
+__host__ __device__ inline size_t invit_scratch_bytes(int n) { return 32 * size_t(n) * n; }
+
+// Scratch of one matrix (invit_scratch_bytes, FP64): the vectors X[k*n + c],
+// then U[(a*n + k)*n + c], a = 0..2 the three diagonals of U (LU of
+// T - lambda_c I with partial pivoting; a = 0 holds the reciprocal pivots).
+// Factors and vectors stay FP64 even for an FP32-derived T: eigenvectors of
+// eigenvalues closer than ~1e-7 ||T|| (but not clustered) are only told apart
+// by factors accurate beyond FP32, and near-degenerate pivots grow the vector
+// past the FP32 range before it is normalised.
 template <typename T>
 __global__ void __launch_bounds__(kEigNT)
 invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ eall,
-             const double* __restrict__ wall, T* __restrict__ Zall, double* __restrict__ scr, int b0) {
+             const double* __restrict__ wall, T* __restrict__ Zall, unsigned char* __restrict__ scr, int b0) {
     extern __shared__ double shd[];
     double* sd = shd;
     double* se = shd + n;
@@ -578,7 +599,9 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
     const T* e = eall + size_t(b) * n;
     const double* w = wall + size_t(b) * n;
     T* Z = Zall + size_t(b) * n * ld;
-    double* S = scr + size_t(bl) * 4 * n * n;
+    const size_t nn = size_t(n) * n;
+    double* const SX = reinterpret_cast<double*>(scr + size_t(bl) * invit_scratch_bytes(n));
+    double* const S = SX + nn;
     for (int k = tid; k < n; k += kEigNT) {
         sd[k] = double(d[k]);
         se[k] = (k < n - 1) ? double(e[k]) : 0.0;
@@ -596,7 +619,6 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
     int cend = c;
     while (cend + 1 < n && cend + 1 - c < kMaxCluster && joins_cluster(w, cend + 1)) ++cend;
 
-    const size_t nn = size_t(n) * n;
     double lam_prev = 0.0;
     for (int mb = c; mb <= cend; ++mb) {
         double lam = w[mb];
@@ -605,7 +627,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
         double* U0 = S + mb;
         double* U1 = S + nn + mb;
         double* U2 = S + 2 * nn + mb;
-        double* X = S + 3 * nn + mb;
+        double* X = SX + mb;
         for (int k = 0; k < n; ++k) X[size_t(k) * n] = hash_uniform(uint32_t(mb), uint32_t(k));
         double scale = 1.0;
         for (int it = 0; it < 2; ++it) {
@@ -640,7 +662,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                     yi = yi - l * yn;
                 }
                 if (fabs(u0) < pivmin) u0 = (u0 < 0.0) ? -pivmin : pivmin;
-                U0[size_t(i) * n] = u0;
+                U0[size_t(i) * n] = 1.0 / u0;  // reciprocal pivot: the back substitution multiplies
                 U1[size_t(i) * n] = u1;
                 U2[size_t(i) * n] = u2;
                 X[size_t(i) * n] = yo;
@@ -651,7 +673,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
             X[size_t(n - 1) * n] = x1;
             double nrm2 = x1 * x1;
             for (int i = n - 2; i >= 0; --i) {
-                const double xi = (X[size_t(i) * n] - U1[size_t(i) * n] * x1 - U2[size_t(i) * n] * x2) /
+                const double xi = (X[size_t(i) * n] - U1[size_t(i) * n] * x1 - U2[size_t(i) * n] * x2) *
                                   U0[size_t(i) * n];
                 X[size_t(i) * n] = xi;
                 nrm2 += xi * xi;
@@ -661,7 +683,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
             // orthogonalise against the cluster's earlier vectors
             if (mb > c) {
                 for (int q = c; q < mb; ++q) {
-                    const double* Xq = S + 3 * nn + q;
+                    const double* Xq = SX + q;
                     double dot = 0.0;
                     for (int k = 0; k < n; ++k) dot += X[size_t(k) * n] * Xq[size_t(k) * n];
                     nrm2 = 0.0;
@@ -685,7 +707,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
             for (int k = 0; k < n; ++k) X[size_t(k) * n] = -X[size_t(k) * n];
     }
     for (int mb = c; mb <= cend; ++mb) {
-        const double* X = S + 3 * nn + mb;
+        const double* X = SX + mb;
         for (int k = 0; k < n; ++k) Z[size_t(k) * ld + mb] = T(X[size_t(k) * n]);
     }
 }
@@ -696,12 +718,18 @@ template <typename T>
 __global__ void __launch_bounds__(128)
 larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __restrict__ Gall,
              T* __restrict__ Tall) {
-    __shared__ T col[kBtNB];
+    // T is built in LDS, transposed with a padded pitch (Tt[c*kLtP + r] =
+    // T(r, c)): the column step reads a row of T with lanes along r and the
+    // final copy reads it with lanes along c, both without bank conflicts
+    constexpr int kLtP = kBtNB + 1;
+    extern __shared__ __align__(16) unsigned char lsm[];
+    T* const Tt = reinterpret_cast<T*>(lsm);
+    T* const col = Tt + kBtNB * kLtP;
     const int b = blockIdx.x, tid = threadIdx.x;
     const T* tau = tauall + size_t(b) * n + j0;
     const T* G = Gall + size_t(b) * kBtNB * kBtNB;
     T* Tm = Tall + size_t(b) * kBtNB * kBtNB;
-    for (int q = tid; q < kBtNB * kBtNB; q += 128) Tm[q] = T(0);
+    for (int q = tid; q < kBtNB * kLtP; q += 128) Tt[q] = T(0);
     __syncthreads();
     for (int l = 0; l < kb; ++l) {
         const T tl = tau[l];
@@ -709,13 +737,19 @@ larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __res
         __syncthreads();
         if (tid < l) {
             T acc = T(0);
-            for (int q = tid; q < l; ++q) acc += Tm[tid * kBtNB + q] * col[q];
-            Tm[tid * kBtNB + l] = acc;
+            for (int q = tid; q < l; ++q) acc += Tt[q * kLtP + tid] * col[q];
+            Tt[l * kLtP + tid] = acc;
         }
-        if (tid == 0) Tm[l * kBtNB + l] = tl;
+        if (tid == 0) Tt[l * kLtP + l] = tl;
         __syncthreads();
     }
+    for (int q = tid; q < kBtNB * kBtNB; q += 128) {
+        const int r = q / kBtNB, c = q % kBtNB;
+        Tm[q] = Tt[c * kLtP + r];
+    }
 }
+
+inline size_t larft_smem_bytes(size_t esz) { return (size_t(kBtNB) * (kBtNB + 1) + kBtNB) * esz; }
 
 struct EighWork {
     size_t off_d, off_e, off_tau, off_vw, off_g, off_t, off_w1, off_w2, off_scr, total;
@@ -742,7 +776,7 @@ inline EighWork eigh_layout(size_t esz, int batch, int n, int ld) {
     o = al(o + esz * batch * kBtNB * size_t(n));
     L.off_w2 = o;
     o = al(o + esz * batch * kBtNB * size_t(n));
-    const size_t per = 32 * size_t(n) * n;  // 4 n^2 doubles
+    const size_t per = invit_scratch_bytes(n);
     const size_t budget = size_t(8) << 30;
     int sub = int(std::max<size_t>(1, budget / per));
     L.sub = std::min(sub, batch);
@@ -764,7 +798,7 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     T* Tm = reinterpret_cast<T*>(work + L.off_t);
     T* W1 = reinterpret_cast<T*>(work + L.off_w1);
     T* W2 = reinterpret_cast<T*>(work + L.off_w2);
-    double* scr = reinterpret_cast<double*>(work + L.off_scr);
+    unsigned char* scr = work + L.off_scr;
 
     // 1. tridiagonalisation
     if (ld % 4 != 0) return fail(CIR_ERR_ARG, "eigh: the leading dimension must be a multiple of 4");
@@ -778,7 +812,7 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
         CIR_CHECK_LAUNCH();
     }
     // 2. eigenvalues
-    const size_t smb = (2 * size_t(n) + 16) * sizeof(double);
+    const size_t smb = (2 * size_t(n) + 12 + kEigNT / 2 + 1) * sizeof(double);
     if (smb > 160 * 1024) return fail(CIR_ERR_ARG, "eigh: n too large for bisection LDS");
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&bisect_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smb)));
@@ -800,6 +834,8 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     }
     delete pinv;
     // 4. back-transform V <- Q V, last block of reflectors first
+    CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&larft_kernel<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(larft_smem_bytes(sizeof(T)))));
     ProfScope pbt(prof, CIR_PH_BACKTRANSFORM, s);
     const int nref = n - 1;
     const int nblk = int(ceil_div(nref, kBtNB));
@@ -829,7 +865,8 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
         g.c_n = 1;
         g.c_b = kBtNB * kBtNB;
         CIR_TRY(gemm(gemm_prec, g, s));
-        hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(128), 0, s, n, kb, j0, tau, G, Tm);
+        hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(128), larft_smem_bytes(sizeof(T)), s, n, kb, j0, tau,
+                           G, Tm);
         CIR_CHECK_LAUNCH();
         // W1 = Y^T Z
         g = GemmArgs{};

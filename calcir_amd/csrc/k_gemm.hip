@@ -1,12 +1,14 @@
 // k_gemm.hip -- batched dense products on gfx950 (see gemm.hpp).
 //
-// MFMA path: 256 threads = 4 waves in a 2x2 arrangement; each wave owns a
-// (BM/2) x (BN/2) sub-tile built from 32x32 MFMA fragments.  Operand tiles
-// are staged k-major in LDS (As[k][m], Bs[k][n]) so that the MFMA operand
-// fetch of a wave (32 consecutive m or n per k) is bank-conflict free for
-// either global layout.  BF16x3 splits each FP32 operand held in LDS into a
-// bf16 high part and a bf16 residual and issues three MFMAs per fragment
-// pair, accumulating in FP32.
+// MFMA paths: 256 threads = 4 waves in a 2x2 arrangement; each wave owns a
+// (BM/2) x (BN/2) sub-tile built from 32x32 MFMA fragments.  Kernels are
+// instantiated per operand layout (which stride is 1), so unit strides are
+// compile-time constants; the next k tile is prefetched into registers while
+// the current one is multiplied from LDS.  BF16x3 splits each FP32 element
+// once, when it is staged, into a bf16 high part and a bf16 residual and
+// issues three MFMAs per fragment pair, accumulating in FP32.
+
+#include <algorithm>
 
 #include "gemm.hpp"
 
@@ -15,12 +17,48 @@ namespace cir {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-template <int BM, int BN, int BK, bool SPLIT>
+// One thread's share of a BR x BK operand tile (BR rows of m or n, BK of k)
+// whose element (row, k) sits at P[row * rs + k * ks]; RC says rows are the
+// unit-stride direction (rs == 1).  Offsets inside one matrix are 32-bit
+// (gemm() checks that they fit), so the base stays a scalar and each element
+// costs one VGPR offset.
+template <int BR, int BK, bool RC>
+struct TileLoad {
+    static constexpr int E = BR * BK / 256;
+    static_assert(E * 256 == BR * BK, "tile must split evenly over 256 threads");
+    float v[E];
+    __device__ static void coord(int tid, int t, int& row, int& kk) {
+        const int e = tid + 256 * t;
+        if (RC) {
+            row = e % BR;
+            kk = e / BR;
+        } else {
+            kk = e % BK;
+            row = e / BK;
+        }
+    }
+    __device__ void load(const float* __restrict__ P, int stride, int row0, int nrow, int k0, int K, int tid) {
+#pragma unroll
+        for (int t = 0; t < E; ++t) {
+            int row, kk;
+            coord(tid, t, row, kk);
+            const int gr = row0 + row, gk = k0 + kk;
+            const int off = RC ? gr + gk * stride : gr * stride + gk;
+            v[t] = (gr < nrow && gk < K) ? P[off] : 0.f;
+        }
+    }
+};
+
+// FP32 MFMA: 256 threads = 4 waves in a 2x2 arrangement; each wave owns a
+// (BM/2) x (BN/2) sub-tile of 32x32 fragments.  Tiles are staged k-major in
+// LDS (As[k][m], Bs[k][n]) so that a wave's operand fetch (32 consecutive m or
+// n per k) is conflict free; the next k tile is loaded into registers while
+// the current one is multiplied.
+template <int BM, int BN, int BK, bool AMC, bool BNC>
 __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 32, FN = WN / 32;
     static_assert(FM >= 1 && FN >= 1, "wave tile must hold a 32x32 fragment");
-    static_assert(!SPLIT || BK % 16 == 0, "bf16 MFMA consumes k in steps of 16");
     __shared__ float As[BK][BM + 4];
     __shared__ float Bs[BK][BN + 4];
 
@@ -32,121 +70,182 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
     const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
-    const bool a_mc = (g.a_m == 1), b_nc = (g.b_n == 1);
+    const int sa = int(AMC ? g.a_k : g.a_m), sb = int(BNC ? g.b_k : g.b_n);
 
     floatx16 acc[FM][FN];
+#pragma unroll
     for (int i = 0; i < FM; ++i)
+#pragma unroll
         for (int j = 0; j < FN; ++j)
+#pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-    // The next k tile is loaded into registers while the current one is
-    // multiplied from LDS.
-    constexpr int AE = BM * BK / 256, BE = BN * BK / 256;
-    static_assert(AE * 256 == BM * BK && BE * 256 == BN * BK, "tile must split evenly over 256 threads");
-    float ra[AE], rb[BE];
-    auto a_coord = [&](int e, int& mm, int& kk) {
-        if (a_mc) {
-            mm = e % BM;
-            kk = e / BM;
-        } else {
-            kk = e % BK;
-            mm = e / BK;
-        }
-    };
-    auto b_coord = [&](int e, int& nn, int& kk) {
-        if (b_nc) {
-            nn = e % BN;
-            kk = e / BN;
-        } else {
-            kk = e % BK;
-            nn = e / BK;
-        }
-    };
-    auto gload = [&](int k0) {
-#pragma unroll
-        for (int t = 0; t < AE; ++t) {
-            int mm, kk;
-            a_coord(tid + 256 * t, mm, kk);
-            const int gm = m0 + mm, gk = k0 + kk;
-            ra[t] = (gm < M && gk < K) ? A[int64_t(gm) * g.a_m + int64_t(gk) * g.a_k] : 0.f;
-        }
-#pragma unroll
-        for (int t = 0; t < BE; ++t) {
-            int nn, kk;
-            b_coord(tid + 256 * t, nn, kk);
-            const int gn = n0 + nn, gk = k0 + kk;
-            rb[t] = (gn < N && gk < K) ? B[int64_t(gk) * g.b_k + int64_t(gn) * g.b_n] : 0.f;
-        }
-    };
-    gload(0);
+    TileLoad<BM, BK, AMC> la;
+    TileLoad<BN, BK, BNC> lb;
+    la.load(A, sa, m0, M, 0, K, tid);
+    lb.load(B, sb, n0, N, 0, K, tid);
     for (int k0 = 0; k0 < K; k0 += BK) {
 #pragma unroll
-        for (int t = 0; t < AE; ++t) {
-            int mm, kk;
-            a_coord(tid + 256 * t, mm, kk);
-            As[kk][mm] = ra[t];
+        for (int t = 0; t < la.E; ++t) {
+            int row, kk;
+            la.coord(tid, t, row, kk);
+            As[kk][row] = la.v[t];
         }
 #pragma unroll
-        for (int t = 0; t < BE; ++t) {
-            int nn, kk;
-            b_coord(tid + 256 * t, nn, kk);
-            Bs[kk][nn] = rb[t];
+        for (int t = 0; t < lb.E; ++t) {
+            int row, kk;
+            lb.coord(tid, t, row, kk);
+            Bs[kk][row] = lb.v[t];
         }
         __syncthreads();
-        if (k0 + BK < K) gload(k0 + BK);
-        if constexpr (!SPLIT) {
+        if (k0 + BK < K) {
+            la.load(A, sa, m0, M, k0 + BK, K, tid);
+            lb.load(B, sb, n0, N, k0 + BK, K, tid);
+        }
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 2) {
-                float af[FM], bf[FN];
-                for (int i = 0; i < FM; ++i) af[i] = As[kk + (lane >> 5)][wm + i * 32 + (lane & 31)];
-                for (int j = 0; j < FN; ++j) bf[j] = Bs[kk + (lane >> 5)][wn + j * 32 + (lane & 31)];
-                for (int i = 0; i < FM; ++i)
-                    for (int j = 0; j < FN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[i], bf[j], acc[i][j], 0, 0, 0);
-            }
-        } else {
+        for (int kk = 0; kk < BK; kk += 2) {
+            float af[FM], bf[FN];
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 16) {
-                bf16x8 ahi[FM], alo[FM], bhi[FN], blo[FN];
-                const int kb = kk + 8 * (lane >> 5);
-                for (int i = 0; i < FM; ++i)
-                    for (int q = 0; q < 8; ++q) {
-                        const float v = As[kb + q][wm + i * 32 + (lane & 31)];
-                        const __bf16 h = __bf16(v);
-                        ahi[i][q] = h;
-                        alo[i][q] = __bf16(v - float(h));
-                    }
+            for (int i = 0; i < FM; ++i) af[i] = As[kk + (lane >> 5)][wm + i * 32 + (lane & 31)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bf[j] = Bs[kk + (lane >> 5)][wn + j * 32 + (lane & 31)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
                 for (int j = 0; j < FN; ++j)
-                    for (int q = 0; q < 8; ++q) {
-                        const float v = Bs[kb + q][wn + j * 32 + (lane & 31)];
-                        const __bf16 h = __bf16(v);
-                        bhi[j][q] = h;
-                        blo[j][q] = __bf16(v - float(h));
-                    }
-                for (int i = 0; i < FM; ++i)
-                    for (int j = 0; j < FN; ++j) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[i], bhi[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[i], blo[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
-                    }
-            }
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
 
+    // epilogue: a fragment's 16 values of C are read together before any store
     const float alpha = float(g.alpha), beta = float(g.beta);
+#pragma unroll
     for (int i = 0; i < FM; ++i)
-        for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn + j * 32 + (lane & 31);
+            float cv[16];
+#pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int n = n0 + wn + j * 32 + (lane & 31);
-                if (m < M && n < N) {
-                    float* c = C + int64_t(m) * g.c_m + int64_t(n) * g.c_n;
-                    float v = alpha * acc[i][j][r];
-                    if (beta != 0.f) v += beta * *c;
-                    *c = v;
-                }
+                cv[r] = (beta != 0.f && m < M && n < N) ? C[int64_t(m) * g.c_m + int64_t(n) * g.c_n] : 0.f;
             }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m < M && n < N) C[int64_t(m) * g.c_m + int64_t(n) * g.c_n] = alpha * acc[i][j][r] + beta * cv[r];
+            }
+        }
+}
+
+// Split-BF16x3: each FP32 element is split once, when its tile is staged,
+// into hi = bf16(v) and lo = bf16(v - hi), stored k-contiguous per row of
+// m (or n) so that an MFMA operand (8 consecutive k of one row) is a single
+// 16-byte LDS read.  The product is lo*hi + hi*lo + hi*hi in FP32.
+template <int BM, int BN, int BK, bool AMC, bool BNC>
+__global__ void __launch_bounds__(256, 2) gemm_bf16x3_kernel(GemmArgs g) {
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int FM = WM / 32, FN = WN / 32;
+    constexpr int KP = BK + 8;  // row pitch in bf16 (16-byte aligned rows, staggered banks)
+    static_assert(FM >= 1 && FN >= 1 && BK % 16 == 0, "tile shape");
+    __shared__ __align__(16) __bf16 Ah[BM * KP];
+    __shared__ __align__(16) __bf16 Al[BM * KP];
+    __shared__ __align__(16) __bf16 Bh[BN * KP];
+    __shared__ __align__(16) __bf16 Bl[BN * KP];
+
+    const int64_t b = blockIdx.z;
+    const float* A = static_cast<const float*>(g.A) + b * g.a_b;
+    const float* B = static_cast<const float*>(g.B) + b * g.b_b;
+    float* C = static_cast<float*>(g.C) + b * g.c_b;
+    const int M = g.M, N = g.N, K = g.K;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31;
+    const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
+    const int sa = int(AMC ? g.a_k : g.a_m), sb = int(BNC ? g.b_k : g.b_n);
+
+    floatx16 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    TileLoad<BM, BK, AMC> la;
+    TileLoad<BN, BK, BNC> lb;
+    la.load(A, sa, m0, M, 0, K, tid);
+    lb.load(B, sb, n0, N, 0, K, tid);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int t = 0; t < la.E; ++t) {
+            int row, kk;
+            la.coord(tid, t, row, kk);
+            const __bf16 h = __bf16(la.v[t]);
+            Ah[row * KP + kk] = h;
+            Al[row * KP + kk] = __bf16(la.v[t] - float(h));
+        }
+#pragma unroll
+        for (int t = 0; t < lb.E; ++t) {
+            int row, kk;
+            lb.coord(tid, t, row, kk);
+            const __bf16 h = __bf16(lb.v[t]);
+            Bh[row * KP + kk] = h;
+            Bl[row * KP + kk] = __bf16(lb.v[t] - float(h));
+        }
+        __syncthreads();
+        if (k0 + BK < K) {
+            la.load(A, sa, m0, M, k0 + BK, K, tid);
+            lb.load(B, sb, n0, N, k0 + BK, K, tid);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            const int kb = kk + 8 * (lane >> 5);
+            bf16x8 ahi[FM], alo[FM], bhi[FN], blo[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const int o = (wm + i * 32 + l32) * KP + kb;
+                ahi[i] = *reinterpret_cast<const bf16x8*>(&Ah[o]);
+                alo[i] = *reinterpret_cast<const bf16x8*>(&Al[o]);
+            }
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int o = (wn + j * 32 + l32) * KP + kb;
+                bhi[j] = *reinterpret_cast<const bf16x8*>(&Bh[o]);
+                blo[j] = *reinterpret_cast<const bf16x8*>(&Bl[o]);
+            }
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[i], bhi[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[i], blo[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: a fragment's 16 values of C are read together before any store
+    const float alpha = float(g.alpha), beta = float(g.beta);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn + j * 32 + l32;
+            float cv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                cv[r] = (beta != 0.f && m < M && n < N) ? C[int64_t(m) * g.c_m + int64_t(n) * g.c_n] : 0.f;
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m < M && n < N) C[int64_t(m) * g.c_m + int64_t(n) * g.c_n] = alpha * acc[i][j][r] + beta * cv[r];
+            }
+        }
 }
 
 // FP64: 64x64 tile, 256 threads, 4x4 outputs per thread.
@@ -211,12 +310,29 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
         }
 }
 
-template <int BM, int BN, int BK, bool SPLIT>
-static int launch_mfma(const GemmArgs& g, hipStream_t s) {
+template <int BM, int BN, int BK, bool SPLIT, bool AMC, bool BNC>
+static int launch_tile(const GemmArgs& g, hipStream_t s) {
     dim3 grid(unsigned(ceil_div(g.M, BM)), unsigned(ceil_div(g.N, BN)), unsigned(g.batch));
-    hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, BK, SPLIT>), grid, dim3(256), 0, s, g);
+    if constexpr (SPLIT)
+        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, BK, AMC, BNC>), grid, dim3(256), 0, s, g);
+    else
+        hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, BK, AMC, BNC>), grid, dim3(256), 0, s, g);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
+}
+
+template <int BM, int BN, int BK, bool SPLIT>
+static int launch_layout(const GemmArgs& g, hipStream_t s) {
+    const bool amc = g.a_m == 1, bnc = g.b_n == 1;
+    if (amc && bnc) return launch_tile<BM, BN, BK, SPLIT, true, true>(g, s);
+    if (amc) return launch_tile<BM, BN, BK, SPLIT, true, false>(g, s);
+    if (bnc) return launch_tile<BM, BN, BK, SPLIT, false, true>(g, s);
+    return launch_tile<BM, BN, BK, SPLIT, false, false>(g, s);
+}
+
+// Largest in-matrix element offset of a strided operand.
+static int64_t max_offset(int rows, int64_t rs, int cols, int64_t cs) {
+    return int64_t(std::max(rows - 1, 0)) * rs + int64_t(std::max(cols - 1, 0)) * cs;
 }
 
 int gemm(int prec, const GemmArgs& g, hipStream_t s) {
@@ -231,10 +347,13 @@ int gemm(int prec, const GemmArgs& g, hipStream_t s) {
         CIR_CHECK_LAUNCH();
         return CIR_OK;
     }
+    if (max_offset(g.M, g.a_m, g.K, g.a_k) >= (int64_t(1) << 31) ||
+        max_offset(g.K, g.b_k, g.N, g.b_n) >= (int64_t(1) << 31))
+        return fail(CIR_ERR_ARG, "gemm: a matrix spans more than 2^31 elements");
     const bool big = g.M >= 128 && g.N >= 128;
     if (prec == CIR_PREC_BF16X3)
-        return big ? launch_mfma<128, 128, 32, true>(g, s) : launch_mfma<64, 64, 32, true>(g, s);
-    return big ? launch_mfma<128, 128, 16, false>(g, s) : launch_mfma<64, 64, 16, false>(g, s);
+        return big ? launch_layout<128, 128, 32, true>(g, s) : launch_layout<64, 64, 32, true>(g, s);
+    return big ? launch_layout<128, 128, 16, false>(g, s) : launch_layout<64, 64, 16, false>(g, s);
 }
 
 }  // namespace cir
