@@ -10,10 +10,12 @@
 //      >= column) is read or written: the symmetric product streams each
 //      lower element once and uses it for both A v and A^T v, with 16-byte
 //      raw buffer loads whose column offsets are scalar and whose next tile
-//      is in flight while a tile is consumed.  The panel's V and W live
-//      row-major (one 2*kTriNB row per matrix row), so the panel dot products
-//      V^T v and W^T v take one coalesced row per wave and lane per column,
-//      and the FP32 trailing update runs on MFMA 32x32x2 straight from it.
+//      is in flight while a tile is consumed; v^T A v is summed alongside.
+//      The panel's V and W live row-major (one 2*kTriNB row per matrix row):
+//      one pass per reflector reads each panel row once to finish the
+//      previous W column (its scale from v^T A v, without a further pass),
+//      form the next column and sum the next panel dot products, and the
+//      FP32 trailing update runs on MFMA 32x32x2 straight from it.
 //      The reflector v_j (v_j[j+1] = 1) replaces column j of A below the
 //      diagonal; rows 0..j of column j are zeroed.
 //   2. bisect_kernel: every eigenvalue of T by FP64 Sturm-sequence
@@ -102,7 +104,7 @@ constexpr int kTileLd = 65;
 
 inline size_t tridiag_smem_elems(int n) {
     const size_t uni = std::max<size_t>(2 * size_t(n), size_t(2) * 2 * kTriNB * kTileLd);
-    return uni + size_t(kTriNW) + 2 * kTriNB + 2 * kTriNB + size_t(kTriNW) * 2 * kTriNB;
+    return uni + size_t(kTriNW) + 2 * kTriNB + 2 * kTriNB + size_t(kTriNW) * 2 * kTriNB + size_t(kTriNW);
 }
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -184,8 +186,7 @@ __device__ void trailing_update_f32(int n, int ld, int jn, float* __restrict__ A
 // updated; column j and the product A v are corrected with the panel's V and W
 // (A_eff = A - V W^T - W V^T), and the trailing block gets the rank-2kTriNB
 // update once per panel.  The panel lives row-major in P (row r: V(r, 0..NB)
-// then W(r, 0..NB)), so each thread reads a row's panel entries as 16-byte
-// vectors.  Output: d, e (T = tridiag(d, e)), tau, and the reflectors
+// then W(r, 0..NB)), read as 16-byte vectors by the lanes of one row group.  Output: d, e (T = tridiag(d, e)), tau, and the reflectors
 // v_j (v_j(j+1) = 1) in column j of A below the diagonal; the rest of A is
 // zeroed.  Same factorisation as LAPACK ssytrd/dsytrd (lower) on the
 // column-major view, which is what MAGMA's ssyevd "Upper" on the row-major
@@ -205,9 +206,10 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
     T* const sL = uni;                   // trailing update: row tile [PW][kTileLd]
     T* const sR = uni + PW * kTileLd;    //                  column tile [PW][kTileLd]
     T* const sred = uni + unisz;         // kTriNW
-    T* const srow = sred + kTriNW;       // PW: row j of W then of V
-    T* const sdot = srow + PW;           // PW: V^T v then W^T v
+    T* const srow = sred + kTriNW;       // PW: panel row j (V then W)
+    T* const sdot = srow + PW;           // PW: y = V^T v then x = W^T v
     T* const sdp = sdot + PW;            // kTriNW * PW: per-wave partial dots
+    T* const sred2 = sdp + kTriNW * PW;  // kTriNW: v^T A v partials
 
     const int b = blockIdx.x;
     T* const A = Aall + size_t(b) * n * ld;
@@ -225,33 +227,109 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
         // read as zeros)
         for (size_t q = size_t(j0) * PW + size_t(tid) * VW; q < size_t(n) * PW; q += size_t(kTriNT) * VW)
             *reinterpret_cast<VT*>(P + q) = VT{};
+        __syncthreads();
+        // Per reflector (column j = j0 + i), with tau_prev, K_prev, x/y (sdot),
+        // v (sv) and p = A_panel v (sp) of reflector i - 1 still in LDS:
+        //   A0  row j of the panel into srow, finishing W(j, i-1)
+        //   A   one pass over rows r >= j, LPR lanes per row (16-byte vector
+        //       of the panel row each): finishes W(r, i-1) = tau_prev (p_r -
+        //       V x - W y)_r + K_prev v_r, forms u_r = A(r, j) - (V W(j,:)^T +
+        //       W V(j,:)^T)_r, and accumulates sum_{r>=j+2} u_r^2 and the raw
+        //       panel dots sum_{r>=j+2} P(r, l) u_r
+        //   B   reflector from u; v, x = W^T v and y = V^T v from the raw dots
+        //   C   p = A_panel v with v^T A_panel v alongside, which gives
+        //       p'.v = tau (v^T A v - 2 x.y) and so K = -tau p'.v / 2 without a
+        //       further pass
+        constexpr int LPR = PW / VW;      // lanes per panel row
+        constexpr int RPI = 64 / LPR;     // rows per wave instruction
+        const int g = lane % LPR, rsub = lane / LPR;
+        T tau_prev = T(0), K_prev = T(0);
+        // group sum over the LPR lanes of one row
+        auto row_sum = [&](T x) {
+#pragma unroll
+            for (int off = 1; off < LPR; off <<= 1) x += __shfl_xor(x, off, 64);
+            return x;
+        };
+        // W(r, i-1) from the lane's panel-row vector xr (entries updated in
+        // place); returns nothing, writes the finished entry to P
+        auto finish_w = [&](VT& xr, int r, int icol) {
+            const T* xe = reinterpret_cast<const T*>(&xr);
+            T part = T(0);
+#pragma unroll
+            for (int q = 0; q < VW; ++q) part += xe[q] * sdot[((g * VW + q) + kTriNB) & (PW - 1)];
+            const T corr = row_sum(part);
+            const T wnew = tau_prev * (sp[r] - corr) + K_prev * sv[r];
+            const int cw = kTriNB + icol - g * VW;  // lane-local index of W(:, icol), if this lane holds it
+            if (cw >= 0 && cw < VW) {
+                T* xw = reinterpret_cast<T*>(&xr);
+#pragma unroll
+                for (int q = 0; q < VW; ++q)
+                    if (q == cw) xw[q] = wnew;
+                P[size_t(r) * PW + kTriNB + icol] = wnew;
+            }
+        };
         for (int i = 0; i < nbp; ++i) {
             const int j = j0 + i;
-            __syncthreads();
-            if (tid < kTriNB) {
-                srow[tid] = (tid < i) ? P[size_t(j) * PW + kTriNB + tid] : T(0);
-                srow[kTriNB + tid] = (tid < i) ? P[size_t(j) * PW + tid] : T(0);
+            // ---- A0: srow = panel row j (W(j, i-1) finished)
+            if (wave == 0 && rsub == 0) {
+                VT xr = *reinterpret_cast<const VT*>(P + size_t(j) * PW + g * VW);
+                if (i > 0) finish_w(xr, j, i - 1);
+                *reinterpret_cast<VT*>(srow + g * VW) = xr;
             }
             __syncthreads();
-            // column j with the panel's pending updates, rows j..n-1 (panel
-            // columns l < i, whole 16-byte vectors)
-            const int iv = (i + VW - 1) / VW * VW;
-            T sig = T(0);
-            for (int r = j + tid; r < n; r += kTriNT) {
-                T a = A[size_t(j) * ld + r];
-                const T* pr = P + size_t(r) * PW;
-                for (int l = 0; l < iv; l += VW) {
-                    const VT v4 = *reinterpret_cast<const VT*>(pr + l);
-                    const VT w4 = *reinterpret_cast<const VT*>(pr + kTriNB + l);
-                    const T* vv = reinterpret_cast<const T*>(&v4);
-                    const T* ww = reinterpret_cast<const T*>(&w4);
+            // ---- A: fused pass over rows j..n-1
+            {
+                T coefc[VW];
 #pragma unroll
-                    for (int q = 0; q < VW; ++q) a -= vv[q] * srow[l + q] + ww[q] * srow[kTriNB + l + q];
+                for (int q = 0; q < VW; ++q) coefc[q] = srow[((g * VW + q) + kTriNB) & (PW - 1)];
+                T rd[VW];
+#pragma unroll
+                for (int q = 0; q < VW; ++q) rd[q] = T(0);
+                T sig = T(0);
+                constexpr int kU = 4;
+                for (int rb = j + wave * RPI; rb < n; rb += kU * kTriNW * RPI) {
+                    VT xr[kU];
+                    T av[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int r = rb + u * kTriNW * RPI + rsub;
+                        xr[u] = (r < n) ? *reinterpret_cast<const VT*>(P + size_t(r) * PW + g * VW) : VT{};
+                        av[u] = (r < n) ? A[size_t(j) * ld + r] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int r = rb + u * kTriNW * RPI + rsub;
+                        const bool live = r < n;
+                        if (i > 0 && live) finish_w(xr[u], r, i - 1);
+                        const T* xe = reinterpret_cast<const T*>(&xr[u]);
+                        T part = T(0);
+#pragma unroll
+                        for (int q = 0; q < VW; ++q) part += xe[q] * coefc[q];
+                        const T uu = av[u] - row_sum(part);
+                        if (live && r >= j + 2) {
+#pragma unroll
+                            for (int q = 0; q < VW; ++q) rd[q] += xe[q] * uu;
+                            if (g == 0) sig += uu * uu;
+                        }
+                        if (live && g == 0) sp[r] = uu;
+                    }
                 }
-                sp[r] = a;
-                if (r >= j + 2) sig += a * a;
+                // partials: rows of a wave instruction, then waves
+#pragma unroll
+                for (int q = 0; q < VW; ++q)
+                    for (int off = LPR; off < 64; off <<= 1) rd[q] += __shfl_xor(rd[q], off, 64);
+                if (rsub == 0) {
+#pragma unroll
+                    for (int q = 0; q < VW; ++q) sdp[wave * PW + g * VW + q] = rd[q];
+                }
+                sig = wave_sum(sig);
+                if (lane == 0) sred[wave] = sig;
             }
-            sig = block_sum1(sig, sred);
+            __syncthreads();
+            // ---- B: the reflector
+            T sig = T(0);
+#pragma unroll
+            for (int w = 0; w < kTriNW; ++w) sig += sred[w];
             const T alpha = sp[j + 1];
             const T dj = sp[j];
             T tj = T(0), beta = alpha, scal = T(0);
@@ -260,6 +338,14 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 beta = (alpha >= T(0)) ? -nrm : nrm;
                 tj = (beta - alpha) / beta;
                 scal = T(1) / (alpha - beta);
+            }
+            if (tid < PW) {
+                // x_l = W(:, l)^T v, y_l = V(:, l)^T v over the formed columns l < i
+                T raw = T(0);
+#pragma unroll
+                for (int w = 0; w < kTriNW; ++w) raw += sdp[w * PW + tid];
+                const int l = tid & (kTriNB - 1);
+                sdot[tid] = (l < i) ? scal * raw + P[size_t(j + 1) * PW + tid] : T(0);
             }
             for (int r = j + 1 + tid; r < n; r += kTriNT) {
                 const T vr = (r == j + 1) ? T(1) : sp[r] * scal;
@@ -274,21 +360,22 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 tau[j] = tj;
             }
             __syncthreads();
-            const int r0 = j + 1;
-            if (tj == T(0)) continue;  // W(:, i) stays zero
-            // p = A_panel(r0:, r0:) v from the lower triangle only: each wave
+            // ---- C: p = A_panel v from the lower triangle only: each wave
             // takes strips of kStrip columns; a lane holds a 16-byte vector of
             // rows of a (64*VW) x kStrip tile, adds the row part
             // sum_c A(r,c) v_c to p_r with LDS atomics and keeps the
             // transposed part sum_r A(r,c) v_r (r > c) of every column in a
             // register, reduced over the wave when the strip ends.  Rows above
             // the diagonal are read (the tile is aligned) and masked.
+            T Kj = T(0);
+            if (tj != T(0)) {
+                T vav = T(0);
             {
                 constexpr int kStrip = strip_cols<T>();
                 const __amdgpu_buffer_rsrc_t arsrc = matrix_rsrc(A, uint32_t(size_t(n) * ld * sizeof(T)));
-                for (int r = r0 + tid; r < n; r += kTriNT) sp[r] = T(0);
+                for (int r = j + 1 + tid; r < n; r += kTriNT) sp[r] = T(0);
                 __syncthreads();
-                for (int c = r0 + wave * kStrip; c < n; c += kTriNW * kStrip) {
+                for (int c = j + 1 + wave * kStrip; c < n; c += kTriNW * kStrip) {
                     const int nc = min(kStrip, n - c);
                     T dotc[kStrip];
 #pragma unroll
@@ -340,8 +427,10 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                             }
                         }
 #pragma unroll
-                        for (int q = 0; q < VW; ++q)
+                        for (int q = 0; q < VW; ++q) {
+                            vav += vr[q] * rowp[q];
                             if (r + q < n && r + q >= c) atomicAdd(&sp[r + q], rowp[q]);
+                        }
                     };
                     // the first tile holds the diagonal; the loads of the next
                     // tile are in flight while a tile is consumed
@@ -365,58 +454,37 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                     }
 #pragma unroll
                     for (int k = 0; k < kStrip; ++k) {
+                        if (k < nc) vav += sv[c + k] * dotc[k];
                         const T dk = wave_sum(dotc[k]);
                         if (lane == k && k < nc) atomicAdd(&sp[c + k], dk);
                     }
                 }
             }
-            // sdot = (V^T v, W^T v): lane l of each wave sums P(r, l) v_r over
-            // the rows r = r0 + wave + kTriNW t (columns not formed yet are
-            // zero), right after the wave's strips of the product
-            {
-                T acc = T(0);
-                int r = r0 + wave;
-                for (; r + 7 * kTriNW < n; r += 8 * kTriNW) {
-                    T pv[8], vv[8];
+                vav = wave_sum(vav);
+                if (lane == 0) sred2[wave] = vav;
+                __syncthreads();  // the product's LDS atomics and the v^T A v partials
+                T vAv = T(0);
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) {
-                        pv[t] = P[size_t(r + t * kTriNW) * PW + lane];
-                        vv[t] = sv[r + t * kTriNW];
-                    }
-#pragma unroll
-                    for (int t = 0; t < 8; ++t) acc += pv[t] * vv[t];
+                for (int w = 0; w < kTriNW; ++w) vAv += sred2[w];
+                T xy = T(0);
+                for (int l = 0; l < kTriNB; ++l) xy += sdot[kTriNB + l] * sdot[l];
+                Kj = T(-0.5) * tj * tj * (vAv - T(2) * xy);
+            }
+            tau_prev = tj;
+            K_prev = Kj;
+        }
+        // finish W(:, nbp - 1) before the trailing update
+        {
+            const int jl = j0 + nbp - 1;
+            for (int rb = jl + 1 + wave * RPI; rb < n; rb += kTriNW * RPI) {
+                const int r = rb + rsub;
+                // a row group (LPR lanes) is live or dead as a whole, so the
+                // row sums never read an inactive lane
+                if (r < n) {
+                    VT xr = *reinterpret_cast<const VT*>(P + size_t(r) * PW + g * VW);
+                    finish_w(xr, r, nbp - 1);
                 }
-                for (; r < n; r += kTriNW) acc += P[size_t(r) * PW + lane] * sv[r];
-                sdp[wave * PW + lane] = acc;
             }
-            __syncthreads();  // the product's LDS atomics and the dot partials
-            if (tid < PW) {
-                T t = T(0);
-#pragma unroll
-                for (int w = 0; w < kTriNW; ++w) t += sdp[w * PW + tid];
-                sdot[tid] = t;
-            }
-            __syncthreads();
-            // p <- tau (p - V (W^T v) - W (V^T v)), w = p - (tau/2)(p.v) v
-            T wv = T(0);
-            for (int r = r0 + tid; r < n; r += kTriNT) {
-                T pr_ = sp[r];
-                const T* pr = P + size_t(r) * PW;
-                for (int l = 0; l < iv; l += VW) {
-                    const VT v4 = *reinterpret_cast<const VT*>(pr + l);
-                    const VT w4 = *reinterpret_cast<const VT*>(pr + kTriNB + l);
-                    const T* vv = reinterpret_cast<const T*>(&v4);
-                    const T* ww = reinterpret_cast<const T*>(&w4);
-#pragma unroll
-                    for (int q = 0; q < VW; ++q) pr_ -= vv[q] * sdot[kTriNB + l + q] + ww[q] * sdot[l + q];
-                }
-                pr_ *= tj;
-                sp[r] = pr_;
-                wv += pr_ * sv[r];
-            }
-            wv = block_sum1(wv, sred);
-            const T K = T(-0.5) * tj * wv;
-            for (int r = r0 + tid; r < n; r += kTriNT) P[size_t(r) * PW + kTriNB + i] = sp[r] + K * sv[r];
         }
         // trailing update: A(jn:, jn:) -= V W^T + W V^T on lower tiles
         const int jn = j0 + nbp;
