@@ -44,6 +44,7 @@ constexpr int kBtNB = 128;   // reflectors per compact-WY block in the back-tran
 constexpr int kEigNT = 256;  // threads per block of bisection / inverse iteration
 constexpr double kClusterRel = 1e-7;
 constexpr int kMaxCluster = 128;
+constexpr int kSingleMinN = 512;  // one matrix at least this large runs the whole-GPU tridiagonalisation
 
 // Eigenvalue k joins the cluster of k-1 (ascending w).
 __device__ __forceinline__ bool joins_cluster(const double* w, int k) {
@@ -110,19 +111,19 @@ inline size_t tridiag_smem_elems(int n) {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // Trailing update of the FP32 tridiagonalisation, A(jn:, jn:) -= V W^T + W V^T
-// on the lower triangle, with MFMA 32x32x2 and no LDS.  Wave w takes the
-// 32-row blocks br = w, w + kTriNW, ... and every block column bc <= br.  The
+// on the lower triangle, with MFMA 32x32x2 and no LDS.  A wave takes the
+// 32-row blocks br = br0, br0 + brstep, ... and every block column bc <= br.  The
 // block product is formed transposed (MFMA row index = column of A, MFMA column
 // index = row of A) so that lanes run along rows of the column-major A.  Step s
 // of the K = 64 sum takes k = 32 h + s from lane half h: the b operand is the
 // panel row (V(r, .), W(r, .)) and the a operand the panel row of the column
 // with its halves swapped, (W(c, .), V(c, .)).
 __device__ void trailing_update_f32(int n, int ld, int jn, float* __restrict__ A, const float* __restrict__ P,
-                                    int lane, int wave) {
+                                    int lane, int br0, int brstep) {
     constexpr int PW = 2 * kTriNB;
     const int nblk = (n - jn + 31) / 32;
     const int h = lane >> 5, l32 = lane & 31;
-    for (int br = wave; br < nblk; br += kTriNW) {
+    for (int br = br0; br < nblk; br += brstep) {
         const int r = jn + br * 32 + l32;
         float bv[32];
         {
@@ -394,7 +395,7 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                         T vr[VW], rowp[VW], vcs[kStrip];
 #pragma unroll
                         for (int q = 0; q < VW; ++q) {
-                            vr[q] = (r + q < n) ? sv[r + q] : T(0);
+                            vr[q] = (r + q < n && r + q > j) ? sv[r + q] : T(0);  // rows <= j hold stale v
                             rowp[q] = T(0);
                         }
                         // v over the strip's columns (zero past n), re-read
@@ -490,7 +491,7 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
         const int jn = j0 + nbp;
         __syncthreads();
         if constexpr (sizeof(T) == 4) {
-            if (jn < n) trailing_update_f32(n, ld, jn, A, P, lane, wave);
+            if (jn < n) trailing_update_f32(n, ld, jn, A, P, lane, wave, kTriNW);
         } else if (jn < n) {
             const int mt = n - jn;
             const int ntile = (mt + 63) / 64;
@@ -549,6 +550,404 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
         __syncthreads();
         if (tid == 0) A[size_t(n - 1) * ld + (n - 1)] = T(0);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Single-matrix tridiagonalisation across the whole GPU (batch == 1: the
+// reference control flow diagonalises one Hamiltonian at a time).  Same
+// factorisation, panel layout and per-reflector steps as tridiag_kernel, with
+// each step a grid of workgroups instead of one workgroup's phases:
+//   tri1_rowpass    rows split over workgroups: finish W(:, i-1), next column
+//                   u, partial panel dots and partial sum of u^2 (with FIN:
+//                   only finish W(:, i-1), for the last column of a panel)
+//   tri1_reflector  one workgroup: the reflector, v, x = W^T v, y = V^T v
+//   tri1_symv       (row tile, strip group) items of the lower triangle:
+//                   p = A_panel v and v^T A v with global atomics
+//   tri1_update_*   the panel's trailing update over row blocks
+// Global state between the launches: u (gu), v (gv), p (gp), x/y (gsdot),
+// tau and v^T A v (gstate), per-workgroup partial sums (gpart).
+// ---------------------------------------------------------------------------
+constexpr int kS1NT = 256;    // threads per workgroup of the row pass, product and update
+constexpr int kS1Rows = 64;   // matrix rows per row-pass workgroup
+constexpr int kS1Group = 2;   // column strips per product item
+
+inline size_t tri1_elems(int n) {
+    return 3 * size_t(n) + 2 * kTriNB + 4 + size_t((n + kS1Rows - 1) / kS1Rows) * (2 * kTriNB + 1);
+}
+
+template <typename T, bool FIN>
+__global__ void __launch_bounds__(kS1NT)
+tri1_rowpass(int n, int ld, int j, int i, const T* __restrict__ A, T* __restrict__ P, const T* __restrict__ gp,
+             const T* __restrict__ gv, const T* __restrict__ gsdot, const T* __restrict__ gstate,
+             T* __restrict__ gu, T* __restrict__ gpart) {
+    using VT = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n;
+    constexpr int PW = 2 * kTriNB;
+    constexpr int LPR = PW / VW, RPI = 64 / LPR, NWV = kS1NT / 64;
+    constexpr int NI = kS1Rows / (NWV * RPI);  // row instructions per lane
+    __shared__ T sd[PW], srow[PW], spart[NWV][PW], ssig[NWV];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane % LPR, rsub = lane / LPR;
+    if (tid < PW) sd[tid] = gsdot[tid];
+    __syncthreads();
+    T tau_prev = T(0), K_prev = T(0);
+    if (i > 0) {
+        tau_prev = gstate[0];
+        T xy = T(0);
+        for (int l = 0; l < kTriNB; ++l) xy += sd[kTriNB + l] * sd[l];
+        K_prev = T(-0.5) * tau_prev * tau_prev * (gstate[1] - T(2) * xy);
+    }
+    auto row_sum = [&](T x) {
+#pragma unroll
+        for (int off = 1; off < LPR; off <<= 1) x += __shfl_xor(x, off, 64);
+        return x;
+    };
+    auto finish_w = [&](VT& xr, int r, bool write) {
+        const T* xe = reinterpret_cast<const T*>(&xr);
+        T part = T(0);
+#pragma unroll
+        for (int q = 0; q < VW; ++q) part += xe[q] * sd[((g * VW + q) + kTriNB) & (PW - 1)];
+        const T corr = row_sum(part);
+        const T wnew = tau_prev * (gp[r] - corr) + K_prev * gv[r];
+        const int cw = kTriNB + i - 1 - g * VW;
+        if (cw >= 0 && cw < VW) {
+            T* xw = reinterpret_cast<T*>(&xr);
+#pragma unroll
+            for (int q = 0; q < VW; ++q)
+                if (q == cw) xw[q] = wnew;
+            if (write) P[size_t(r) * PW + kTriNB + i - 1] = wnew;
+        }
+    };
+    if (!FIN) {
+        if (wave == 0 && rsub == 0) {
+            VT xr = *reinterpret_cast<const VT*>(P + size_t(j) * PW + g * VW);
+            if (i > 0) finish_w(xr, j, false);
+            *reinterpret_cast<VT*>(srow + g * VW) = xr;
+        }
+        __syncthreads();
+    }
+    T coefc[VW], rd[VW];
+#pragma unroll
+    for (int q = 0; q < VW; ++q) {
+        coefc[q] = FIN ? T(0) : srow[((g * VW + q) + kTriNB) & (PW - 1)];
+        rd[q] = T(0);
+    }
+    T sig = T(0);
+    const int r0 = j + blockIdx.x * kS1Rows;
+    VT xr[NI];
+    T av[NI];
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+        const int r = r0 + (t * NWV + wave) * RPI + rsub;
+        xr[t] = (r < n) ? *reinterpret_cast<const VT*>(P + size_t(r) * PW + g * VW) : VT{};
+        av[t] = (!FIN && r < n) ? A[size_t(j) * ld + r] : T(0);
+    }
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+        const int r = r0 + (t * NWV + wave) * RPI + rsub;
+        // a row group (LPR lanes) is live or dead as a whole, so the row sums
+        // never read an inactive lane
+        if (r < n) {
+            if (i > 0) finish_w(xr[t], r, true);
+            if (!FIN) {
+                const T* xe = reinterpret_cast<const T*>(&xr[t]);
+                T part = T(0);
+#pragma unroll
+                for (int q = 0; q < VW; ++q) part += xe[q] * coefc[q];
+                const T u = av[t] - row_sum(part);
+                if (r >= j + 2) {
+#pragma unroll
+                    for (int q = 0; q < VW; ++q) rd[q] += xe[q] * u;
+                    if (g == 0) sig += u * u;
+                }
+                if (g == 0) gu[r] = u;
+            }
+        }
+    }
+    if (FIN) return;
+#pragma unroll
+    for (int q = 0; q < VW; ++q)
+        for (int off = LPR; off < 64; off <<= 1) rd[q] += __shfl_xor(rd[q], off, 64);
+    if (rsub == 0) {
+#pragma unroll
+        for (int q = 0; q < VW; ++q) spart[wave][g * VW + q] = rd[q];
+    }
+    sig = wave_sum(sig);
+    if (lane == 0) ssig[wave] = sig;
+    __syncthreads();
+    T* out = gpart + size_t(blockIdx.x) * (PW + 1);
+    if (tid < PW) {
+        T s = T(0);
+#pragma unroll
+        for (int w = 0; w < NWV; ++w) s += spart[w][tid];
+        out[tid] = s;
+    }
+    if (tid == PW) {
+        T s = T(0);
+#pragma unroll
+        for (int w = 0; w < NWV; ++w) s += ssig[w];
+        out[PW] = s;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+tri1_reflector(int n, int ld, int j, int i, int nblk, T* __restrict__ A, T* __restrict__ P, const T* __restrict__ gu,
+               T* __restrict__ gv, T* __restrict__ gp, T* __restrict__ gsdot, T* __restrict__ gstate,
+               const T* __restrict__ gpart, T* __restrict__ d, T* __restrict__ e, T* __restrict__ tau) {
+    constexpr int PW = 2 * kTriNB;
+    __shared__ T red[PW + 1];
+    const int tid = threadIdx.x;
+    if (tid < PW + 1) {
+        T s = T(0);
+        for (int b = 0; b < nblk; ++b) s += gpart[size_t(b) * (PW + 1) + tid];
+        red[tid] = s;
+    }
+    __syncthreads();
+    const T sig = red[PW];
+    const T alpha = gu[j + 1];
+    const T dj = gu[j];
+    T tj = T(0), beta = alpha, scal = T(0);
+    if (sig != T(0)) {
+        const T nrm = sqrt(alpha * alpha + sig);
+        beta = (alpha >= T(0)) ? -nrm : nrm;
+        tj = (beta - alpha) / beta;
+        scal = T(1) / (alpha - beta);
+    }
+    if (tid < PW) {
+        // x_l = W(:, l)^T v, y_l = V(:, l)^T v over the formed columns l < i
+        const int l = tid & (kTriNB - 1);
+        gsdot[tid] = (l < i) ? scal * red[tid] + P[size_t(j + 1) * PW + tid] : T(0);
+    }
+    for (int r = j + 1 + tid; r < n; r += 1024) {
+        const T vr = (r == j + 1) ? T(1) : gu[r] * scal;
+        gv[r] = vr;
+        A[size_t(j) * ld + r] = vr;
+        P[size_t(r) * PW + i] = vr;
+        gp[r] = T(0);
+    }
+    for (int r = tid; r <= j; r += 1024) A[size_t(j) * ld + r] = T(0);
+    if (tid == 0) {
+        d[j] = dj;
+        e[j] = beta;
+        tau[j] = tj;
+        gstate[0] = tj;
+        gstate[1] = T(0);
+    }
+}
+
+// Item w of the product: row tile t = w % ntile (64*VW rows from the aligned
+// start of row j+1), strip group sg = w / ntile (kS1Group strips of kStrip
+// columns from column j+1).  The row part of a tile is summed over the
+// group's strips in registers before its atomics.
+template <typename T>
+__global__ void __launch_bounds__(kS1NT)
+tri1_symv(int n, int ld, int j, int ntile, int nitems, const T* __restrict__ A, const T* __restrict__ gv,
+          T* __restrict__ gp, T* __restrict__ gstate) {
+    using VT = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n;
+    constexpr int kStrip = strip_cols<T>();
+    constexpr int kTile = 64 * VW;
+    if (gstate[0] == T(0)) return;  // tau = 0: no product needed
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * (kS1NT / 64) + (int(threadIdx.x) >> 6));
+    if (w >= nitems) return;
+    const int t = w % ntile, sg = w / ntile;
+    const int rt = ((j + 1) & ~(VW - 1)) + t * kTile;
+    const int r = rt + lane * VW;
+    const __amdgpu_buffer_rsrc_t arsrc = matrix_rsrc(A, uint32_t(size_t(n) * ld * sizeof(T)));
+    const int loff = (r < n) ? r * int(sizeof(T)) : kOffNone;
+    T vr[VW], rowp[VW];
+#pragma unroll
+    for (int q = 0; q < VW; ++q) {
+        vr[q] = (r + q < n && r + q > j) ? gv[r + q] : T(0);
+        rowp[q] = T(0);
+    }
+    T vav = T(0);
+    for (int s = 0; s < kS1Group; ++s) {
+        const int c = j + 1 + (sg * kS1Group + s) * kStrip;
+        if (c >= n || rt + kTile <= c) break;  // later strips lie further right
+        const int nc = min(kStrip, n - c);
+        VT a[kStrip];
+#pragma unroll
+        for (int k = 0; k < kStrip; ++k)
+            a[k] = buffer_load16<VT>(arsrc, loff, min(c + k, n - 1) * ld * int(sizeof(T)));
+        T vcs[kStrip], dotc[kStrip];
+#pragma unroll
+        for (int k = 0; k < kStrip; ++k) {
+            vcs[k] = (k < nc) ? gv[c + k] : T(0);
+            dotc[k] = T(0);
+        }
+        if (rt < c + kStrip) {
+#pragma unroll
+            for (int k = 0; k < kStrip; ++k) {
+                const T* ak = reinterpret_cast<const T*>(&a[k]);
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    const int rr = r + q;
+                    const T val = (rr >= c + k) ? ak[q] : T(0);
+                    rowp[q] += val * vcs[k];
+                    dotc[k] += (rr > c + k) ? val * vr[q] : T(0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kStrip; ++k) {
+                const T* ak = reinterpret_cast<const T*>(&a[k]);
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    rowp[q] += ak[q] * vcs[k];
+                    dotc[k] += ak[q] * vr[q];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kStrip; ++k) {
+            vav += vcs[k] * dotc[k];
+            const T dk = wave_sum(dotc[k]);
+            if (lane == k && k < nc) atomicAdd(&gp[c + k], dk);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < VW; ++q) {
+        vav += vr[q] * rowp[q];
+        if (r + q < n && r + q > j && rowp[q] != T(0)) atomicAdd(&gp[r + q], rowp[q]);
+    }
+    vav = wave_sum(vav);
+    if (lane == 0) atomicAdd(&gstate[1], vav);
+}
+
+// FP32 trailing update: wave w of the grid takes row blocks w, w + nwaves, ...
+__global__ void __launch_bounds__(kS1NT)
+tri1_update_f32(int n, int ld, int jn, float* __restrict__ A, const float* __restrict__ P) {
+    const int lane = threadIdx.x & 63;
+    const int w = int(blockIdx.x) * (kS1NT / 64) + (int(threadIdx.x) >> 6);
+    trailing_update_f32(n, ld, jn, A, P, lane, w, int(gridDim.x) * (kS1NT / 64));
+}
+
+// FP64 trailing update: one 64 x 64 lower tile per workgroup (blockIdx.x =
+// row tile, blockIdx.y = column tile), panel tiles staged in LDS.
+inline size_t tri1_update_f64_smem() { return size_t(2) * 2 * kTriNB * kTileLd * sizeof(double); }
+
+__global__ void __launch_bounds__(kS1NT)
+tri1_update_f64(int n, int ld, int jn, int nbp, double* __restrict__ A, const double* __restrict__ P) {
+    constexpr int PW = 2 * kTriNB;
+    const int tr = blockIdx.x, tc = blockIdx.y;
+    if (tc > tr) return;
+    extern __shared__ __align__(16) unsigned char smem_u[];
+    double* const sL = reinterpret_cast<double*>(smem_u);
+    double* const sR = sL + PW * kTileLd;
+    const int tid = threadIdx.x;
+    const int rbase = jn + tr * 64, cbase = jn + tc * 64;
+    for (int q = tid; q < PW * 64; q += kS1NT) {
+        const int xx = q / PW, l = q % PW;
+        const int r = rbase + xx, c = cbase + xx;
+        sL[l * kTileLd + xx] = (r < n) ? P[size_t(r) * PW + l] : 0.0;
+        const int ls = l < kTriNB ? l + kTriNB : l - kTriNB;
+        sR[ls * kTileLd + xx] = (c < n) ? P[size_t(c) * PW + l] : 0.0;
+    }
+    __syncthreads();
+    constexpr int NWV = kS1NT / 64, kCols = 64 / NWV;
+    const int x = tid & 63, yb = tid >> 6;
+    const int r = rbase + x;
+    if (r >= n) return;
+    double a[kCols];
+#pragma unroll
+    for (int q = 0; q < kCols; ++q) {
+        const int c = cbase + yb + NWV * q;
+        a[q] = (c < n && r >= c) ? A[size_t(c) * ld + r] : 0.0;
+    }
+    for (int l = 0; l < 2 * nbp; ++l) {
+        const int lr = l < nbp ? l : l - nbp + kTriNB;
+        const double lv = sL[lr * kTileLd + x];
+#pragma unroll
+        for (int q = 0; q < kCols; ++q) a[q] -= lv * sR[lr * kTileLd + yb + NWV * q];
+    }
+#pragma unroll
+    for (int q = 0; q < kCols; ++q) {
+        const int c = cbase + yb + NWV * q;
+        if (c < n && r >= c) A[size_t(c) * ld + r] = a[q];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kS1NT) tri1_clear(int n, int j0, T* __restrict__ P) {
+    constexpr int PW = 2 * kTriNB;
+    for (size_t q = size_t(j0) * PW + size_t(blockIdx.x) * kS1NT + threadIdx.x; q < size_t(n) * PW;
+         q += size_t(gridDim.x) * kS1NT)
+        P[q] = T(0);
+}
+
+template <typename T>
+__global__ void tri1_last(int n, int ld, T* __restrict__ A, T* __restrict__ d, T* __restrict__ e,
+                          T* __restrict__ tau) {
+    if (threadIdx.x == 0) {
+        d[n - 1] = A[size_t(n - 1) * ld + (n - 1)];
+        e[n - 1] = T(0);
+        tau[n - 1] = T(0);
+        A[size_t(n - 1) * ld + (n - 1)] = T(0);
+    }
+}
+
+template <typename T>
+int tridiag_single(int n, int ld, T* A, T* d, T* e, T* tau, T* P, T* buf, hipStream_t s) {
+    constexpr int VW = Vec16<T>::n;
+    constexpr int kStrip = strip_cols<T>();
+    constexpr int kTile = 64 * VW;
+    if constexpr (sizeof(T) == 8)
+        CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&tri1_update_f64),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(tri1_update_f64_smem())));
+    T* gu = buf;
+    T* gv = gu + n;
+    T* gp = gv + n;
+    T* gsdot = gp + n;
+    T* gstate = gsdot + 2 * kTriNB;
+    T* gpart = gstate + 4;
+    for (int j0 = 0; j0 < n - 1; j0 += kTriNB) {
+        const int nbp = std::min(kTriNB, n - 1 - j0);
+        hipLaunchKernelGGL(tri1_clear<T>, dim3(256), dim3(kS1NT), 0, s, n, j0, P);
+        CIR_CHECK_LAUNCH();
+        for (int i = 0; i < nbp; ++i) {
+            const int j = j0 + i;
+            const int g1 = int(ceil_div(n - j, kS1Rows));
+            hipLaunchKernelGGL((tri1_rowpass<T, false>), dim3(g1), dim3(kS1NT), 0, s, n, ld, j, i, A, P, gp, gv,
+                               gsdot, gstate, gu, gpart);
+            CIR_CHECK_LAUNCH();
+            hipLaunchKernelGGL(tri1_reflector<T>, dim3(1), dim3(1024), 0, s, n, ld, j, i, g1, A, P, gu, gv, gp,
+                               gsdot, gstate, gpart, d, e, tau);
+            CIR_CHECK_LAUNCH();
+            if (j + 1 < n) {
+                const int rb0 = (j + 1) & ~(VW - 1);
+                const int ntile = int(ceil_div(n - rb0, kTile));
+                const int nstrip = int(ceil_div(n - (j + 1), kStrip));
+                const int nsg = int(ceil_div(nstrip, kS1Group));
+                const int nitems = ntile * nsg;
+                hipLaunchKernelGGL(tri1_symv<T>, dim3(unsigned(ceil_div(nitems, kS1NT / 64))), dim3(kS1NT), 0, s,
+                                   n, ld, j, ntile, nitems, A, gv, gp, gstate);
+                CIR_CHECK_LAUNCH();
+            }
+        }
+        // the panel's last W column, then the trailing update
+        const int jn = j0 + nbp;
+        if (jn < n) {
+            hipLaunchKernelGGL((tri1_rowpass<T, true>), dim3(unsigned(ceil_div(n - jn, kS1Rows))), dim3(kS1NT), 0, s,
+                               n, ld, jn, nbp, A, P, gp, gv, gsdot, gstate, gu, gpart);
+            CIR_CHECK_LAUNCH();
+            const int nblk = int(ceil_div(n - jn, 32));
+            if constexpr (sizeof(T) == 4) {
+                hipLaunchKernelGGL(tri1_update_f32, dim3(unsigned(ceil_div(nblk, kS1NT / 64))), dim3(kS1NT), 0, s, n,
+                                   ld, jn, reinterpret_cast<float*>(A), reinterpret_cast<const float*>(P));
+            } else {
+                const int nt = int(ceil_div(n - jn, 64));
+                hipLaunchKernelGGL(tri1_update_f64, dim3(nt, nt), dim3(kS1NT), tri1_update_f64_smem(), s, n, ld, jn,
+                                   nbp, reinterpret_cast<double*>(A), reinterpret_cast<const double*>(P));
+            }
+            CIR_CHECK_LAUNCH();
+        }
+    }
+    hipLaunchKernelGGL(tri1_last<T>, dim3(1), dim3(64), 0, s, n, ld, A, d, e, tau);
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
 }
 
 // Number of eigenvalues of T below x (Sturm sequence count).
@@ -820,7 +1219,7 @@ larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __res
 inline size_t larft_smem_bytes(size_t esz) { return (size_t(kBtNB) * (kBtNB + 1) + kBtNB) * esz; }
 
 struct EighWork {
-    size_t off_d, off_e, off_tau, off_vw, off_g, off_t, off_w1, off_w2, off_scr, total;
+    size_t off_d, off_e, off_tau, off_vw, off_g, off_t, off_w1, off_w2, off_s1, off_scr, total;
     int sub;  // matrices per inverse-iteration pass
 };
 
@@ -844,6 +1243,8 @@ inline EighWork eigh_layout(size_t esz, int batch, int n, int ld) {
     o = al(o + esz * batch * kBtNB * size_t(n));
     L.off_w2 = o;
     o = al(o + esz * batch * kBtNB * size_t(n));
+    L.off_s1 = o;  // single-matrix tridiagonalisation state
+    o = al(o + esz * tri1_elems(n));
     const size_t per = invit_scratch_bytes(n);
     const size_t budget = size_t(8) << 30;
     int sub = int(std::max<size_t>(1, budget / per));
@@ -868,13 +1269,17 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     T* W2 = reinterpret_cast<T*>(work + L.off_w2);
     unsigned char* scr = work + L.off_scr;
 
-    // 1. tridiagonalisation
+    // 1. tridiagonalisation: one matrix spreads over the whole GPU, a batch
+    // runs one workgroup per matrix
     if (ld % 4 != 0) return fail(CIR_ERR_ARG, "eigh: the leading dimension must be a multiple of 4");
-    const size_t sm = tridiag_smem_elems(n) * sizeof(T);
-    if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "eigh: n too large for the LDS of one workgroup");
-    CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<T>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
-    {
+    if (batch == 1 && n >= kSingleMinN) {
+        ProfScope ps(prof, CIR_PH_TRIDIAG, s);
+        CIR_TRY(tridiag_single<T>(n, ld, A, dd, ee, tau, vw, reinterpret_cast<T*>(work + L.off_s1), s));
+    } else {
+        const size_t sm = tridiag_smem_elems(n) * sizeof(T);
+        if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "eigh: n too large for the LDS of one workgroup");
+        CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
         ProfScope ps(prof, CIR_PH_TRIDIAG, s);
         hipLaunchKernelGGL(tridiag_kernel<T>, dim3(batch), dim3(kTriNT), sm, s, n, ld, A, dd, ee, tau, vw);
         CIR_CHECK_LAUNCH();

@@ -166,6 +166,34 @@ def test_eigh_c2_size_invariants():
     assert np.abs(Hh @ V - V * w).max() < 5e-3
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_eigh_single_matrix_whole_gpu_path(dtype):
+    # batch 1 with n >= 512 runs the tridiagonalisation as a grid per step
+    # (row pass, reflector, product, trailing update); ragged n and padded ld
+    rng = np.random.default_rng(17)
+    for n in (512, 701, 1000):
+        ld = (n + 3 + 8) // 4 * 4
+        announce("eigh single-matrix path n=%d ld=%d dtype=%s" % (n, ld, dtype))
+        a = rng.standard_normal((n, n))
+        Hn = (a + a.T) * 20.0 + np.diag(rng.uniform(-300, 300, n))
+        Hd = np.zeros((1, n, ld))
+        Hd[0, :, :n] = Hn.T
+        Hc = Hn.astype(np.float32).astype(np.float64) if dtype == torch.float32 else Hn
+        w, V = S.eigh(torch.from_numpy(Hd).to(dtype).to(DEV).contiguous(), n)
+        w = w.cpu().numpy()[0]
+        V = V.cpu().double().numpy()[0][:, :n]
+        wn = np.linalg.eigvalsh(Hc)
+        scale = np.abs(wn).max()
+        eps = 6e-8 if dtype == torch.float32 else 2.2e-16
+        assert np.all(np.diff(w) >= 0)
+        assert np.abs(w - wn).max() <= 50 * eps * scale * np.sqrt(n), np.abs(w - wn).max()
+        assert np.abs(Hc @ V - V * w).max() / scale < 50 * eps * np.sqrt(n)
+        # inverse-iteration orthogonality ~ eps ||T|| / gap: random spectra at
+        # n >= 512 have gaps ~1e-2 ||T|| / n, so FP64 is held to 2000 eps sqrt(n)
+        orth_tol = (50 if dtype == torch.float32 else 2000) * eps * np.sqrt(n)
+        assert np.abs(V.T @ V - np.eye(n)).max() < orth_tol
+
+
 @pytest.mark.parametrize("nwater", [128, 1024])
 def test_eigh_bf16x3_backtransform(nwater):
     # BF16X3 changes only the back-transform products: the eigenvalues are the

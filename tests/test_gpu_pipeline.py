@@ -28,7 +28,7 @@ def _run_pair(prec, mode, oracle_prec=O.F64, nwater=128, nframes=60, seed=1001, 
     return p, g, o
 
 
-def _check(g, o, tol_tcf, tol_spec, tol_dist, dlam=1e-3):
+def _check(g, o, tol_tcf, tol_spec, tol_dist, dlam=1e-3, n=256):
     for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
         assert rel(g[k], o[k]) < tol_tcf, (k, rel(g[k], o[k]))
     for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
@@ -37,7 +37,7 @@ def _check(g, o, tol_tcf, tol_spec, tol_dist, dlam=1e-3):
     # Pw is a count: a mode whose eigenvalue lies within the precision's
     # eigenvalue error dlam (cm^-1) of a bin edge may change bin.  Expected
     # moves ~ mode-steps * 2 dlam / omegaStep; allow 4x that, plus 2.
-    steps = g["nsteps"] * 256  # mode-steps (N = 256 chromophores)
+    steps = g["nsteps"] * n  # mode-steps (n chromophores)
     unit = 1.0 / KW["omegaStep"] / steps
     moves = 0.5 * np.abs(g["pw"] - o["pw"]).sum() / unit
     allowed = 4 * steps * 2 * dlam / KW["omegaStep"] + 2
@@ -111,3 +111,13 @@ def test_pipeline_uncoupled_closed_form():
     assert rel(g["tcf_ir"], ref) < 1e-10
     # C_IR(0) = sum |mu|^2 (real)
     assert abs(g["tcf_ir"][0].imag) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol,dlam", [("fp64", 1e-10, 1e-9), ("fp32", 1e-4, 1e-3)])
+def test_pipeline_plain_whole_gpu_eigensolver(prec, tol, dlam):
+    # N = 512: the plain control flow diagonalises one Hamiltonian at a time,
+    # which runs the tridiagonalisation across the whole GPU (tri1_* kernels)
+    p, g, o = _run_pair(prec, "plain", nwater=256, nframes=20, seed=1003, nsamples=2, ntcfpoints=6,
+                        sampleEvery=0.05)
+    assert g["nsteps"] == 12
+    _check(g, o, tol, tol, tol, dlam=dlam, n=512)
