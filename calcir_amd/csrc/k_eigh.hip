@@ -44,7 +44,9 @@ constexpr int kBtNB = 128;   // reflectors per compact-WY block in the back-tran
 constexpr int kEigNT = 256;  // threads per block of bisection / inverse iteration
 constexpr double kClusterRel = 1e-7;
 constexpr int kMaxCluster = 128;
-constexpr int kSingleMinN = 512;  // one matrix at least this large runs the whole-GPU tridiagonalisation
+constexpr int kSingleMinN = 512;     // matrices at least this large may run the whole-GPU tridiagonalisation:
+constexpr int kSingleMaxBatch = 8;   //   batches of at most this many,
+constexpr int kSingleLargeN = 6144;  //   or matrices at least this large
 
 // Eigenvalue k joins the cluster of k-1 (ascending w).
 __device__ __forceinline__ bool joins_cluster(const double* w, int k) {
@@ -1269,12 +1271,16 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     T* W2 = reinterpret_cast<T*>(work + L.off_w2);
     unsigned char* scr = work + L.off_scr;
 
-    // 1. tridiagonalisation: one matrix spreads over the whole GPU, a batch
-    // runs one workgroup per matrix
+    // 1. tridiagonalisation: a batch runs one workgroup per matrix; a few
+    // matrices, or very large ones (whose batches are small and whose
+    // one-workgroup product is slow), run one at a time across the whole GPU
     if (ld % 4 != 0) return fail(CIR_ERR_ARG, "eigh: the leading dimension must be a multiple of 4");
-    if (batch == 1 && n >= kSingleMinN) {
+    if (n >= kSingleMinN && (batch <= kSingleMaxBatch || n >= kSingleLargeN)) {
         ProfScope ps(prof, CIR_PH_TRIDIAG, s);
-        CIR_TRY(tridiag_single<T>(n, ld, A, dd, ee, tau, vw, reinterpret_cast<T*>(work + L.off_s1), s));
+        T* s1 = reinterpret_cast<T*>(work + L.off_s1);
+        for (int b = 0; b < batch; ++b)
+            CIR_TRY(tridiag_single<T>(n, ld, A + size_t(b) * n * ld, dd + size_t(b) * n, ee + size_t(b) * n,
+                                      tau + size_t(b) * n, vw + size_t(b) * 2 * kTriNB * ld, s1, s));
     } else {
         const size_t sm = tridiag_smem_elems(n) * sizeof(T);
         if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "eigh: n too large for the LDS of one workgroup");

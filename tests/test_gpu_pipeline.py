@@ -121,3 +121,35 @@ def test_pipeline_plain_whole_gpu_eigensolver(prec, tol, dlam):
                         sampleEvery=0.05)
     assert g["nsteps"] == 12
     _check(g, o, tol, tol, tol, dlam=dlam, n=512)
+
+
+def test_pipeline_8192_oh_box_against_gpu_fp64():
+    # 4096 waters (N = 8192, H 256 MiB per frame): the CPU oracle cannot
+    # diagonalise this in test time, so the FP64 pipeline (pinned to the
+    # oracle above) is the reference; FP32 within 1e-4, BF16x3 within 1e-3
+    nwater, nframes = 4096, 3
+    xyz, box, times = frames(nwater, nframes, 1005)
+    kw = dict(KW, nsamples=1, ntcfpoints=3, sampleEvery=0.01)
+    res = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        announce("pipeline N=8192 prec=%s" % prec)
+        res[prec] = C.run(C.default_params(precision=prec, mode="batched", **kw), xyz, box, times)
+    ref = res["fp64"]
+    assert ref["nsteps"] == 3 and ref["nframes_used"] == 3
+    for prec, tol in (("fp32", 1e-4), ("bf16x3", 1e-3)):
+        g = res[prec]
+        for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+            assert rel(g[k], ref[k]) < tol, (prec, k, rel(g[k], ref[k]))
+        for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
+            assert rel_l2(g[k], ref[k]) < tol, (prec, k, rel_l2(g[k], ref[k]))
+        assert rel(g["sw"], ref["sw"]) < tol
+
+
+@pytest.mark.parametrize("nwater", [50, 64, 333])
+def test_pipeline_uneven_box_sizes(nwater):
+    # boxes of uneven (and ragged: N = 100, 666) sizes, as in a batch of
+    # independent boxes: batched FP32 within 1e-4 of the FP64 oracle
+    p, g, o = _run_pair("fp32", "batched", nwater=nwater, nframes=24, seed=3000 + nwater, nsamples=2,
+                        ntcfpoints=8, sampleEvery=0.05)
+    assert g["nsteps"] == 16
+    _check(g, o, 1e-4, 1e-4, 1e-4, n=2 * nwater)
