@@ -97,6 +97,9 @@ SIGNATURES = {
     "cir_write_outputs": (_i, [ctypes.POINTER(Params), ctypes.POINTER(Results)]),
     "cir_synth_box_nm": (_f, [_i]),
     "cir_synth_trajectory": (_i, [_i, _i, ctypes.c_uint64, _f, _vp, _vp]),
+    "cir_xtc_scan": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "cir_xtc_read": (_i, [ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "cir_xtc_write": (_i, [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _f, _i]),
     "cir_main": (_i, [_i, ctypes.POINTER(ctypes.c_char_p)]),
     "cir_device_count": (_i, [ctypes.POINTER(_i)]),
     "cir_create": (_i, [ctypes.POINTER(Params), _i, _vp, ctypes.POINTER(_vp)]),
@@ -214,6 +217,38 @@ def synth_trajectory(nwater, nframes, seed, box_nm=None):
     check(lib().cir_synth_trajectory(nwater, nframes, seed, L, xyz.ctypes.data, times.ctypes.data))
     box = np.full((nframes, 3), L, np.float32)
     return xyz, box, times
+
+
+def xtc_scan(path):
+    """(natoms, nframes) of a GROMACS XTC file."""
+    na, nf = ctypes.c_int(), ctypes.c_int()
+    check(lib().cir_xtc_scan(os.fsencode(path), ctypes.byref(na), ctypes.byref(nf)))
+    return na.value, nf.value
+
+
+def xtc_read(path, first=0, count=None):
+    """Frames of an XTC file: (xyz [count, natoms, 3] nm, box diagonal
+    [count, 3], times [count] ps, steps [count])."""
+    natoms, nframes = xtc_scan(path)
+    count = nframes - first if count is None else count
+    xyz = np.zeros((count, natoms, 3), np.float32)
+    box = np.zeros((count, 3), np.float32)
+    times = np.zeros(count, np.float32)
+    steps = np.zeros(count, np.int32)
+    check(lib().cir_xtc_read(os.fsencode(path), first, count, natoms, xyz.ctypes.data, box.ctypes.data,
+                             times.ctypes.data, steps.ctypes.data))
+    return xyz, box, times, steps
+
+
+def xtc_write(path, xyz, box, times, precision=1000.0, append=False):
+    """Writes frames (xyz [nframes, natoms, 3] nm, box diagonal [nframes, 3],
+    times [nframes] ps) as XTC with the given precision."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    box = np.ascontiguousarray(box, np.float32)
+    times = np.ascontiguousarray(times, np.float32)
+    nframes, natoms = xyz.shape[0], xyz.shape[1]
+    check(lib().cir_xtc_write(os.fsencode(path), natoms, nframes, xyz.ctypes.data, box.ctypes.data,
+                              times.ctypes.data, float(precision), int(bool(append))))
 
 
 def _results_buffers(p):

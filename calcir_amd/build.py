@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libcalcir.so")
 CLI = os.path.join(HERE, "bin", "calcir")
 
 HIP_SOURCES = ["k_sites.hip", "k_gemm.hip", "k_eigh.hip", "k_spectra.hip", "pipeline.hip"]
-CXX_SOURCES = ["common.cpp", "host_io.cpp", "synth.cpp", "cli.cpp"]
+CXX_SOURCES = ["common.cpp", "host_io.cpp", "synth.cpp", "xtc.cpp", "cli.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,

@@ -2,8 +2,9 @@
 // 765-1011) over the libcalcir pipeline.
 //
 // Usage: calcir <input-file>.  The trajectory key xtcf names the frame
-// source; "synth:<nwater>:<nframes>:<seed>[:<box_nm>]" generates a seeded
-// synthetic rigid-water trajectory (frames 10 fs apart, from t = 0).
+// source: a GROMACS XTC file, or "synth:<nwater>:<nframes>:<seed>[:<box_nm>]"
+// for a seeded synthetic rigid-water trajectory (frames 10 fs apart, from
+// t = 0).
 
 #include <hip/hip_runtime.h>
 
@@ -62,22 +63,37 @@ extern "C" int cir_main(int argc, char** argv) {
         return EXIT_FAILURE;
     }
 
-    int nwater = 0, nframes = 0;
+    int nwater = 0, nframes = 0, natoms = 0;
     unsigned long long seed = 0;
     float boxl = 0.0f;
-    if (!parse_synth(p.xtcf, &nwater, &nframes, &seed, &boxl)) {
-        std::printf("WARNING: The file %s could not be opened. Is the name correct?\n", p.xtcf);
-        std::printf("(this build reads the synthetic source \"synth:<nwater>:<nframes>:<seed>[:<box_nm>]\")\n");
-        return EXIT_FAILURE;
+    std::vector<float> xyz, times, box;
+    if (parse_synth(p.xtcf, &nwater, &nframes, &seed, &boxl)) {
+        natoms = 4 * nwater;
+        if (p.natom_mol != 4) {
+            std::printf("WARNING: the synthetic source writes 4-site waters; natom_mol must be 4.\n");
+            return EXIT_FAILURE;
+        }
+        xyz.resize(size_t(nframes) * natoms * 3);
+        times.resize(nframes);
+        box.assign(size_t(nframes) * 3, boxl);
+        st = cir_synth_trajectory(nwater, nframes, seed, boxl, xyz.data(), times.data());
+        if (st != CIR_OK) return fail_msg(st);
+    } else {
+        // calcIR.cu:212-219: the trajectory must open and hold frames
+        if (cir_xtc_scan(p.xtcf, &natoms, &nframes) != CIR_OK || nframes < 1) {
+            std::printf("WARNING: The file %s could not be opened. Is the name correct?\n", p.xtcf);
+            return EXIT_FAILURE;
+        }
+        if (nframes < 2) {
+            std::printf("WARNING: %s holds fewer than two frames.\n", p.xtcf);
+            return EXIT_FAILURE;
+        }
+        xyz.resize(size_t(nframes) * natoms * 3);
+        times.resize(nframes);
+        box.resize(size_t(nframes) * 3);
+        st = cir_xtc_read(p.xtcf, 0, nframes, natoms, xyz.data(), box.data(), times.data(), nullptr);
+        if (st != CIR_OK) return fail_msg(st);
     }
-    const int natoms = 4 * nwater;
-    if (p.natom_mol != 4) {
-        std::printf("WARNING: the synthetic source writes 4-site waters; natom_mol must be 4.\n");
-        return EXIT_FAILURE;
-    }
-    std::vector<float> xyz(size_t(nframes) * natoms * 3), times(nframes), box(size_t(nframes) * 3, boxl);
-    st = cir_synth_trajectory(nwater, nframes, seed, boxl, xyz.data(), times.data());
-    if (st != CIR_OK) return fail_msg(st);
     const int nmol = natoms / p.natom_mol, nchrom = nmol * p.nchrom_mol;
     std::printf(">>> Will read the trajectory from: %s.\n", p.xtcf);
     std::printf(">>> Found %d atoms and %d molecules.\n", natoms, nmol);

@@ -1,0 +1,568 @@
+// xtc.cpp -- GROMACS XTC trajectory reader and writer (host C++), the frame
+// source the reference gets from xdrfile (read_xtc_natoms, read_xtc,
+// read_xtc_n_frames, xdr_seek; calcIR.cu:212-219, 341-395, calcIR.h:16-18).
+//
+// A frame is XDR (big-endian) data:
+//   int magic (1995), int natoms, int step, float time, float box[3][3],
+//   then the coordinates: int natoms; natoms <= 9: 3*natoms floats; else
+//   float precision, int minint[3], int maxint[3], int smallidx, int nbytes
+//   and nbytes of bit-packed integers (padded to 4 bytes).
+// The bit packing is the published xdr3dfcoord scheme: coordinates rounded to
+// integers at the given precision, offsets from minint packed as one mixed-
+// radix number per atom, and runs of atoms close to their predecessor coded
+// as small differences whose radix (magicints[smallidx]) adapts along the
+// frame; the first two atoms of a run are exchanged, which favours water.
+// The magicints table is part of the format.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "calcir.h"
+#include "common.hpp"
+
+namespace cir {
+namespace {
+
+constexpr int kXtcMagic = 1995;
+constexpr int kFirstIdx = 9;
+// radices of the small-difference code (format constant)
+constexpr int kMagicInts[] = {
+    0,        0,        0,       0,       0,       0,       0,       0,       0,       8,       10,      12,
+    16,       20,       25,      32,      40,      50,      64,      80,      101,     128,     161,     203,
+    256,      322,      406,     512,     645,     812,     1024,    1290,    1625,    2048,    2580,    3250,
+    4096,     5060,     6501,    8192,    10321,   13003,   16384,   20642,   26007,   32768,   41285,   52015,
+    65536,    82570,    104031,  131072,  165140,  208063,  262144,  330280,  416127,  524287,  660561,  832255,
+    1048576,  1321122,  1664510, 2097152, 2642245, 3329021, 4194304, 5284491, 6658042, 8388607, 10568983,
+    13316085, 16777216};
+constexpr int kLastIdx = int(sizeof(kMagicInts) / sizeof(kMagicInts[0]));
+
+// ---- XDR primitives -------------------------------------------------------
+struct XdrIn {
+    FILE* f;
+    bool ok = true;
+    uint32_t u32() {
+        unsigned char b[4];
+        if (std::fread(b, 1, 4, f) != 4) {
+            ok = false;
+            return 0;
+        }
+        return (uint32_t(b[0]) << 24) | (uint32_t(b[1]) << 16) | (uint32_t(b[2]) << 8) | uint32_t(b[3]);
+    }
+    int i32() { return int(u32()); }
+    float f32() {
+        const uint32_t u = u32();
+        float x;
+        std::memcpy(&x, &u, 4);
+        return x;
+    }
+    bool bytes(unsigned char* p, size_t n) {
+        if (n && std::fread(p, 1, n, f) != n) return ok = false;
+        const size_t pad = (4 - n % 4) % 4;
+        unsigned char z[4];
+        if (pad && std::fread(z, 1, pad, f) != pad) return ok = false;
+        return true;
+    }
+};
+
+struct XdrOut {
+    std::vector<unsigned char> b;
+    void u32(uint32_t u) {
+        b.push_back((u >> 24) & 0xff);
+        b.push_back((u >> 16) & 0xff);
+        b.push_back((u >> 8) & 0xff);
+        b.push_back(u & 0xff);
+    }
+    void i32(int v) { u32(uint32_t(v)); }
+    void f32(float x) {
+        uint32_t u;
+        std::memcpy(&u, &x, 4);
+        u32(u);
+    }
+    void bytes(const unsigned char* p, size_t n) {
+        b.insert(b.end(), p, p + n);
+        for (size_t q = n; q % 4; ++q) b.push_back(0);
+    }
+};
+
+// ---- bit stream -------------------------------------------------------------
+struct BitReader {
+    const unsigned char* p;
+    size_t n, cnt = 0;
+    unsigned int lastbits = 0, lastbyte = 0;
+    bool ok = true;
+    unsigned int next_byte() {
+        if (cnt >= n) {
+            ok = false;
+            return 0;
+        }
+        return p[cnt++];
+    }
+    int bits(int nbits) {
+        const unsigned int mask = (nbits >= 32) ? 0xffffffffu : ((1u << nbits) - 1);
+        unsigned int num = 0;
+        while (nbits >= 8) {
+            lastbyte = (lastbyte << 8) | next_byte();
+            num |= (lastbyte >> lastbits) << (nbits - 8);
+            nbits -= 8;
+        }
+        if (nbits > 0) {
+            if (int(lastbits) < nbits) {
+                lastbits += 8;
+                lastbyte = (lastbyte << 8) | next_byte();
+            }
+            lastbits -= nbits;
+            num |= (lastbyte >> lastbits) & ((1u << nbits) - 1);
+        }
+        return int(num & mask);
+    }
+    // three integers packed as one mixed-radix number of nbits bits
+    void ints3(int nbits, const unsigned int sizes[3], int out[3]) {
+        int bytes[32] = {0};
+        int nb = 0;
+        while (nbits > 8) {
+            bytes[nb++] = bits(8);
+            nbits -= 8;
+        }
+        if (nbits > 0) bytes[nb++] = bits(nbits);
+        for (int i = 2; i > 0; --i) {
+            unsigned int num = 0;
+            for (int j = nb - 1; j >= 0; --j) {
+                num = (num << 8) | unsigned(bytes[j]);
+                const unsigned int q = num / sizes[i];
+                bytes[j] = int(q);
+                num -= q * sizes[i];
+            }
+            out[i] = int(num);
+        }
+        out[0] = bytes[0] | (bytes[1] << 8) | (bytes[2] << 16) | (bytes[3] << 24);
+    }
+};
+
+struct BitWriter {
+    std::vector<unsigned char> buf;
+    unsigned int lastbits = 0, lastbyte = 0;
+    void bits(int nbits, unsigned int num) {
+        while (nbits >= 8) {
+            lastbyte = (lastbyte << 8) | ((num >> (nbits - 8)) & 0xff);
+            buf.push_back((unsigned char)(lastbyte >> lastbits));
+            nbits -= 8;
+        }
+        if (nbits > 0) {
+            lastbyte = (lastbyte << nbits) | (num & ((1u << nbits) - 1));
+            lastbits += nbits;
+            if (lastbits >= 8) {
+                lastbits -= 8;
+                buf.push_back((unsigned char)(lastbyte >> lastbits));
+            }
+        }
+    }
+    void flush() {
+        if (lastbits > 0) buf.push_back((unsigned char)(lastbyte << (8 - lastbits)));
+        lastbits = 0;
+    }
+    void ints3(int nbits, const unsigned int sizes[3], const unsigned int nums[3]) {
+        unsigned int bytes[32];
+        int nb = 0;
+        unsigned int tmp = nums[0];
+        do {
+            bytes[nb++] = tmp & 0xff;
+            tmp >>= 8;
+        } while (tmp != 0);
+        for (int i = 1; i < 3; ++i) {
+            tmp = nums[i];
+            int bc = 0;
+            for (; bc < nb; ++bc) {
+                tmp = bytes[bc] * sizes[i] + tmp;
+                bytes[bc] = tmp & 0xff;
+                tmp >>= 8;
+            }
+            while (tmp != 0) {
+                bytes[bc++] = tmp & 0xff;
+                tmp >>= 8;
+            }
+            nb = bc;
+        }
+        if (nbits >= nb * 8) {
+            for (int i = 0; i < nb; ++i) bits(8, bytes[i]);
+            bits(nbits - nb * 8, 0);
+        } else {
+            int i = 0;
+            for (; i < nb - 1; ++i) bits(8, bytes[i]);
+            bits(nbits - (nb - 1) * 8, bytes[i]);
+        }
+    }
+};
+
+int size_of_int(unsigned int size) {
+    unsigned int num = 1;
+    int nbits = 0;
+    while (size >= num && nbits < 32) {
+        ++nbits;
+        num <<= 1;
+    }
+    return nbits;
+}
+
+// bits needed for the mixed-radix number of three integers below sizes[]
+int size_of_ints3(const unsigned int sizes[3]) {
+    unsigned int bytes[32];
+    int nb = 1;
+    bytes[0] = 1;
+    for (int i = 0; i < 3; ++i) {
+        unsigned int tmp = 0;
+        int bc = 0;
+        for (; bc < nb; ++bc) {
+            tmp = bytes[bc] * sizes[i] + tmp;
+            bytes[bc] = tmp & 0xff;
+            tmp >>= 8;
+        }
+        while (tmp != 0) {
+            bytes[bc++] = tmp & 0xff;
+            tmp >>= 8;
+        }
+        nb = bc;
+    }
+    unsigned int num = 1;
+    int nbits = 0;
+    --nb;
+    while (bytes[nb] >= num) {
+        ++nbits;
+        num *= 2;
+    }
+    return nbits + nb * 8;
+}
+
+#pragma clang fp contract(off)
+float round_half_away(float x, float precision) {
+    float lf = x * precision;
+    lf += (lf >= 0.0f) ? 0.5f : -0.5f;
+    return lf;
+}
+
+// ---- frame coordinates --------------------------------------------------------
+bool read_coords(XdrIn& x, int natoms, float* out) {
+    const int lsize = x.i32();
+    if (!x.ok || lsize != natoms) return false;
+    if (natoms <= 9) {
+        for (int q = 0; q < 3 * natoms; ++q) out[q] = x.f32();
+        return x.ok;
+    }
+    const float precision = x.f32();
+    int minint[3], maxint[3];
+    for (int k = 0; k < 3; ++k) minint[k] = x.i32();
+    for (int k = 0; k < 3; ++k) maxint[k] = x.i32();
+    unsigned int sizeint[3];
+    int bitsizeint[3] = {0, 0, 0}, bitsize = 0;
+    for (int k = 0; k < 3; ++k) sizeint[k] = unsigned(maxint[k] - minint[k]) + 1;
+    if ((sizeint[0] | sizeint[1] | sizeint[2]) > 0xffffff) {
+        for (int k = 0; k < 3; ++k) bitsizeint[k] = size_of_int(sizeint[k]);
+    } else {
+        bitsize = size_of_ints3(sizeint);
+    }
+    int smallidx = x.i32();
+    const int nbytes = x.i32();
+    if (!x.ok || smallidx < kFirstIdx || smallidx >= kLastIdx || nbytes < 0) return false;
+    std::vector<unsigned char> data(size_t(nbytes) + 8, 0);
+    if (!x.bytes(data.data(), size_t(nbytes))) return false;
+    BitReader br{data.data(), size_t(nbytes)};
+    int smaller = kMagicInts[std::max(kFirstIdx, smallidx - 1)] / 2;
+    int smallnum = kMagicInts[smallidx] / 2;
+    unsigned int sizesmall[3] = {unsigned(kMagicInts[smallidx]), unsigned(kMagicInts[smallidx]),
+                                 unsigned(kMagicInts[smallidx])};
+    const float inv = 1.0f / precision;
+    int i = 0, run = 0;
+    float* o = out;
+    while (i < natoms) {
+        int thiscoord[3], prevcoord[3];
+        if (bitsize == 0) {
+            for (int k = 0; k < 3; ++k) thiscoord[k] = br.bits(bitsizeint[k]);
+        } else {
+            br.ints3(bitsize, sizeint, thiscoord);
+        }
+        ++i;
+        for (int k = 0; k < 3; ++k) thiscoord[k] += minint[k];
+        for (int k = 0; k < 3; ++k) prevcoord[k] = thiscoord[k];
+        int is_smaller = 0;
+        if (br.bits(1) == 1) {
+            run = br.bits(5);
+            is_smaller = run % 3;
+            run -= is_smaller;
+            --is_smaller;
+        }
+        if (run > 0) {
+            if (i + run / 3 > natoms) return false;  // run past the end of the frame
+            for (int k = 0; k < run; k += 3) {
+                br.ints3(smallidx, sizesmall, thiscoord);
+                ++i;
+                for (int c = 0; c < 3; ++c) thiscoord[c] += prevcoord[c] - smallnum;
+                if (k == 0) {
+                    // the writer exchanged the first two atoms of the run
+                    for (int c = 0; c < 3; ++c) std::swap(thiscoord[c], prevcoord[c]);
+                    for (int c = 0; c < 3; ++c) *o++ = float(prevcoord[c]) * inv;
+                } else {
+                    for (int c = 0; c < 3; ++c) prevcoord[c] = thiscoord[c];
+                }
+                for (int c = 0; c < 3; ++c) *o++ = float(thiscoord[c]) * inv;
+            }
+        } else {
+            for (int c = 0; c < 3; ++c) *o++ = float(thiscoord[c]) * inv;
+        }
+        smallidx += is_smaller;
+        if (smallidx < kFirstIdx || smallidx >= kLastIdx) return false;
+        if (is_smaller < 0) {
+            smallnum = smaller;
+            smaller = (smallidx > kFirstIdx) ? kMagicInts[smallidx - 1] / 2 : 0;
+        } else if (is_smaller > 0) {
+            smaller = smallnum;
+            smallnum = kMagicInts[smallidx] / 2;
+        }
+        sizesmall[0] = sizesmall[1] = sizesmall[2] = unsigned(kMagicInts[smallidx]);
+    }
+    return br.ok && (o - out) == 3 * natoms;
+}
+
+bool write_coords(XdrOut& x, int natoms, const float* fp, float precision) {
+    x.i32(natoms);
+    if (natoms <= 9) {
+        for (int q = 0; q < 3 * natoms; ++q) x.f32(fp[q]);
+        return true;
+    }
+    x.f32(precision);
+    std::vector<int> ip(3 * size_t(natoms) + 3);
+    int minint[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, maxint[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    int mindiff = INT32_MAX;
+    const float maxabs = float(INT32_MAX) - 256.0f;
+    for (int a = 0; a < natoms; ++a) {
+        for (int k = 0; k < 3; ++k) {
+            // rounded in FP32, as the format's writers do: x*precision is
+            // one float product, then +-0.5 and truncation
+            const float lf = round_half_away(fp[3 * a + k], precision);
+            if (!(std::fabs(lf) <= maxabs)) return false;
+            const int li = int(lf);
+            ip[3 * a + k] = li;
+            minint[k] = std::min(minint[k], li);
+            maxint[k] = std::max(maxint[k], li);
+        }
+        if (a > 0) {
+            const long long diff = std::llabs((long long)ip[3 * a] - ip[3 * a - 3]) +
+                                   std::llabs((long long)ip[3 * a + 1] - ip[3 * a - 2]) +
+                                   std::llabs((long long)ip[3 * a + 2] - ip[3 * a - 1]);
+            if (diff < mindiff) mindiff = int(std::min<long long>(diff, INT32_MAX));
+        }
+    }
+    for (int k = 0; k < 3; ++k) x.i32(minint[k]);
+    for (int k = 0; k < 3; ++k) x.i32(maxint[k]);
+    unsigned int sizeint[3];
+    int bitsizeint[3] = {0, 0, 0}, bitsize = 0;
+    for (int k = 0; k < 3; ++k) {
+        if ((long long)maxint[k] - minint[k] >= 0x7fffffffLL) return false;
+        sizeint[k] = unsigned(maxint[k] - minint[k]) + 1;
+    }
+    if ((sizeint[0] | sizeint[1] | sizeint[2]) > 0xffffff) {
+        for (int k = 0; k < 3; ++k) bitsizeint[k] = size_of_int(sizeint[k]);
+    } else {
+        bitsize = size_of_ints3(sizeint);
+    }
+    int smallidx = kFirstIdx;
+    while (smallidx < kLastIdx - 1 && kMagicInts[smallidx] < mindiff) ++smallidx;
+    x.i32(smallidx);
+    const int maxidx = std::min(kLastIdx - 1, smallidx + 8);
+    const int minidx = maxidx - 8;
+    int smaller = kMagicInts[std::max(kFirstIdx, smallidx - 1)] / 2;
+    int smallnum = kMagicInts[smallidx] / 2;
+    unsigned int sizesmall[3] = {unsigned(kMagicInts[smallidx]), unsigned(kMagicInts[smallidx]),
+                                 unsigned(kMagicInts[smallidx])};
+    const int larger = kMagicInts[maxidx] / 2;
+    BitWriter bw;
+    int i = 0, prevrun = -1;
+    int prevcoord[3] = {0, 0, 0};
+    auto absdiff = [](int a, int b) { return std::llabs((long long)a - b); };
+    while (i < natoms) {
+        int is_small = 0, is_smaller;
+        int* thiscoord = &ip[3 * size_t(i)];
+        if (smallidx < maxidx && i >= 1 && absdiff(thiscoord[0], prevcoord[0]) < larger &&
+            absdiff(thiscoord[1], prevcoord[1]) < larger && absdiff(thiscoord[2], prevcoord[2]) < larger)
+            is_smaller = 1;
+        else if (smallidx > minidx)
+            is_smaller = -1;
+        else
+            is_smaller = 0;
+        if (i + 1 < natoms && absdiff(thiscoord[0], thiscoord[3]) < smallnum &&
+            absdiff(thiscoord[1], thiscoord[4]) < smallnum && absdiff(thiscoord[2], thiscoord[5]) < smallnum) {
+            // exchange the first two atoms of the run (water: H then O)
+            for (int c = 0; c < 3; ++c) std::swap(thiscoord[c], thiscoord[3 + c]);
+            is_small = 1;
+        }
+        unsigned int tmp[3];
+        for (int c = 0; c < 3; ++c) tmp[c] = unsigned(thiscoord[c] - minint[c]);
+        if (bitsize == 0) {
+            for (int c = 0; c < 3; ++c) bw.bits(bitsizeint[c], tmp[c]);
+        } else {
+            bw.ints3(bitsize, sizeint, tmp);
+        }
+        for (int c = 0; c < 3; ++c) prevcoord[c] = thiscoord[c];
+        thiscoord += 3;
+        ++i;
+        int run = 0;
+        unsigned int runv[8 * 3];
+        if (is_small == 0 && is_smaller == -1) is_smaller = 0;
+        while (is_small && run < 8 * 3) {
+            if (is_smaller == -1) {
+                const long long d0 = (long long)thiscoord[0] - prevcoord[0];
+                const long long d1 = (long long)thiscoord[1] - prevcoord[1];
+                const long long d2 = (long long)thiscoord[2] - prevcoord[2];
+                if (d0 * d0 + d1 * d1 + d2 * d2 >= (long long)smaller * smaller) is_smaller = 0;
+            }
+            for (int c = 0; c < 3; ++c) runv[run++] = unsigned(thiscoord[c] - prevcoord[c] + smallnum);
+            for (int c = 0; c < 3; ++c) prevcoord[c] = thiscoord[c];
+            ++i;
+            thiscoord += 3;
+            is_small = 0;
+            if (i < natoms && absdiff(thiscoord[0], prevcoord[0]) < smallnum &&
+                absdiff(thiscoord[1], prevcoord[1]) < smallnum && absdiff(thiscoord[2], prevcoord[2]) < smallnum)
+                is_small = 1;
+        }
+        if (run != prevrun || is_smaller != 0) {
+            prevrun = run;
+            bw.bits(1, 1);
+            bw.bits(5, unsigned(run + is_smaller + 1));
+        } else {
+            bw.bits(1, 0);
+        }
+        for (int k = 0; k < run; k += 3) bw.ints3(smallidx, sizesmall, &runv[k]);
+        if (is_smaller != 0) {
+            smallidx += is_smaller;
+            if (is_smaller < 0) {
+                smallnum = smaller;
+                smaller = (smallidx > kFirstIdx) ? kMagicInts[smallidx - 1] / 2 : 0;
+            } else {
+                smaller = smallnum;
+                smallnum = kMagicInts[smallidx] / 2;
+            }
+            sizesmall[0] = sizesmall[1] = sizesmall[2] = unsigned(kMagicInts[smallidx]);
+        }
+    }
+    bw.flush();
+    x.i32(int(bw.buf.size()));
+    x.bytes(bw.buf.data(), bw.buf.size());
+    return true;
+}
+
+// Reads one frame header; returns false at a clean end of file.
+bool read_header(XdrIn& x, int* natoms, int* step, float* time, float box[9]) {
+    const int magic = x.i32();
+    if (!x.ok) return false;
+    if (magic != kXtcMagic) {
+        x.ok = false;
+        return false;
+    }
+    *natoms = x.i32();
+    *step = x.i32();
+    *time = x.f32();
+    for (int q = 0; q < 9; ++q) box[q] = x.f32();
+    return x.ok;
+}
+
+}  // namespace
+}  // namespace cir
+
+using namespace cir;
+
+extern "C" int cir_xtc_scan(const char* path, int* natoms, int* nframes) {
+    if (!path || !natoms || !nframes) return fail(CIR_ERR_ARG, "cir_xtc_scan: null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
+    XdrIn x{f};
+    int n = 0, na = -1;
+    std::vector<float> buf;
+    for (;;) {
+        int nat, step;
+        float t, box[9];
+        if (!read_header(x, &nat, &step, &t, box)) break;
+        if (na >= 0 && nat != na) {
+            std::fclose(f);
+            return fail(CIR_ERR_IO, "xtc: atom count changes between frames");
+        }
+        na = nat;
+        buf.resize(3 * size_t(nat));
+        if (!read_coords(x, nat, buf.data())) {
+            std::fclose(f);
+            return fail(CIR_ERR_IO, "xtc: corrupt coordinates in frame " + std::to_string(n));
+        }
+        ++n;
+    }
+    const bool clean = std::feof(f) != 0;
+    std::fclose(f);
+    if (!clean && n == 0) return fail(CIR_ERR_IO, std::string(path) + " is not an XTC file");
+    *natoms = (na < 0) ? 0 : na;
+    *nframes = n;
+    return CIR_OK;
+}
+
+extern "C" int cir_xtc_read(const char* path, int first, int count, int natoms, float* xyz, float* box,
+                            float* times, int* steps) {
+    if (!path || first < 0 || count < 0 || !xyz) return fail(CIR_ERR_ARG, "cir_xtc_read: bad argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
+    XdrIn x{f};
+    std::vector<float> skip(3 * size_t(natoms));
+    int got = 0;
+    for (int fr = 0; got < count; ++fr) {
+        int nat, step;
+        float t, bx[9];
+        if (!read_header(x, &nat, &step, &t, bx)) {
+            std::fclose(f);
+            return fail(CIR_ERR_FRAME, "xtc: frame " + std::to_string(fr) + " missing");
+        }
+        if (nat != natoms) {
+            std::fclose(f);
+            return fail(CIR_ERR_IO, "xtc: atom count differs from the expected " + std::to_string(natoms));
+        }
+        float* dst = (fr >= first) ? xyz + size_t(got) * 3 * natoms : skip.data();
+        if (!read_coords(x, nat, dst)) {
+            std::fclose(f);
+            return fail(CIR_ERR_IO, "xtc: corrupt coordinates in frame " + std::to_string(fr));
+        }
+        if (fr >= first) {
+            if (box)  // the diagonal, as the reference uses it (calcIR.cu:404-405)
+                for (int k = 0; k < 3; ++k) box[3 * size_t(got) + k] = bx[4 * k];
+            if (times) times[got] = t;
+            if (steps) steps[got] = step;
+            ++got;
+        }
+    }
+    std::fclose(f);
+    return CIR_OK;
+}
+
+extern "C" int cir_xtc_write(const char* path, int natoms, int nframes, const float* xyz, const float* box,
+                             const float* times, float precision, int append) {
+    if (!path || natoms < 0 || nframes < 0 || (nframes > 0 && !xyz) || !(precision > 0.0f))
+        return fail(CIR_ERR_ARG, "cir_xtc_write: bad argument");
+    FILE* f = std::fopen(path, append ? "ab" : "wb");
+    if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path + " for writing");
+    for (int fr = 0; fr < nframes; ++fr) {
+        XdrOut x;
+        x.i32(kXtcMagic);
+        x.i32(natoms);
+        x.i32(fr);
+        x.f32(times ? times[fr] : float(fr));
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) x.f32((r == c && box) ? box[3 * size_t(fr) + r] : 0.0f);
+        if (!write_coords(x, natoms, xyz + size_t(fr) * 3 * natoms, precision)) {
+            std::fclose(f);
+            return fail(CIR_ERR_ARG, "xtc: coordinates out of range at this precision");
+        }
+        if (std::fwrite(x.b.data(), 1, x.b.size(), f) != x.b.size()) {
+            std::fclose(f);
+            return fail(CIR_ERR_IO, std::string("write to ") + path + " failed");
+        }
+    }
+    std::fclose(f);
+    return CIR_OK;
+}

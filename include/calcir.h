@@ -34,6 +34,7 @@
  *   cufftExecC2R x 3                   calcIR.cu:775-827 -> cir_spectrum_c2r
  *   output writers                     calcIR.cu:843-919 -> cir_write_outputs
  *   minImage/mag3/dot3                 calcIR.cu:1457-1476, calcIR.h:79-88 -> cir_minimage
+ *   xdrfile read_xtc* / xdr_seek       calcIR.cu:212-219, 341-395 -> cir_xtc_scan/_read (+ _write)
  */
 #ifndef CALCIR_H
 #define CALCIR_H
@@ -160,6 +161,19 @@ int   cir_write_outputs(const cir_params* p, const cir_results* r);
 float cir_synth_box_nm(int nwater);
 int   cir_synth_trajectory(int nwater, int nframes, uint64_t seed, float box_nm, float* xyz,
                            float* times);
+/* GROMACS XTC trajectories, replacing xdrfile's read_xtc_natoms /
+ * read_xtc_n_frames / read_xtc / xdr_seek (calcIR.cu:212-219, 341-395,
+ * calcIR.h:16-18).  cir_xtc_scan counts atoms and frames.  cir_xtc_read
+ * decodes frames first..first+count-1 into caller arrays: xyz count*natoms*3
+ * nm, box count*3 (the diagonal, as calcIR.cu:404-405 uses it), times count
+ * ps, steps count (box, times, steps may be NULL).  cir_xtc_write writes (or
+ * with append != 0 appends) frames with the given precision (1000 = 0.001 nm,
+ * GROMACS' default) and a diagonal box; frame k gets step k. */
+int   cir_xtc_scan(const char* path, int* natoms, int* nframes);
+int   cir_xtc_read(const char* path, int first, int count, int natoms, float* xyz, float* box, float* times,
+                   int* steps);
+int   cir_xtc_write(const char* path, int natoms, int nframes, const float* xyz, const float* box,
+                    const float* times, float precision, int append);
 /* The drop-in CLI: argv as calcIR.exe (calcIR.cu:9-18). */
 int   cir_main(int argc, char** argv);
 

@@ -1,6 +1,8 @@
 """End-to-end parity of the whole calcIR job on the GPU (cir_run through the
 C ABI) against the CPU oracle's reference control flow."""
 
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -153,3 +155,28 @@ def test_pipeline_uneven_box_sizes(nwater):
                         ntcfpoints=8, sampleEvery=0.05)
     assert g["nsteps"] == 16
     _check(g, o, 1e-4, 1e-4, 1e-4, n=2 * nwater)
+
+
+def test_cli_reads_an_xtc_trajectory(tmp_path):
+    # the drop-in executable on an XTC input (xdrfile's role, calcIR.cu:212-395)
+    # gives the same files as the in-memory pipeline on the decoded frames
+    xyz, box, times = frames(64, 30, 7)
+    traj = tmp_path / "traj.xtc"
+    C.xtc_write(traj, xyz, box, times)
+    xq, bq, tq, _ = C.xtc_read(traj)
+    inp = tmp_path / "input.inp"
+    inp.write_text("xtcf %s\noutf %s\nmodel tip4p2005\nspecies H2O\nntcfpoints 10\nnsamples 2\n"
+                   "sampleEvery 0.1\navef 3400\nomegaStart 2800\nomegaStop 4000\nnzeros 502\n"
+                   % (traj, tmp_path / "cli"))
+    announce("calcir CLI on an xtc file")
+    r = subprocess.run([C.CLI_PATH, str(inp)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "Found 256 atoms and 64 molecules." in r.stdout
+    p = C.read_params(str(inp))
+    p.outf = str(tmp_path / "mem").encode()
+    C.write_outputs(p, C.run(p, xq, bq, tq))
+    for suffix in ("irls", "vvls", "vhls", "irrtcf", "vhitcf", "Pw", "spdn"):
+        a = np.loadtxt(tmp_path / ("cli_%s.dat" % suffix))
+        b = np.loadtxt(tmp_path / ("mem_%s.dat" % suffix))
+        assert a.shape == b.shape, suffix
+        np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6 * np.abs(b).max(), err_msg=suffix)

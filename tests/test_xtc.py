@@ -1,0 +1,119 @@
+"""GROMACS XTC frames (the reference's trajectory source through xdrfile,
+calcIR.cu:212-219, 341-395): the XDR frame layout, the compressed coordinate
+code (round trips at several precisions, the large-range path, water runs),
+frame subsets, appends and corrupt input.
+
+No XTC file written by GROMACS is available in this environment, so the
+decoder is pinned by (a) a byte-exact hand-assembled uncompressed frame and
+(b) round trips through the writer of the same published code; agreement
+with GROMACS-written files is parity unpinned."""
+
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import calcir_amd as C
+
+
+def _uncompressed_frame(natoms, step, time, box, xyz):
+    b = struct.pack(">iiif", 1995, natoms, step, time)
+    b += struct.pack(">9f", *box)
+    b += struct.pack(">i", natoms)
+    b += struct.pack(">%df" % (3 * natoms), *xyz.ravel())
+    return b
+
+
+def test_uncompressed_frame_layout(tmp_path):
+    rng = np.random.default_rng(0)
+    f = tmp_path / "small.xtc"
+    frames = []
+    with open(f, "wb") as fh:
+        for k in range(3):
+            xyz = rng.uniform(-5, 5, (7, 3)).astype(np.float32)
+            box = [2.5 + k, 0.1, 0.2, 0.3, 3.5 + k, 0.4, 0.5, 0.6, 4.5 + k]
+            fh.write(_uncompressed_frame(7, 100 * k, 0.25 * k, box, xyz))
+            frames.append((xyz, box))
+    assert C.xtc_scan(f) == (7, 3)
+    xyz, box, times, steps = C.xtc_read(f)
+    for k in range(3):
+        assert np.array_equal(xyz[k], frames[k][0])          # exact: stored as floats
+        b = frames[k][1]
+        assert np.array_equal(box[k], np.float32([b[0], b[4], b[8]]))  # the diagonal
+    assert np.array_equal(times, np.float32([0.0, 0.25, 0.5]))
+    assert list(steps) == [0, 100, 200]
+
+
+@pytest.mark.parametrize("nwater,precision", [(3, 1000.0), (64, 1000.0), (1024, 1000.0), (256, 100.0),
+                                              (256, 10000.0)])
+def test_round_trip_water(tmp_path, nwater, precision):
+    xyz, box, times = C.synth_trajectory(nwater, 4, 11 + nwater)
+    f = tmp_path / "w.xtc"
+    C.xtc_write(f, xyz, box, times, precision=precision)
+    assert C.xtc_scan(f) == (4 * nwater, 4)
+    x2, b2, t2, s2 = C.xtc_read(f)
+    natoms = 4 * nwater
+    if natoms <= 9:
+        assert np.array_equal(x2, xyz)
+    else:
+        # x*precision rounded half away from zero in FP32, then scaled back
+        lf = xyz * np.float32(precision)
+        q = np.trunc(lf + np.where(lf >= 0, np.float32(0.5), np.float32(-0.5)))
+        assert np.array_equal(x2, q.astype(np.float32) * np.float32(1.0 / precision))
+        assert np.abs(x2 - xyz).max() <= 0.5 / precision + 1e-6
+    assert np.array_equal(b2, box) and np.array_equal(t2, times) and list(s2) == [0, 1, 2, 3]
+
+
+def test_water_runs_compress(tmp_path):
+    # O, H1, H2, M within 0.1 nm of each other: the small-difference runs
+    # must carry most atoms (well under 12 bytes per atom)
+    xyz, box, times = C.synth_trajectory(1024, 1, 5)
+    f = tmp_path / "c.xtc"
+    C.xtc_write(f, xyz, box, times)
+    size = os.path.getsize(f)
+    assert size < 4.5 * 4096, size / 4096
+
+
+def test_large_range_path_and_scattered_atoms(tmp_path):
+    rng = np.random.default_rng(3)
+    # a range of 2e4 nm at precision 1000 exceeds 2^24 integers per axis:
+    # coordinates are coded per axis (no mixed radix)
+    xyz = rng.uniform(-1e4, 1e4, (2, 50, 3)).astype(np.float32)
+    box = np.full((2, 3), 1e4, np.float32)
+    times = np.float32([0.0, 0.002])
+    f = tmp_path / "big.xtc"
+    C.xtc_write(f, xyz, box, times)
+    x2 = C.xtc_read(f)[0]
+    assert np.abs(x2 - xyz).max() <= 0.5 / 1000 + 2e-3  # float spacing at 1e4 nm is ~1e-3
+    # scattered atoms (no runs) in a normal box
+    xyz = rng.uniform(0, 3, (1, 300, 3)).astype(np.float32)
+    C.xtc_write(f, xyz, np.full((1, 3), 3.0, np.float32), np.float32([0.0]))
+    assert np.abs(C.xtc_read(f)[0] - xyz).max() <= 0.5 / 1000 + 1e-6
+
+
+def test_subset_append_and_errors(tmp_path):
+    xyz, box, times = C.synth_trajectory(32, 6, 9)
+    f = tmp_path / "a.xtc"
+    C.xtc_write(f, xyz[:4], box[:4], times[:4])
+    C.xtc_write(f, xyz[4:], box[4:], times[4:], append=True)
+    assert C.xtc_scan(f) == (128, 6)
+    full = C.xtc_read(f)[0]
+    part, _, t, _ = C.xtc_read(f, first=2, count=3)
+    assert np.array_equal(part, full[2:5]) and np.array_equal(t, times[2:5])
+    # truncated file: the frames before the cut are found, the cut one is corrupt
+    data = open(f, "rb").read()
+    g = tmp_path / "cut.xtc"
+    open(g, "wb").write(data[: len(data) - 100])
+    with pytest.raises(C.CalcirError):
+        C.xtc_scan(g)
+    with pytest.raises(C.CalcirError) as e:
+        C.xtc_read(f, first=4, count=3)
+    assert e.value.status == 6  # a missing frame, as a missing desired time
+    h = tmp_path / "junk.xtc"
+    open(h, "wb").write(b"not an xtc file at all")
+    with pytest.raises(C.CalcirError):
+        C.xtc_scan(h)
+    with pytest.raises(C.CalcirError) as e:
+        C.xtc_scan(tmp_path / "missing.xtc")
+    assert e.value.status == 2
