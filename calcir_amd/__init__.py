@@ -34,6 +34,7 @@ MODE_PLAIN, MODE_BATCHED = 0, 1
 NSITE = 20
 SITE_E, SITE_W, SITE_X, SITE_P, SITE_MUP, SITE_U, SITE_R, SITE_MU, SITE_A = 0, 1, 2, 3, 4, 5, 8, 11, 14
 STRLEN = 256
+INCOMPLETE = 12  # CIR_INCOMPLETE: stopped at max_frames after writing a checkpoint
 
 
 class CalcirError(RuntimeError):
@@ -57,7 +58,8 @@ class Params(ctypes.Structure):
         ("dt", ctypes.c_float), ("t1", ctypes.c_float), ("avef", ctypes.c_float),
         ("beginTime", ctypes.c_float), ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float),
         ("cutoff", ctypes.c_float), ("precision", ctypes.c_int), ("mode", ctypes.c_int),
-        ("chunk", ctypes.c_int),
+        ("chunk", ctypes.c_int), ("ckpt_every", ctypes.c_int), ("resume", ctypes.c_int),
+        ("max_frames", ctypes.c_int),
     ]
 
 
@@ -155,12 +157,15 @@ def _setstr(p, field, value):
 
 def default_params(**kw):
     """Defaults of calcIR.cu:36-59, then keyword overrides (input-file key names;
-    ``map`` may be given for ``imap``; precision/mode accept their strings)."""
+    ``map`` may be given for ``imap``, ``checkpoint_every`` for ``ckpt_every``;
+    precision/mode accept their strings)."""
     p = Params()
     lib().cir_params_default(ctypes.byref(p))
     for k, v in kw.items():
         if k == "map":
             k = "imap"
+        if k == "checkpoint_every":
+            k = "ckpt_every"
         if k == "precision" and isinstance(v, str):
             v = {"fp32": PREC_FP32, "fp64": PREC_FP64, "bf16x3": PREC_BF16X3}[v]
         if k == "mode" and isinstance(v, str):

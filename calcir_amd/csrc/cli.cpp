@@ -28,6 +28,10 @@ bool parse_synth(const char* spec, int* nwater, int* nframes, unsigned long long
     return true;
 }
 
+// Exit status of a run stopped at max_frames (distinct from EXIT_FAILURE so
+// that a job script can loop on it).
+constexpr int kExitIncomplete = 3;
+
 int fail_msg(int st) {
     std::printf("ERROR: %s: %s\n", cir_strerror(st), cir_last_error());
     return EXIT_FAILURE;
@@ -107,6 +111,12 @@ extern "C" int cir_main(int argc, char** argv) {
     st = cir_create(&p, natoms, nullptr, &ctx);
     if (st != CIR_OK) return fail_msg(st);
     st = cir_run(ctx, nframes, xyz.data(), box.data(), times.data());
+    if (st == CIR_INCOMPLETE) {
+        // max_frames reached: the checkpoint holds the state; no output files
+        std::printf("\n>>> %s\n>>> Run again with resume 1 to continue.\n", cir_last_error());
+        cir_destroy(ctx);
+        return kExitIncomplete;
+    }
     if (st != CIR_OK) {
         cir_destroy(ctx);
         return fail_msg(st);

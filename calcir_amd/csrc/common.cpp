@@ -70,6 +70,7 @@ extern "C" const char* cir_strerror(int status) {
         case CIR_ERR_EIGH: return "eigensolver failed";
         case CIR_ERR_NONFINITE: return "non-finite coordinate or box";
         case CIR_ERR_NODEVICE: return "no HIP device available";
+        case CIR_INCOMPLETE: return "stopped at max_frames; resume from the checkpoint";
         default: return "unknown status";
     }
 }

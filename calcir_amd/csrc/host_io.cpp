@@ -53,6 +53,9 @@ extern "C" void cir_params_default(cir_params* p) {
     p->precision = CIR_PREC_FP32;
     p->mode = CIR_MODE_BATCHED;
     p->chunk = 0;
+    p->ckpt_every = 0;
+    p->resume = 0;
+    p->max_frames = 0;
 }
 
 extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
@@ -91,6 +94,9 @@ extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
         // extensions
         else if (!std::strcmp(para, "cutoff")) std::sscanf(value, "%f", &p->cutoff);
         else if (!std::strcmp(para, "chunk")) std::sscanf(value, "%d", &p->chunk);
+        else if (!std::strcmp(para, "checkpoint_every")) std::sscanf(value, "%d", &p->ckpt_every);
+        else if (!std::strcmp(para, "resume")) std::sscanf(value, "%d", &p->resume);
+        else if (!std::strcmp(para, "max_frames")) std::sscanf(value, "%d", &p->max_frames);
         else if (!std::strcmp(para, "precision")) {
             if (!std::strcmp(value, "fp64")) p->precision = CIR_PREC_FP64;
             else if (!std::strcmp(value, "bf16x3")) p->precision = CIR_PREC_BF16X3;
@@ -129,6 +135,10 @@ extern "C" void cir_params_echo(const cir_params* p) {
     std::printf("\tSetting t1 to %f (ps)\n", p->t1);
     std::printf("\tSetting avef to %f\n", p->avef);
     std::printf("\tSetting equilibration time to %f (ps)\n", p->beginTime);
+    // extension keys are echoed only when set, so a reference input echoes as before
+    if (p->ckpt_every > 0 || p->resume != 0 || p->max_frames > 0)
+        std::printf("\tCheckpoint %s_ckpt.bin: every %d frames, resume %d, max_frames %d\n", p->cptf, p->ckpt_every,
+                    p->resume, p->max_frames);
 }
 
 extern "C" int cir_resolve_model(const char* model, int* imodel) {

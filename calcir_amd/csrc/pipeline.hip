@@ -15,6 +15,7 @@
 //     forms of calcIR.cu:628-745.
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -240,6 +241,124 @@ int auto_chunk(const cir_ctx* c, int nuniq) {
     return std::max(1, std::min(B, nuniq));
 }
 
+// ---- checkpoint / resume (batched mode) ------------------------------------
+// File <cptf>_ckpt.bin: a header (magic, job key, index of the next distinct
+// frame, section sizes), then the propagation state Y (N x 18 nslot reals),
+// the TCF accumulators (nsamples x ntcfpoints x 6 doubles), the IPR and
+// spectral-density accumulators (4 nomega doubles) and the out-of-window
+// count.  The job key hashes the parameters, sizes and frame times; a file
+// written for another job is refused.  Writes go to a temporary name and are
+// renamed into place.
+struct CkptHeader {
+    char magic[8];
+    uint64_t key;
+    int64_t next;
+    int64_t ybytes, outbytes, accbytes;
+};
+constexpr char kCkptMagic[8] = {'C', 'I', 'R', 'C', 'K', 'P', 'T', '1'};
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t q = 0; q < n; ++q) h = (h ^ b[q]) * 1099511628211ull;
+    }
+    template <typename V>
+    void val(const V& v) {
+        add(&v, sizeof(v));
+    }
+};
+
+// The key covers the parameters, the sizes, every frame time and the
+// coordinates of the first and last distinct frames (read back from the
+// device; hashing every frame would cost a full trajectory copy).
+int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d, const float* times,
+            uint64_t* out) {
+    const cir_params& p = c->p;
+    Fnv f;
+    f.val(c->N);
+    f.val(c->natoms);
+    f.val(c->esz);
+    f.val(c->imodel);
+    f.val(c->ispecies);
+    f.val(p.precision);
+    f.val(p.ntcfpoints);
+    f.val(p.nsamples);
+    f.val(p.sampleEvery);
+    f.val(p.omegaStep);
+    f.val(p.natom_mol);
+    f.val(p.nchrom_mol);
+    f.val(p.nzeros);
+    f.val(p.imap);
+    f.val(p.dt);
+    f.val(p.t1);
+    f.val(p.avef);
+    f.val(p.beginTime);
+    f.val(p.omegaStart);
+    f.val(p.omegaStop);
+    f.val(p.cutoff);
+    f.val(S.nslot);
+    f.val(int64_t(S.uniq.size()));
+    f.val(nframes);
+    f.add(times, sizeof(float) * size_t(nframes));
+    const size_t fb = size_t(c->natoms) * 3 * sizeof(float);
+    std::vector<unsigned char> fr(fb);
+    for (int f0 : {S.uniq.front(), S.uniq.back()}) {
+        CIR_HIP(hipMemcpyAsync(fr.data(), reinterpret_cast<const char*>(xyz_d) + size_t(f0) * fb, fb,
+                               hipMemcpyDeviceToHost, c->stream));
+        CIR_HIP(hipStreamSynchronize(c->stream));
+        f.add(fr.data(), fb);
+    }
+    *out = f.h;
+    return CIR_OK;
+}
+
+std::string ckpt_path(const cir_params& p) { return std::string(p.cptf) + "_ckpt.bin"; }
+
+int ckpt_write(const std::string& path, uint64_t key, int64_t next, const std::vector<unsigned char>& y,
+               const std::vector<double>& out, const std::vector<double>& acc, long long skip) {
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.key = key;
+    h.next = next;
+    h.ybytes = int64_t(y.size());
+    h.outbytes = int64_t(out.size() * sizeof(double));
+    h.accbytes = int64_t(acc.size() * sizeof(double));
+    const std::string tmp = path + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return fail(CIR_ERR_IO, "could not open " + tmp + " for the checkpoint");
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+    ok = ok && std::fwrite(y.data(), 1, y.size(), f) == y.size();
+    ok = ok && std::fwrite(out.data(), sizeof(double), out.size(), f) == out.size();
+    ok = ok && std::fwrite(acc.data(), sizeof(double), acc.size(), f) == acc.size();
+    ok = ok && std::fwrite(&skip, sizeof(skip), 1, f) == 1;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) return fail(CIR_ERR_IO, "checkpoint write to " + path + " failed");
+    return CIR_OK;
+}
+
+int ckpt_read(const std::string& path, uint64_t key, int64_t* next, std::vector<unsigned char>& y,
+              std::vector<double>& out, std::vector<double>& acc, long long* skip) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return fail(CIR_ERR_IO, "resume: no checkpoint " + path);
+    CkptHeader h{};
+    bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, kCkptMagic, 8) == 0;
+    if (ok && h.key != key) {
+        std::fclose(f);
+        return fail(CIR_ERR_ARG, "resume: " + path + " was written for another job (parameters or frames differ)");
+    }
+    ok = ok && h.ybytes == int64_t(y.size()) && h.outbytes == int64_t(out.size() * sizeof(double)) &&
+         h.accbytes == int64_t(acc.size() * sizeof(double));
+    ok = ok && std::fread(y.data(), 1, y.size(), f) == y.size();
+    ok = ok && std::fread(out.data(), sizeof(double), out.size(), f) == out.size();
+    ok = ok && std::fread(acc.data(), sizeof(double), acc.size(), f) == acc.size();
+    ok = ok && std::fread(skip, sizeof(*skip), 1, f) == 1;
+    std::fclose(f);
+    if (!ok) return fail(CIR_ERR_IO, "resume: " + path + " is not a complete checkpoint");
+    *next = h.next;
+    return CIR_OK;
+}
+
 int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
     const cir_params& p = c->p;
     Schedule S;
@@ -291,7 +410,26 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     void* Ycur = Y0.p;
     void* Ynext = Y1.p;
     Prof* pf = &c->prof;
-    for (int c0 = 0; c0 < nu; c0 += B) {
+    const bool use_ckpt = p.ckpt_every > 0 || p.resume != 0 || p.max_frames > 0;
+    uint64_t key = 0;
+    if (use_ckpt) CIR_TRY(job_key(c, S, nframes, xyz_d, times, &key));
+    int start = 0;
+    if (p.resume) {
+        std::vector<unsigned char> hy(Y0.bytes);
+        std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom));
+        long long hs = 0;
+        int64_t next = 0;
+        CIR_TRY(ckpt_read(ckpt_path(p), key, &next, hy, ho, ha, &hs));
+        if (next < 0 || next > nu) return fail(CIR_ERR_IO, "resume: checkpoint position out of range");
+        CIR_HIP(hipMemcpyAsync(Y0.p, hy.data(), hy.size(), hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(out.p, ho.data(), out.bytes, hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(acc.p, ha.data(), acc.bytes, hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(skip.p, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+        CIR_HIP(hipStreamSynchronize(st));
+        start = int(next);
+    }
+    long long done_here = 0, since_ckpt = 0;
+    for (int c0 = start; c0 < nu; c0 += B) {
         const int nb = std::min(B, nu - c0);
         CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxc.as<float>()));
         {
@@ -337,6 +475,26 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
             }
             std::swap(Ycur, Ynext);
         }
+        done_here += nb;
+        since_ckpt += nb;
+        const bool more = c0 + nb < nu;
+        const bool halt = more && p.max_frames > 0 && done_here >= p.max_frames;
+        if (more && (halt || (p.ckpt_every > 0 && since_ckpt >= p.ckpt_every))) {
+            std::vector<unsigned char> hy(Y0.bytes);
+            std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom));
+            long long hs = 0;
+            CIR_HIP(hipMemcpyAsync(hy.data(), Ycur, hy.size(), hipMemcpyDeviceToHost, st));
+            CIR_HIP(hipMemcpyAsync(ho.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
+            CIR_HIP(hipMemcpyAsync(ha.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
+            CIR_HIP(hipMemcpyAsync(&hs, skip.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+            CIR_HIP(hipStreamSynchronize(st));
+            CIR_TRY(ckpt_write(ckpt_path(p), key, c0 + nb, hy, ho, ha, hs));
+            since_ckpt = 0;
+        }
+        if (halt) {
+            set_error("stopped after " + std::to_string(done_here) + " distinct frames; checkpoint " + ckpt_path(p));
+            return CIR_INCOMPLETE;
+        }
     }
     std::vector<double> hout(size_t(ns) * nt * 6), hacc(4 * size_t(nom));
     long long hskip = 0;
@@ -350,6 +508,8 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
 
 int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
     const cir_params& p = c->p;
+    if (p.ckpt_every > 0 || p.resume != 0 || p.max_frames > 0)
+        return fail(CIR_ERR_ARG, "checkpoint_every, resume and max_frames need mode batched");
     Schedule S;
     CIR_TRY(build_schedule(p, nframes, times, S));
     const int N = c->N, ld = c->ld, ns = p.nsamples, nt = p.ntcfpoints, nom = c->nomega;

@@ -58,7 +58,8 @@ typedef enum cir_status {
     CIR_ERR_NOMEM = 8,      /* device or host allocation failed                 */
     CIR_ERR_EIGH = 9,       /* eigensolver failure (calcIR.cu:445)              */
     CIR_ERR_NONFINITE = 10, /* non-finite coordinate or box                     */
-    CIR_ERR_NODEVICE = 11   /* no HIP device available                          */
+    CIR_ERR_NODEVICE = 11,  /* no HIP device available                          */
+    CIR_INCOMPLETE = 12     /* stopped at max_frames after writing a checkpoint */
 } cir_status;
 
 enum { CIR_PREC_FP32 = 0, CIR_PREC_FP64 = 1, CIR_PREC_BF16X3 = 2 };
@@ -81,7 +82,8 @@ typedef struct cir_params {
     char  xtcf[CIR_STRLEN];    /* trajectory; "synth:<nwater>:<nframes>:<seed>[:<L_nm>]" selects the
                                   seeded synthetic source                            */
     char  outf[CIR_STRLEN];
-    char  cptf[CIR_STRLEN];    /* parsed and unused, as in the reference           */
+    char  cptf[CIR_STRLEN];    /* checkpoint prefix (the reference parses it and never
+                                  uses it); see ckpt_every below                     */
     char  model[CIR_STRLEN];
     char  species[CIR_STRLEN];
     int   ntcfpoints;
@@ -103,6 +105,12 @@ typedef struct cir_params {
     int   precision;           /* key "precision": fp32 | fp64 | bf16x3 (default fp32) */
     int   mode;                /* key "mode": batched (default) | plain             */
     int   chunk;               /* key "chunk": frames per eigensolver batch, 0 = auto */
+    /* checkpoint/resume (batched mode), file <cptf>_ckpt.bin: */
+    int   ckpt_every;          /* key "checkpoint_every": write after at least this many
+                                  distinct frames (at chunk ends), 0 = never           */
+    int   resume;              /* key "resume": 1 = continue from the checkpoint        */
+    int   max_frames;          /* key "max_frames": stop after this many distinct frames
+                                  in this run, checkpoint and return CIR_INCOMPLETE; 0 = all */
 } cir_params;
 
 /* Results of a run, all arrays caller-allocated (host).  NULL members are

@@ -180,3 +180,104 @@ def test_cli_reads_an_xtc_trajectory(tmp_path):
         b = np.loadtxt(tmp_path / ("mem_%s.dat" % suffix))
         assert a.shape == b.shape, suffix
         np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6 * np.abs(b).max(), err_msg=suffix)
+
+
+def test_checkpoint_halt_and_resume_reproduce_the_uninterrupted_run(tmp_path):
+    # max_frames stops a run at a chunk end with a checkpoint (CIR_INCOMPLETE);
+    # resume continues from it.  Two halts and a final resume give the
+    # uninterrupted result; periodic checkpoints leave the result unchanged.
+    xyz, box, times = frames(128, 60, 1001)
+    ck = str(tmp_path / "job")
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        announce("checkpoint prec=%s uninterrupted" % prec)
+        ref = C.run(C.default_params(precision=prec, mode="batched", chunk=8, **KW), xyz, box, times)
+        announce("checkpoint prec=%s every 8 frames" % prec)
+        per = C.run(C.default_params(precision=prec, mode="batched", chunk=8, cptf=ck, checkpoint_every=8, **KW),
+                    xyz, box, times)
+        assert (tmp_path / "job_ckpt.bin").exists()
+        for k in ("tcf_ir", "tcf_vv", "tcf_vh", "sw", "pw"):
+            assert rel(per[k], ref[k]) < tol, (prec, k)
+        (tmp_path / "job_ckpt.bin").unlink()
+        for step, (resume, maxf) in enumerate(((0, 16), (1, 16), (1, 0))):
+            announce("checkpoint prec=%s leg %d resume=%d max_frames=%d" % (prec, step, resume, maxf))
+            p = C.default_params(precision=prec, mode="batched", chunk=8, cptf=ck, resume=resume, max_frames=maxf,
+                                 **KW)
+            if maxf:
+                with pytest.raises(C.CalcirError) as e:
+                    C.run(p, xyz, box, times)
+                assert e.value.status == C.INCOMPLETE
+            else:
+                g = C.run(p, xyz, box, times)
+        assert g["nframes_used"] == ref["nframes_used"] == 50 and g["nsteps"] == ref["nsteps"]
+        for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+            assert rel(g[k], ref[k]) < tol, (prec, k, rel(g[k], ref[k]))
+        for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
+            assert rel_l2(g[k], ref[k]) < tol, (prec, k)
+        assert rel(g["sw"], ref["sw"]) < tol
+        # Pw sums weights with device atomics (order-dependent in the last bits)
+        assert np.abs(g["pw"] - ref["pw"]).max() <= (1e-12 if prec == "fp64" else 1e-3) * ref["pw"].max()
+        assert int(g["hist_skipped"]) == int(ref["hist_skipped"]) or prec != "fp64"
+
+
+def test_checkpoint_guards(tmp_path):
+    xyz, box, times = frames(64, 40, 77)
+    ck = str(tmp_path / "g")
+    kw = dict(KW, nsamples=2, ntcfpoints=10)
+    announce("checkpoint guards: missing file")
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(precision="fp64", cptf=ck, resume=1, **kw), xyz, box, times)
+    assert e.value.status == 2
+    announce("checkpoint guards: halt")
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(precision="fp64", cptf=ck, chunk=4, max_frames=4, **kw), xyz, box, times)
+    assert e.value.status == C.INCOMPLETE
+    # another job (different avef, different frames) must not resume from it
+    announce("checkpoint guards: other job")
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(precision="fp64", cptf=ck, resume=1, **dict(kw, avef=3450.0)), xyz, box, times)
+    assert e.value.status == 1
+    moved = xyz.copy()
+    moved[0, 5, 0] += 0.01
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(precision="fp64", cptf=ck, resume=1, **kw), moved, box, times)
+    assert e.value.status == 1
+    x41, b41, t41 = frames(64, 41, 77)
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(precision="fp64", cptf=ck, resume=1, **kw), x41, b41, t41)
+    assert e.value.status == 1
+    # a truncated file is refused
+    data = (tmp_path / "g_ckpt.bin").read_bytes()
+    (tmp_path / "g_ckpt.bin").write_bytes(data[:len(data) // 2])
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(precision="fp64", cptf=ck, resume=1, **kw), xyz, box, times)
+    assert e.value.status == 2
+    # the reference control flow has no checkpoints
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(precision="fp64", mode="plain", cptf=ck, max_frames=4, **kw), xyz, box, times)
+    assert e.value.status == 1
+
+
+def test_cli_halts_and_resumes(tmp_path):
+    # the drop-in executable: max_frames exits with status 3 and writes no
+    # output files; resume 1 finishes and writes the same files as one run
+    base = ("xtcf synth:64:40:7\nmodel tip4p2005\nspecies H2O\nntcfpoints 10\nnsamples 2\nsampleEvery 0.1\n"
+            "avef 3400\nomegaStart 2800\nomegaStop 4000\nnzeros 502\nprecision fp64\nchunk 4\n")
+    one = tmp_path / "one.inp"
+    one.write_text(base + "outf %s\n" % (tmp_path / "one"))
+    announce("calcir CLI uninterrupted")
+    r = subprocess.run([C.CLI_PATH, str(one)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    leg = tmp_path / "leg.inp"
+    leg.write_text(base + "outf %s\ncptf %s\nmax_frames 8\n" % (tmp_path / "two", tmp_path / "two"))
+    announce("calcir CLI halted")
+    r = subprocess.run([C.CLI_PATH, str(leg)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 3, r.stdout[-2000:]
+    assert "resume 1" in r.stdout and not (tmp_path / "two_irls.dat").exists()
+    leg.write_text(base + "outf %s\ncptf %s\nresume 1\n" % (tmp_path / "two", tmp_path / "two"))
+    announce("calcir CLI resumed")
+    r = subprocess.run([C.CLI_PATH, str(leg)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    for suffix in ("irls", "vvls", "vhls", "irrtcf", "vhitcf", "Pw", "spdn"):
+        a = np.loadtxt(tmp_path / ("two_%s.dat" % suffix))
+        b = np.loadtxt(tmp_path / ("one_%s.dat" % suffix))
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max(), err_msg=suffix)

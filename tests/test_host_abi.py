@@ -73,13 +73,19 @@ def test_parser_keys_comments_and_quirks(tmp_path):
         "species         HOD/D2O\n"
         "cutoff          0.75\n"
         "precision       bf16x3\n"
-        "mode            plain\n")
+        "mode            plain\n"
+        "checkpoint_every 40\n"
+        "resume          1\n"
+        "max_frames      200\n")
     p = C.read_params(str(f))
     assert p.xtcf == b"nvt.xtc" and p.outf == b"test" and p.model == b"tip4p" and p.species == b"HOD/D2O"
     assert p.ntcfpoints == 400 and p.nsamples == 2 and p.imap == 1
     assert p.dt == pytest.approx(0.02) and p.t1 == pytest.approx(0.3) and p.avef == pytest.approx(3400.0)
     assert p.beginTime == pytest.approx(1.5) and p.sampleEvery == pytest.approx(4.0)
     assert p.cutoff == pytest.approx(0.75) and p.precision == C.PREC_BF16X3 and p.mode == C.MODE_PLAIN
+    assert (p.ckpt_every, p.resume, p.max_frames) == (40, 1, 200)
+    assert (C.default_params().ckpt_every, C.default_params().resume, C.default_params().max_frames) == (0, 0, 0)
+    assert C.default_params(checkpoint_every=5).ckpt_every == 5
     # a line holding one token takes the next line's first token as its value
     g = tmp_path / "quirk.inp"
     g.write_text("outf\nspec2 extra words\nntcfpoints 7\n")
