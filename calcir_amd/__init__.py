@@ -59,7 +59,7 @@ class Params(ctypes.Structure):
         ("beginTime", ctypes.c_float), ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float),
         ("cutoff", ctypes.c_float), ("precision", ctypes.c_int), ("mode", ctypes.c_int),
         ("chunk", ctypes.c_int), ("ckpt_every", ctypes.c_int), ("resume", ctypes.c_int),
-        ("max_frames", ctypes.c_int),
+        ("max_frames", ctypes.c_int), ("adj_store", ctypes.c_int),
     ]
 
 
@@ -111,6 +111,11 @@ SIGNATURES = {
     "cir_set_profiling": (_i, [_vp, _i]),
     "cir_phase_times": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll), _i]),
     "cir_destroy": (None, [_vp]),
+    "cir_map_coefs": (_i, [_i, _i, _vp]),
+    "cir_get_coefs": (_i, [_vp, _vp]),
+    "cir_set_coefs": (_i, [_vp, _vp]),
+    "cir_forward_ls": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "cir_backward_ls": (_i, [_vp, _vp, _vp]),
     "cir_eproj_sites": (_i, [_i, _i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "cir_hamiltonian": (_i, [_i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "cir_eigh_workspace": (ctypes.c_size_t, [_i, _i, _i, _i]),
@@ -166,6 +171,8 @@ def default_params(**kw):
             k = "imap"
         if k == "checkpoint_every":
             k = "ckpt_every"
+        if k == "adjoint_store":
+            k = "adj_store"
         if k == "precision" and isinstance(v, str):
             v = {"fp32": PREC_FP32, "fp64": PREC_FP64, "bf16x3": PREC_BF16X3}[v]
         if k == "mode" and isinstance(v, str):
@@ -267,12 +274,25 @@ def _results_buffers(p):
     return r, bufs
 
 
-def run(p, xyz, box, times, stream=None, profile=False):
+NCOEF = 13
+COEF_NAMES = ("w0", "w1", "w2", "x0", "x1", "p0", "p1", "mu0", "mu1", "mu2", "wi0", "wi1", "wi2")
+
+
+def map_coefs(imap, ispecies):
+    """The 13 map coefficients of a (map, species) row (calcIR.cu:1203-1242),
+    in COEF_NAMES order."""
+    c = np.zeros(NCOEF)
+    check(lib().cir_map_coefs(int(imap), int(ispecies), c.ctypes.data))
+    return c
+
+
+def run(p, xyz, box, times, stream=None, profile=False, coefs=None):
     """The whole calcIR job (every sample, calcIR.cu:360-919) on one GPU.
 
     xyz: [nframes, natoms, 3] float32 nm (host numpy); box [nframes, 3];
     times [nframes] ps.  Returns a dict with complex TCFs (decayed and
-    1/nsamples scaled), the C2R outputs, Sw, Pw, Rw, Rmw and counters."""
+    1/nsamples scaled), the C2R outputs, Sw, Pw, Rw, Rmw and counters.
+    coefs (13 doubles, COEF_NAMES order) replaces the map table row."""
     xyz = np.ascontiguousarray(xyz, np.float32)
     box = np.ascontiguousarray(box, np.float32)
     times = np.ascontiguousarray(times, np.float32)
@@ -282,6 +302,11 @@ def run(p, xyz, box, times, stream=None, profile=False):
     check(L.cir_create(ctypes.byref(p), natoms, stream, ctypes.byref(ctx)))
     phases = None
     try:
+        if coefs is not None:
+            cf = np.ascontiguousarray(coefs, np.float64)
+            if cf.shape != (NCOEF,):
+                raise ValueError("coefs must hold %d values" % NCOEF)
+            check(L.cir_set_coefs(ctx, cf.ctypes.data))
         if profile:
             check(L.cir_set_profiling(ctx, 1))
         check(L.cir_run(ctx, nframes, xyz.ctypes.data, box.ctypes.data, times.ctypes.data))

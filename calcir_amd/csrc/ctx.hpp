@@ -1,0 +1,87 @@
+// ctx.hpp -- the run context shared by the pipeline (pipeline.hip) and the
+// lineshape adjoint (adjoint.hip): device buffers, the frame schedule and the
+// context itself.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cir {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t n) {
+        release();
+        if (n == 0) n = 256;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+        }
+        bytes = n;
+        return CIR_OK;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// Which trajectory frame each (sample, tcf point) uses and, for the batched
+// flow, the distinct frames in ascending order with their per-slot actions
+// (calcIR.cu:360-395 frame choice).
+struct Schedule {
+    std::vector<int> fidx;        // ns*nt trajectory frame per (sample, point)
+    std::vector<int> uniq;        // distinct frames, ascending
+    std::vector<int> uof;         // trajectory frame -> distinct index (-1 if unused)
+    std::vector<double> mult;     // (sample, point) pairs per distinct frame
+    std::vector<double> orig;     // samples starting at the distinct frame
+    int nslot = 0;
+    std::vector<int> action;      // nuniq*nslot
+    std::vector<int> target;      // nuniq*nslot
+};
+
+int build_schedule(const cir_params& p, int nframes, const float* times, Schedule& S);
+int gather_frames(cir_ctx* c, const float* xyz_d, const float* box_d, const int* frames, int nb, float* xyzc,
+                  float* boxc);
+int auto_chunk(const cir_ctx* c, int nuniq);
+
+// State the lineshape adjoint keeps between its forward and backward calls.
+struct AdjState;
+void adj_release(AdjState* a);
+struct AdjDeleter {
+    void operator()(AdjState* a) const { adj_release(a); }
+};
+
+}  // namespace cir
+
+struct cir_ctx {
+    cir_params p;
+    int imodel = 0, ispecies = 0;
+    int natoms = 0, nmol = 0, N = 0, ld = 0, nomega = 0, nfft = 0;
+    int real_prec = CIR_PREC_FP32;  // storage precision of real arrays
+    int prop_prec = CIR_PREC_FP32;  // product precision of the propagator and the eigenvector back-transform
+    size_t esz = 4;
+    hipStream_t stream = nullptr;
+    cir::MapCoef mc{};              // map coefficients in use (the table row unless overridden)
+    std::vector<double> tcf[3], ftcf[3], sw, pw, rw, rmw;
+    int64_t nsteps = 0, nframes_used = 0, hist_skipped = 0;
+    double seconds = 0.0;
+    bool have = false;
+    cir::Prof prof;
+    double phase_sec[CIR_NPHASE] = {};
+    long long phase_cnt[CIR_NPHASE] = {};
+    std::unique_ptr<cir::AdjState, cir::AdjDeleter> adj;
+};

@@ -15,19 +15,10 @@
 // that H(i,j) == H(j,i) bit for bit.
 
 #include "common.hpp"
+#include "sites.hpp"
+#include "spectra.hpp"
 
 namespace cir {
-
-template <typename T>
-struct MapT {
-    T w0, w1, w2, x0, x1, p0, p1, m0, m1, m2, wi0, wi1, wi2;
-};
-
-template <typename T>
-MapT<T> map_t(const MapCoef& c) {
-    return MapT<T>{T(c.w[0]), T(c.w[1]), T(c.w[2]), T(c.x[0]), T(c.x[1]), T(c.p[0]), T(c.p[1]),
-                   T(c.mup[0]), T(c.mup[1]), T(c.mup[2]), T(c.wi[0]), T(c.wi[1]), T(c.wi[2])};
-}
 
 constexpr int kFieldBlock = 256;
 constexpr int kFieldTile = 256;  // molecules staged per LDS tile
@@ -216,17 +207,9 @@ hamiltonian_kernel(int N, int ld, int nchrom_mol, int hod, T avef, T wi0, T wi1,
                 wi2 * (rs[3][tx] * cs[3][jl]);
         } else {
             // calcIR.cu:1353-1401, symmetric operation order
-            T dr[3];
-            for (int c = 0; c < 3; ++c) dr[c] = minimage<T>(rs[8 + c][tx] - cs[8 + c][jl], bb[c]);
-            const T r = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-            const T inv = T(1) / r;
-            const T d0 = dr[0] * inv, d1 = dr[1] * inv, d2 = dr[2] * inv;
-            const T uu = rs[5][tx] * cs[5][jl] + rs[6][tx] * cs[6][jl] + rs[7][tx] * cs[7][jl];
-            const T ur = rs[5][tx] * d0 + rs[6][tx] * d1 + rs[7][tx] * d2;
-            const T vr = cs[5][jl] * d0 + cs[6][jl] * d1 + cs[7][jl] * d2;
-            const T rb = r * T(kBohrPerNm);
-            const T t = ur * vr;
-            h = (uu - T(3) * t) / (rb * rb * rb) * (rs[4][tx] * cs[4][jl]) * T(kCmPerHartree);
+            const T g = tdc_geometry<T>(rs[5][tx], rs[6][tx], rs[7][tx], rs[8][tx], rs[9][tx], rs[10][tx],
+                                        cs[5][jl], cs[6][jl], cs[7][jl], cs[8][jl], cs[9][jl], cs[10][jl], bb);
+            h = g * (rs[4][tx] * cs[4][jl]) * T(kCmPerHartree);
         }
         Hf[size_t(j) * ld + i] = h;
     }
@@ -256,6 +239,29 @@ int launch_hamiltonian(int nframes, int N, int ld, int nchrom_mol, int ispecies,
     return CIR_OK;
 }
 
+int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, const MapCoef& mc,
+                float cutoff, const float* xyz_d, const float* box_d, void* sites_d, hipStream_t s) {
+    if (nframes <= 0 || natom_mol < 3 || nchrom_mol <= 0 || natoms <= 0 || natoms % natom_mol || !xyz_d ||
+        !box_d || !sites_d)
+        return fail(CIR_ERR_ARG, "cir_eproj_sites: bad sizes or null pointer");
+    if (int64_t(nframes) > 65535) return fail(CIR_ERR_ARG, "cir_eproj_sites: nframes > 65535");
+    if (prec == CIR_PREC_FP64)
+        return launch_eproj_sites<double>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, mc,
+                                          sites_d, s);
+    return launch_eproj_sites<float>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, mc,
+                                     sites_d, s);
+}
+
+int hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies, const MapCoef& mc, float avef,
+                const float* box_d, const void* sites_d, void* H_d, hipStream_t s) {
+    if (nframes <= 0 || n <= 0 || ld < n || nchrom_mol <= 0 || !box_d || !sites_d || !H_d)
+        return fail(CIR_ERR_ARG, "cir_hamiltonian: bad sizes or null pointer");
+    if (nframes > 65535) return fail(CIR_ERR_ARG, "cir_hamiltonian: nframes > 65535");
+    if (prec == CIR_PREC_FP64)
+        return launch_hamiltonian<double>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d, H_d, s);
+    return launch_hamiltonian<float>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d, H_d, s);
+}
+
 }  // namespace cir
 
 using namespace cir;
@@ -266,16 +272,8 @@ extern "C" int cir_eproj_sites(int prec, int nframes, int natoms, int natom_mol,
                                void* stream) {
     MapCoef mc;
     if (!map_coef(imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map/species not in the map table");
-    if (nframes <= 0 || natom_mol < 3 || nchrom_mol <= 0 || natoms <= 0 || natoms % natom_mol ||
-        !xyz_d || !box_d || !sites_d)
-        return fail(CIR_ERR_ARG, "cir_eproj_sites: bad sizes or null pointer");
-    if (int64_t(nframes) > 65535) return fail(CIR_ERR_ARG, "cir_eproj_sites: nframes > 65535");
-    hipStream_t s = as_stream(stream);
-    if (prec == CIR_PREC_FP64)
-        return launch_eproj_sites<double>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d,
-                                          box_d, mc, sites_d, s);
-    return launch_eproj_sites<float>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d,
-                                     box_d, mc, sites_d, s);
+    return eproj_sites(prec, nframes, natoms, natom_mol, nchrom_mol, imodel, mc, cutoff, xyz_d, box_d, sites_d,
+                       as_stream(stream));
 }
 
 extern "C" int cir_hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies,
@@ -283,13 +281,6 @@ extern "C" int cir_hamiltonian(int prec, int nframes, int n, int ld, int nchrom_
                                void* H_d, void* stream) {
     MapCoef mc;
     if (!map_coef(imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map/species not in the map table");
-    if (nframes <= 0 || n <= 0 || ld < n || nchrom_mol <= 0 || !box_d || !sites_d || !H_d)
-        return fail(CIR_ERR_ARG, "cir_hamiltonian: bad sizes or null pointer");
-    if (nframes > 65535) return fail(CIR_ERR_ARG, "cir_hamiltonian: nframes > 65535");
-    hipStream_t s = as_stream(stream);
-    if (prec == CIR_PREC_FP64)
-        return launch_hamiltonian<double>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d,
-                                          H_d, s);
-    return launch_hamiltonian<float>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d, H_d,
-                                     s);
+    return hamiltonian(prec, nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d, H_d,
+                       as_stream(stream));
 }

@@ -23,72 +23,14 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "gemm.hpp"
 #include "spectra.hpp"
 
-namespace cir {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    int alloc(size_t n) {
-        release();
-        if (n == 0) n = 256;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
-        }
-        bytes = n;
-        return CIR_OK;
-    }
-    template <typename T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-}  // namespace cir
-
-struct cir_ctx {
-    cir_params p;
-    int imodel = 0, ispecies = 0;
-    int natoms = 0, nmol = 0, N = 0, ld = 0, nomega = 0, nfft = 0;
-    int real_prec = CIR_PREC_FP32;  // storage precision of real arrays
-    int prop_prec = CIR_PREC_FP32;  // product precision of the propagator and the eigenvector back-transform
-    size_t esz = 4;
-    hipStream_t stream = nullptr;
-    std::vector<double> tcf[3], ftcf[3], sw, pw, rw, rmw;
-    int64_t nsteps = 0, nframes_used = 0, hist_skipped = 0;
-    double seconds = 0.0;
-    bool have = false;
-    cir::Prof prof;
-    double phase_sec[CIR_NPHASE] = {};
-    long long phase_cnt[CIR_NPHASE] = {};
-};
-
 using namespace cir;
 
-namespace {
+namespace cir {
 
-struct Schedule {
-    std::vector<int> fidx;        // ns*nt trajectory frame per (sample, point)
-    std::vector<int> uniq;        // distinct frames, ascending
-    std::vector<double> mult;     // (sample, point) pairs per distinct frame
-    std::vector<double> orig;     // samples starting at the distinct frame
-    int nslot = 0;
-    std::vector<int> action;      // nuniq*nslot
-    std::vector<int> target;      // nuniq*nslot
-};
 
 int build_schedule(const cir_params& p, int nframes, const float* times, Schedule& S) {
     const int ns = p.nsamples, nt = p.ntcfpoints;
@@ -110,7 +52,8 @@ int build_schedule(const cir_params& p, int nframes, const float* times, Schedul
                 return fail(CIR_ERR_FRAME, "dt must select increasing trajectory frames");
             S.fidx[size_t(s) * nt + k] = f;
         }
-    std::vector<int> uof(nframes, -1);
+    std::vector<int>& uof = S.uof;
+    uof.assign(nframes, -1);
     for (int f : S.fidx) uof[f] = 0;
     S.uniq.clear();
     for (int f = 0; f < nframes; ++f)
@@ -297,6 +240,7 @@ int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d
     f.val(p.omegaStart);
     f.val(p.omegaStop);
     f.val(p.cutoff);
+    f.val(c->mc);
     f.val(S.nslot);
     f.val(int64_t(S.uniq.size()));
     f.val(nframes);
@@ -434,12 +378,12 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
         CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxc.as<float>()));
         {
             ProfScope ps(pf, CIR_PH_SITES, st);
-            CIR_TRY(cir_eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
+            CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc,
                                     p.cutoff, xyzc.as<float>(), boxc.as<float>(), sites[set].p, st));
         }
         {
             ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
-            CIR_TRY(cir_hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, boxc.as<float>(),
+            CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxc.as<float>(),
                                     sites[set].p, H.p, st));
         }
         CIR_TRY(eigh_batched(c->prop_prec, nb, N, ld, H.p, V[set].p, w[set].as<double>(), work.p, st, pf));
@@ -544,12 +488,12 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
             Prof* pf = &c->prof;
             {
                 ProfScope ps(pf, CIR_PH_SITES, st);
-                CIR_TRY(cir_eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->ispecies, p.imap,
+                CIR_TRY(eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc,
                                         p.cutoff, xyz1.as<float>(), box1.as<float>(), sites1.p, st));
             }
             {
                 ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
-                CIR_TRY(cir_hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, p.imap, p.avef, box1.as<float>(),
+                CIR_TRY(hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, box1.as<float>(),
                                         sites1.p, H.p, st));
             }
             CIR_TRY(eigh_batched(c->prop_prec, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
@@ -635,6 +579,7 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     c->p = *p;
     c->imodel = imodel;
     c->ispecies = ispecies;
+    c->mc = mc;
     c->natoms = natoms;
     c->nmol = natoms / p->natom_mol;
     c->N = c->nmol * p->nchrom_mol;

@@ -6,6 +6,14 @@
 
 namespace cir {
 
+// Field projection + site records and the Hamiltonian with an explicit map
+// coefficient set (k_sites.hip); the cir_eproj_sites / cir_hamiltonian entry
+// points resolve the set from the map table.
+int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, const MapCoef& mc,
+                float cutoff, const float* xyz_d, const float* box_d, void* sites_d, hipStream_t s);
+int hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies, const MapCoef& mc, float avef,
+                const float* box_d, const void* sites_d, void* H_d, hipStream_t s);
+
 size_t eigh_workspace(int prec, int batch, int n, int ld);
 int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w, void* work, hipStream_t s,
                  Prof* prof = nullptr);

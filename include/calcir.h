@@ -111,6 +111,11 @@ typedef struct cir_params {
     int   resume;              /* key "resume": 1 = continue from the checkpoint        */
     int   max_frames;          /* key "max_frames": stop after this many distinct frames
                                   in this run, checkpoint and return CIR_INCOMPLETE; 0 = all */
+    /* lineshape adjoint (cir_forward_ls / cir_backward_ls): */
+    int   adj_store;           /* key "adjoint_store": 0 = keep every frame's eigensystem
+                                  when it fits in device memory, else recompute them chunk
+                                  by chunk in the reverse sweep; 1 = always keep; 2 = always
+                                  recompute                                            */
 } cir_params;
 
 /* Results of a run, all arrays caller-allocated (host).  NULL members are
@@ -201,6 +206,33 @@ int   cir_set_profiling(cir_ctx* ctx, int on);
  * (arrays of nphase <= CIR_NPHASE entries). */
 int   cir_phase_times(cir_ctx* ctx, double* seconds, long long* groups, int nphase);
 void  cir_destroy(cir_ctx* ctx);
+
+/* --- map coefficients and the lineshape adjoint ---------------------------- */
+/* The 13 map coefficients of a (map, species) row of calcIR.cu:1203-1242, in
+ * the order of CIR_COEF_*: omega = w0 + w1 E + w2 E^2, x10 = x0 + x1 omega,
+ * p10 = p0 + p1 omega, mu' = mu0 + mu1 E + mu2 E^2, intramolecular coupling
+ * (wi0 + wi1 (E_i + E_j)) x_i x_j + wi2 p_i p_j (calcIR.cu:1253-1341). */
+#define CIR_NCOEF 13
+enum {
+    CIR_COEF_W0 = 0, CIR_COEF_W1, CIR_COEF_W2, CIR_COEF_X0, CIR_COEF_X1, CIR_COEF_P0, CIR_COEF_P1,
+    CIR_COEF_MU0, CIR_COEF_MU1, CIR_COEF_MU2, CIR_COEF_WI0, CIR_COEF_WI1, CIR_COEF_WI2
+};
+int   cir_map_coefs(int imap, int ispecies, double* coef /* CIR_NCOEF */);
+/* Coefficients a context uses (the table row of its map and species unless
+ * overridden); cir_set_coefs(ctx, NULL) restores the table row.  Applies to
+ * cir_run and cir_forward_ls. */
+int   cir_get_coefs(const cir_ctx* ctx, double* coef);
+int   cir_set_coefs(cir_ctx* ctx, const double* coef);
+/* Forward pass of the IR lineshape: the C2R output of the decayed, averaged
+ * IR correlation function (ftcf_ir, ntcfpoints+nzeros doubles; the same as
+ * cir_run's ftcf_ir), keeping every distinct frame's eigensystem and the
+ * propagated dipole vectors on the device for cir_backward_ls.  Host arrays
+ * as cir_run.  No reference counterpart: the reference has no derivatives. */
+int   cir_forward_ls(cir_ctx* ctx, int nframes, const float* xyz, const float* box, const float* times,
+                     double* ftcf_ir);
+/* Reverse pass: given dL/dftcf_ir (ntcfpoints+nzeros doubles) for any loss L
+ * of the last forward pass's output, writes dL/dcoef (CIR_NCOEF doubles). */
+int   cir_backward_ls(cir_ctx* ctx, const double* dL_dftcf_ir, double* dL_dcoef);
 
 /* --- per-stage device entry points --------------------------------------- */
 /* Field projection (calcIR.cu:1020-1168) fused with the per-row site record

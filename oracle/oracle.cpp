@@ -72,6 +72,16 @@ bool map_table(int imap, int ispecies, MapCoef* c) {
     return false;
 }
 
+// The table row, or the caller's coefficients when the config carries them.
+bool cfg_coef(const orc_config* cfg, MapCoef* c) {
+    if (!map_table(cfg->imap, cfg->ispecies, c)) return false;
+    if (cfg->use_coef) {
+        const double* k = cfg->coef;
+        *c = {{k[0], k[1], k[2]}, {k[3], k[4]}, {k[5], k[6]}, {k[7], k[8], k[9]}, {k[10], k[11], k[12]}};
+    }
+    return true;
+}
+
 // calcIR.cu:1457-1460: dx - L*round(dx/L), round half away from zero.
 template <typename T>
 inline T minimage(T dx, T boxl) {
@@ -516,7 +526,7 @@ int run_t(const orc_config* cfg, int natoms, int nframes, const float* xyz, cons
           const float* times, orc_result* out) {
     using C = std::complex<T>;
     MapCoef mc;
-    if (!map_table(cfg->imap, cfg->ispecies, &mc)) return 2;
+    if (!cfg_coef(cfg, &mc)) return 2;
     if (cfg->natom_mol <= 0 || cfg->nchrom_mol <= 0 || natoms % cfg->natom_mol) return 1;
     if (nframes < 2 || cfg->ntcfpoints <= 0 || cfg->nsamples <= 0) return 1;
     const int nthreads = threads_of(cfg);
@@ -754,7 +764,7 @@ int orc_eproj(int prec, const orc_config* cfg, const float* x, int natoms, const
 int orc_sites(int prec, const orc_config* cfg, const float* x, int natoms, const float box[3],
               const double* eproj, double* site) {
     MapCoef mc;
-    if (!cfg || !map_table(cfg->imap, cfg->ispecies, &mc)) return 2;
+    if (!cfg || !cfg_coef(cfg, &mc)) return 2;
     if (prec == ORC_F32) {
         std::vector<Site<float>> s;
         sites_t<float>(cfg, mc, x, natoms, box, eproj, s);
@@ -770,7 +780,7 @@ int orc_sites(int prec, const orc_config* cfg, const float* x, int natoms, const
 int orc_hamiltonian(int prec, const orc_config* cfg, const float* x, int natoms, const float box[3],
                     const double* eproj, double* H) {
     MapCoef mc;
-    if (!cfg || !map_table(cfg->imap, cfg->ispecies, &mc)) return 2;
+    if (!cfg || !cfg_coef(cfg, &mc)) return 2;
     const int N = (natoms / cfg->natom_mol) * cfg->nchrom_mol;
     if (prec == ORC_F32) {
         std::vector<Site<float>> s;

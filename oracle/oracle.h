@@ -30,7 +30,10 @@
  *   orc_run             uncoupled (HOD) closed-form TCF, literal F-matrix vs
  *                       vector propagation, C_IR(0) = sum |mu|^2, VV(0)/VH(0) =
  *                       orientational averages of alpha, FP32 vs FP64
- *                       (tests/test_oracle_pipeline.py)
+ *                       (tests/test_oracle_pipeline.py); coefficient override
+ *                       equal to the table row reproduces the table run, and
+ *                       the closed-form map polynomials hold at overridden
+ *                       coefficients (tests/test_oracle_maps_hamiltonian.py)
  */
 #ifndef CALCIR_ORACLE_H
 #define CALCIR_ORACLE_H
@@ -81,6 +84,10 @@ typedef struct orc_config {
     int   omegaStep;
     int   propagation;   /* ORC_PROP_LITERAL or ORC_PROP_VECTOR            */
     int   nthreads;      /* OpenMP threads (0 = library default)           */
+    /* map coefficients in place of the calcIR.cu:1203-1242 table row when
+     * use_coef != 0: w0 w1 w2, x0 x1, p0 p1, mu0 mu1 mu2, wi0 wi1 wi2 */
+    int    use_coef;
+    double coef[13];
 } orc_config;
 
 /* Outputs of orc_run.  All arrays are caller-allocated. */

@@ -30,6 +30,7 @@ class Config(ctypes.Structure):
         ("sampleEvery", ctypes.c_float), ("beginTime", ctypes.c_float),
         ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float), ("omegaStep", ctypes.c_int),
         ("propagation", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("use_coef", ctypes.c_int), ("coef", ctypes.c_double * 13),
     ]
 
 
@@ -85,10 +86,27 @@ def lib():
 def config(imodel=1, ispecies=0, imap=0, cutoff=0.7831, avef=3415.2, dt=0.010, t1=0.260,
            ntcfpoints=150, nsamples=1, nzeros=25600, sampleEvery=10.0, beginTime=0.0,
            omegaStart=2000.0, omegaStop=5000.0, omegaStep=5, propagation=PROP_VECTOR, nthreads=0,
-           natom_mol=4, nchrom_mol=2):
-    return Config(natom_mol, nchrom_mol, imodel, ispecies, imap, cutoff, avef, dt, t1, ntcfpoints,
-                  nsamples, nzeros, sampleEvery, beginTime, omegaStart, omegaStop, omegaStep,
-                  propagation, nthreads)
+           natom_mol=4, nchrom_mol=2, coef=None):
+    c = Config(natom_mol, nchrom_mol, imodel, ispecies, imap, cutoff, avef, dt, t1, ntcfpoints,
+               nsamples, nzeros, sampleEvery, beginTime, omegaStart, omegaStop, omegaStep,
+               propagation, nthreads)
+    if coef is not None:
+        set_coef(c, coef)
+    return c
+
+
+def set_coef(cfg, coef):
+    """Override the map coefficients (13 values, w0 w1 w2 x0 x1 p0 p1 mu0 mu1
+    mu2 wi0 wi1 wi2); None restores the table row."""
+    if coef is None:
+        cfg.use_coef = 0
+        return cfg
+    coef = np.asarray(coef, np.float64)
+    assert coef.shape == (13,)
+    cfg.use_coef = 1
+    for q in range(13):
+        cfg.coef[q] = float(coef[q])
+    return cfg
 
 
 def _c(a, dt=np.float32):
@@ -192,9 +210,9 @@ def run(prec, cfg, xyz, box, times):
     return out
 
 
-def config_from_params(p):
+def config_from_params(p, coef=None):
     """orc_config from a calcir_amd.Params (resolving model/species strings as
-    calcIR.cu:324-339 does)."""
+    calcIR.cu:324-339 does), with optional overriding map coefficients."""
     model = p.model.decode()
     species = p.species.decode()
     imodel = 1 if model in ("tip4p2005", "e3b3") else 0 if model in ("tip4p", "e3b2") else -1
@@ -203,4 +221,4 @@ def config_from_params(p):
                   t1=p.t1, ntcfpoints=p.ntcfpoints, nsamples=p.nsamples, nzeros=p.nzeros,
                   sampleEvery=p.sampleEvery, beginTime=p.beginTime, omegaStart=p.omegaStart,
                   omegaStop=p.omegaStop, omegaStep=p.omegaStep, natom_mol=p.natom_mol,
-                  nchrom_mol=p.nchrom_mol)
+                  nchrom_mol=p.nchrom_mol, coef=coef)

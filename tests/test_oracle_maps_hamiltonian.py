@@ -131,3 +131,40 @@ def test_hamiltonian_symmetric_and_species_maps_differ():
     H3 = O.hamiltonian(O.F64, O.config(ispecies=3), xyz[0], box[0], O.eproj(O.F64, O.config(), xyz[0], box[0]))
     assert np.allclose(H0, H0.T, rtol=1e-12, atol=1e-10)
     assert not np.allclose(np.diag(H0), np.diag(H3))
+
+
+def test_coefficient_override_closed_form_and_identity():
+    # overridden coefficients enter the same closed-form polynomials, and an
+    # override equal to the table row reproduces the table run exactly
+    x = _two_waters(sep=0.5)  # O...H within the cutoff: the coupling is nonzero
+    box = np.array([4.0, 4.0, 4.0], np.float32)
+    rng = np.random.default_rng(5)
+    coef = np.array([3600.0, -3000.0, -1.4e5, 0.2, -1.5e-5, 1.7, 6e-4, 0.17, 11.0, 70.0, -1300.0, 26000.0, -2.0])
+    coef *= 1 + 0.05 * rng.standard_normal(13)
+    cfg = O.config(imodel=0, ispecies=0, imap=0, coef=coef)
+    E = np.array([0.0, 0.01, -0.02, 0.035])
+    s = O.sites(O.F64, cfg, x, box, E)
+    w = coef[0] + coef[1] * E + coef[2] * E * E
+    assert np.allclose(s[:, O.SITE_W], w, rtol=1e-14)
+    assert np.allclose(s[:, O.SITE_X], coef[3] + coef[4] * w, rtol=1e-14)
+    assert np.allclose(s[:, O.SITE_P], coef[5] + coef[6] * w, rtol=1e-14)
+    assert np.allclose(s[:, O.SITE_MUP], coef[7] + coef[8] * E + coef[9] * E * E, rtol=1e-14)
+    # intramolecular coupling of molecule A's two chromophores
+    e = O.eproj(O.F64, cfg, x, box)
+    s = O.sites(O.F64, cfg, x, box, e)
+    H = O.hamiltonian(O.F64, cfg, x, box, e)
+    xx, pp, ee = s[:, O.SITE_X], s[:, O.SITE_P], e
+    h01 = (coef[10] + coef[11] * (ee[0] + ee[1])) * xx[0] * xx[1] + coef[12] * pp[0] * pp[1]
+    assert H[0, 1] == pytest.approx(h01, rel=1e-13)
+    # override == table row: identical pipeline output
+    from cases import frames
+    xyz, bx, times = frames(16, 24, 3)
+    base = dict(imodel=1, ispecies=0, imap=0, ntcfpoints=6, nsamples=2, sampleEvery=0.05, nzeros=58,
+                omegaStart=2800.0, omegaStop=4000.0, avef=3400.0)
+    table = np.array([3670.2, -3541.7, -152677.0, 0.19285, -1.7261e-5, 1.6466, 5.7692e-4, 0.1646, 11.39, 63.41,
+                      -1361.0, 27165.0, -1.887])
+    a = O.run(O.F64, O.config(**base), xyz, bx, times)
+    b = O.run(O.F64, O.config(coef=table, **base), xyz, bx, times)
+    assert np.array_equal(a["ftcf_ir"], b["ftcf_ir"]) and np.array_equal(a["tcf_vv"], b["tcf_vv"])
+    c = O.run(O.F64, O.config(coef=table * 1.01, **base), xyz, bx, times)
+    assert not np.allclose(a["ftcf_ir"], c["ftcf_ir"])
