@@ -59,7 +59,7 @@ class Params(ctypes.Structure):
         ("beginTime", ctypes.c_float), ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float),
         ("cutoff", ctypes.c_float), ("precision", ctypes.c_int), ("mode", ctypes.c_int),
         ("chunk", ctypes.c_int), ("ckpt_every", ctypes.c_int), ("resume", ctypes.c_int),
-        ("max_frames", ctypes.c_int), ("adj_store", ctypes.c_int),
+        ("max_frames", ctypes.c_int), ("adj_store", ctypes.c_int), ("triclinic", ctypes.c_int),
     ]
 
 
@@ -102,6 +102,8 @@ SIGNATURES = {
     "cir_xtc_scan": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "cir_xtc_read": (_i, [ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "cir_xtc_write": (_i, [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _f, _i]),
+    "cir_xtc_read_full": (_i, [ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "cir_xtc_write_full": (_i, [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _f, _i]),
     "cir_main": (_i, [_i, ctypes.POINTER(ctypes.c_char_p)]),
     "cir_device_count": (_i, [ctypes.POINTER(_i)]),
     "cir_create": (_i, [ctypes.POINTER(Params), _i, _vp, ctypes.POINTER(_vp)]),
@@ -238,29 +240,33 @@ def xtc_scan(path):
     return na.value, nf.value
 
 
-def xtc_read(path, first=0, count=None):
+def xtc_read(path, first=0, count=None, full_box=False):
     """Frames of an XTC file: (xyz [count, natoms, 3] nm, box diagonal
-    [count, 3], times [count] ps, steps [count])."""
+    [count, 3] -- or with full_box the matrices [count, 3, 3], rows a, b, c --
+    times [count] ps, steps [count])."""
     natoms, nframes = xtc_scan(path)
     count = nframes - first if count is None else count
     xyz = np.zeros((count, natoms, 3), np.float32)
-    box = np.zeros((count, 3), np.float32)
+    box = np.zeros((count, 3, 3) if full_box else (count, 3), np.float32)
     times = np.zeros(count, np.float32)
     steps = np.zeros(count, np.int32)
-    check(lib().cir_xtc_read(os.fsencode(path), first, count, natoms, xyz.ctypes.data, box.ctypes.data,
-                             times.ctypes.data, steps.ctypes.data))
+    fn = lib().cir_xtc_read_full if full_box else lib().cir_xtc_read
+    check(fn(os.fsencode(path), first, count, natoms, xyz.ctypes.data, box.ctypes.data, times.ctypes.data,
+             steps.ctypes.data))
     return xyz, box, times, steps
 
 
 def xtc_write(path, xyz, box, times, precision=1000.0, append=False):
-    """Writes frames (xyz [nframes, natoms, 3] nm, box diagonal [nframes, 3],
-    times [nframes] ps) as XTC with the given precision."""
+    """Writes frames (xyz [nframes, natoms, 3] nm, box diagonal [nframes, 3]
+    or matrices [nframes, 3, 3], times [nframes] ps) as XTC with the given
+    precision."""
     xyz = np.ascontiguousarray(xyz, np.float32)
     box = np.ascontiguousarray(box, np.float32)
     times = np.ascontiguousarray(times, np.float32)
     nframes, natoms = xyz.shape[0], xyz.shape[1]
-    check(lib().cir_xtc_write(os.fsencode(path), natoms, nframes, xyz.ctypes.data, box.ctypes.data,
-                              times.ctypes.data, float(precision), int(bool(append))))
+    fn = lib().cir_xtc_write_full if box.ndim == 3 else lib().cir_xtc_write
+    check(fn(os.fsencode(path), natoms, nframes, xyz.ctypes.data, box.ctypes.data, times.ctypes.data,
+             float(precision), int(bool(append))))
 
 
 def _results_buffers(p):

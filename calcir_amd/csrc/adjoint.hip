@@ -254,8 +254,8 @@ adj_rsym_kernel(int N, int ld, double tau, const double* __restrict__ lam, const
 // derivatives are added into part[blockIdx.x * 13 + c].
 template <typename T>
 __global__ void __launch_bounds__(256)
-adj_coef_kernel(int N, int ld, int nchrom_mol, int hod, MapT<T> mc, const float* __restrict__ box,
-                const T* __restrict__ S, const T* __restrict__ Gs, const double* __restrict__ gmu,
+adj_coef_kernel(int N, int ld, int nchrom_mol, int hod, MapT<T> mc, const float* __restrict__ box, int boxk,
+                int frame, const T* __restrict__ S, const T* __restrict__ Gs, const double* __restrict__ gmu,
                 double* __restrict__ part) {
     __shared__ T cj[10][256];  // E, x, p, s, u(3), r(3) of a column tile
     __shared__ double red[kCoefRows][CIR_NCOEF];
@@ -267,7 +267,7 @@ adj_coef_kernel(int N, int ld, int nchrom_mol, int hod, MapT<T> mc, const float*
     const T pi = S[CIR_SITE_P * N + ii], mupi = S[CIR_SITE_MUP * N + ii];
     const T ui[3] = {S[CIR_SITE_U * N + ii], S[(CIR_SITE_U + 1) * N + ii], S[(CIR_SITE_U + 2) * N + ii]};
     const T ri[3] = {S[CIR_SITE_R * N + ii], S[(CIR_SITE_R + 1) * N + ii], S[(CIR_SITE_R + 2) * N + ii]};
-    const T bb[3] = {T(box[0]), T(box[1]), T(box[2])};
+    const PBox<T> bb = load_box<T>(box, boxk, frame);
     const int moli = ii / nchrom_mol;
     T gx = T(0), gp = T(0), gs = T(0), c0 = T(0), c1 = T(0), c2 = T(0);
     if (Gs != nullptr && !hod) {
@@ -432,7 +432,7 @@ int forward_impl(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d
     CIR_TRY(A->H.alloc(size_t(B) * N * ld * esz));
     CIR_TRY(A->work.alloc(eigh_workspace(rp, B, N, ld)));
     CIR_TRY(A->sites.alloc(size_t(nu) * CIR_NSITE * N * esz));
-    CIR_TRY(A->box.alloc(size_t(nu) * 3 * sizeof(float)));
+    CIR_TRY(A->box.alloc(size_t(nu) * box_floats(c) * sizeof(float)));
     CIR_TRY(A->fv.alloc(size_t(ns) * nt * N * 6 * esz));
     CIR_TRY(A->pairs.alloc(L.data.size() * sizeof(int)));
     CIR_HIP(hipMemcpyAsync(A->pairs.p, L.data.data(), L.data.size() * sizeof(int), hipMemcpyHostToDevice, st));
@@ -449,13 +449,14 @@ int forward_impl(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d
     float* boxes = A->box.as<float>();
     for (int c0 = 0; c0 < nu; c0 += B) {
         const int nb = std::min(B, nu - c0);
-        CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxes + 3 * size_t(c0)));
+        const int bs = box_floats(c);
+        CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxes + bs * size_t(c0)));
         void* sc = Sb + size_t(c0) * CIR_NSITE * N * esz;
         CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc, p.cutoff,
-                            xyzc.as<float>(), boxes + 3 * size_t(c0), sc, st));
+                            xyzc.as<float>(), boxes + bs * size_t(c0), c->boxk, sc, st));
         const int v0 = A->store ? c0 : 0;  // slot of frame c0 in V, w
-        CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxes + 3 * size_t(c0), sc,
-                            A->H.p, st));
+        CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxes + bs * size_t(c0),
+                            c->boxk, sc, A->H.p, st));
         CIR_TRY(eigh_batched(c->prop_prec, nb, N, ld, A->H.p, Vb + size_t(v0) * N * ld * esz,
                              A->w.as<double>() + size_t(v0) * N, A->work.p, st, nullptr));
         for (int u = c0; u < c0 + nb; ++u) {
@@ -574,7 +575,7 @@ int backward_impl(cir_ctx* c, const double* dL_dy, double* grad) {
             const int nb = std::min(A->B, nu - cbeg);
             const void* sc = Sb + size_t(cbeg) * CIR_NSITE * N * esz;
             CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef,
-                                A->box.as<float>() + 3 * size_t(cbeg), sc, A->H.p, st));
+                                A->box.as<float>() + box_floats(c) * size_t(cbeg), c->boxk, sc, A->H.p, st));
             CIR_TRY(eigh_batched(c->prop_prec, nb, N, ld, A->H.p, A->V.p, A->w.as<double>(), A->work.p, st, nullptr));
         }
         const int vu = A->store ? u : u - cbeg;
@@ -628,7 +629,7 @@ int backward_impl(cir_ctx* c, const double* dL_dy, double* grad) {
             }
         }
         hipLaunchKernelGGL(adj_coef_kernel<T>, dim3(unsigned(nrb)), dim3(256), 0, st, N, ld, p.nchrom_mol, hod, mc,
-                           A->box.as<float>() + 3 * size_t(u), Su, Gs, gmu.as<double>(), part.as<double>());
+                           A->box.as<float>(), c->boxk, u, Su, Gs, gmu.as<double>(), part.as<double>());
         CIR_CHECK_LAUNCH();
     }
     std::vector<double> hp(size_t(nrb) * CIR_NCOEF);
@@ -694,13 +695,13 @@ extern "C" int cir_forward_ls(cir_ctx* c, int nframes, const float* xyz, const f
     const size_t fb = size_t(nframes) * c->natoms * 3 * sizeof(float);
     for (size_t q = 0; q < size_t(nframes) * c->natoms * 3; ++q)
         if (!std::isfinite(xyz[q])) return fail(CIR_ERR_NONFINITE, "non-finite coordinate");
-    for (int q = 0; q < 3 * nframes; ++q)
-        if (!(box[q] > 0.0f) || !std::isfinite(box[q])) return fail(CIR_ERR_NONFINITE, "box lengths must be positive");
+    CIR_TRY(check_boxes(c, nframes, box));
+    const size_t nbx = size_t(nframes) * box_floats(c);
     DevBuf dx, db;
     CIR_TRY(dx.alloc(fb));
-    CIR_TRY(db.alloc(size_t(nframes) * 3 * sizeof(float)));
+    CIR_TRY(db.alloc(nbx * sizeof(float)));
     CIR_HIP(hipMemcpyAsync(dx.p, xyz, fb, hipMemcpyHostToDevice, c->stream));
-    CIR_HIP(hipMemcpyAsync(db.p, box, size_t(nframes) * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CIR_HIP(hipMemcpyAsync(db.p, box, nbx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (c->real_prec == CIR_PREC_FP64)
         return forward_impl<double>(c, nframes, dx.as<float>(), db.as<float>(), times, ftcf_ir);
     return forward_impl<float>(c, nframes, dx.as<float>(), db.as<float>(), times, ftcf_ir);

@@ -79,6 +79,10 @@ extern "C" int cir_main(int argc, char** argv) {
         }
         xyz.resize(size_t(nframes) * natoms * 3);
         times.resize(nframes);
+        if (p.triclinic) {
+            std::printf("WARNING: the synthetic source writes cubic boxes; set triclinic 0.\n");
+            return EXIT_FAILURE;
+        }
         box.assign(size_t(nframes) * 3, boxl);
         st = cir_synth_trajectory(nwater, nframes, seed, boxl, xyz.data(), times.data());
         if (st != CIR_OK) return fail_msg(st);
@@ -94,9 +98,24 @@ extern "C" int cir_main(int argc, char** argv) {
         }
         xyz.resize(size_t(nframes) * natoms * 3);
         times.resize(nframes);
-        box.resize(size_t(nframes) * 3);
-        st = cir_xtc_read(p.xtcf, 0, nframes, natoms, xyz.data(), box.data(), times.data(), nullptr);
+        std::vector<float> box9(size_t(nframes) * 9);
+        st = cir_xtc_read_full(p.xtcf, 0, nframes, natoms, xyz.data(), box9.data(), times.data(), nullptr);
         if (st != CIR_OK) return fail_msg(st);
+        // rectangular boxes go through as their diagonal (calcIR.cu:404-405);
+        // any off-diagonal element switches to the triclinic minimum image
+        bool tri = p.triclinic != 0;
+        for (int f = 0; f < nframes && !tri; ++f)
+            for (int k = 0; k < 9; ++k)
+                if (k % 4 != 0 && box9[9 * size_t(f) + k] != 0.0f) tri = true;
+        if (tri) {
+            p.triclinic = 1;
+            box = box9;
+            std::printf(">>> Triclinic box: using the triclinic minimum image.\n");
+        } else {
+            box.resize(size_t(nframes) * 3);
+            for (int f = 0; f < nframes; ++f)
+                for (int k = 0; k < 3; ++k) box[3 * size_t(f) + k] = box9[9 * size_t(f) + 4 * k];
+        }
     }
     const int nmol = natoms / p.natom_mol, nchrom = nmol * p.nchrom_mol;
     std::printf(">>> Will read the trajectory from: %s.\n", p.xtcf);

@@ -116,6 +116,64 @@ __device__ __forceinline__ T minimage(T dx, T boxl) {
     return dx - boxl * round(dx / boxl);
 }
 
+// Periodic box of one frame.  Rectangular boxes (the reference's case,
+// calcIR.cu:404-405) hold the diagonal; triclinic boxes follow the GROMACS
+// convention a = (a0,0,0), b = (b0,b1,0), c = (c0,c1,c2), stored per frame as
+// the row-major 3x3 matrix of a, b, c.
+template <typename T>
+struct PBox {
+    T a0, b0, b1, c0, c1, c2;
+    bool tri;  // some off-diagonal element is nonzero
+};
+
+// boxk 0: 3 floats (diagonal) per frame; 1: 9 floats (a, b, c rows).
+template <typename T>
+__device__ __forceinline__ PBox<T> load_box(const float* box, int boxk, int f) {
+    PBox<T> B;
+    if (boxk == 0) {
+        const float* q = box + 3 * size_t(f);
+        B.a0 = T(q[0]);
+        B.b1 = T(q[1]);
+        B.c2 = T(q[2]);
+        B.b0 = B.c0 = B.c1 = T(0);
+        B.tri = false;
+    } else {
+        const float* q = box + 9 * size_t(f);
+        B.a0 = T(q[0]);
+        B.b0 = T(q[3]);
+        B.b1 = T(q[4]);
+        B.c0 = T(q[6]);
+        B.c1 = T(q[7]);
+        B.c2 = T(q[8]);
+        B.tri = q[3] != 0.0f || q[6] != 0.0f || q[7] != 0.0f;
+    }
+    return B;
+}
+
+// Minimum image of a separation.  Rectangular: calcIR.cu:1457-1460 per axis
+// (bit for bit the reference form).  Triclinic: the lattice vectors are
+// removed in the order c, b, a, each by the rounded fractional coordinate
+// along its own axis (the GROMACS form for reduced boxes).
+template <typename T>
+__device__ __forceinline__ void minimage3(T& dx, T& dy, T& dz, const PBox<T>& B) {
+#pragma clang fp contract(off)
+    if (!B.tri) {
+        dx = minimage<T>(dx, B.a0);
+        dy = minimage<T>(dy, B.b1);
+        dz = minimage<T>(dz, B.c2);
+        return;
+    }
+    T s = round(dz / B.c2);
+    dx = dx - s * B.c0;
+    dy = dy - s * B.c1;
+    dz = dz - s * B.c2;
+    s = round(dy / B.b1);
+    dx = dx - s * B.b0;
+    dy = dy - s * B.b1;
+    s = round(dx / B.a0);
+    dx = dx - s * B.a0;
+}
+
 // Neumaier-compensated accumulation.
 template <typename T>
 struct Compensated {

@@ -57,6 +57,7 @@ int build_schedule(const cir_params& p, int nframes, const float* times, Schedul
 int gather_frames(cir_ctx* c, const float* xyz_d, const float* box_d, const int* frames, int nb, float* xyzc,
                   float* boxc);
 int auto_chunk(const cir_ctx* c, int nuniq);
+int check_boxes(const cir_ctx* c, int nframes, const float* box);
 
 // State the lineshape adjoint keeps between its forward and backward calls.
 struct AdjState;
@@ -76,6 +77,7 @@ struct cir_ctx {
     size_t esz = 4;
     hipStream_t stream = nullptr;
     cir::MapCoef mc{};              // map coefficients in use (the table row unless overridden)
+    int boxk = 0;                   // 0: 3 box floats per frame (rectangular), 1: 9 (triclinic)
     std::vector<double> tcf[3], ftcf[3], sw, pw, rw, rmw;
     int64_t nsteps = 0, nframes_used = 0, hist_skipped = 0;
     double seconds = 0.0;
@@ -85,3 +87,8 @@ struct cir_ctx {
     long long phase_cnt[CIR_NPHASE] = {};
     std::unique_ptr<cir::AdjState, cir::AdjDeleter> adj;
 };
+
+namespace cir {
+// Floats per frame in the context's box arrays.
+inline int box_floats(const cir_ctx* c) { return c->boxk ? 9 : 3; }
+}  // namespace cir

@@ -26,11 +26,10 @@ constexpr int kFieldTile = 256;  // molecules staged per LDS tile
 // The discrete cutoff decision (calcIR.cu:1106-1113), always in FP32 with
 // the reference's operation order, identical in every precision form.
 __device__ __forceinline__ bool within_cutoff(float ox, float oy, float oz, float hx, float hy,
-                                              float hz, float bx, float by, float bz, float cutoff) {
+                                              float hz, const PBox<float>& B, float cutoff) {
 #pragma clang fp contract(off)
-    const float d0 = minimage<float>(ox - hx, bx);
-    const float d1 = minimage<float>(oy - hy, by);
-    const float d2 = minimage<float>(oz - hz, bz);
+    float d0 = ox - hx, d1 = oy - hy, d2 = oz - hz;
+    minimage3<float>(d0, d1, d2, B);
     const float r = __fsqrt_rn(d0 * d0 + d1 * d1 + d2 * d2);
     return !(r > cutoff);
 }
@@ -38,7 +37,7 @@ __device__ __forceinline__ bool within_cutoff(float ox, float oy, float oz, floa
 template <typename T>
 __global__ void __launch_bounds__(kFieldBlock)
 eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float cutoff,
-                   const float* __restrict__ xyz, const float* __restrict__ box, MapT<T> mc,
+                   const float* __restrict__ xyz, const float* __restrict__ box, int boxk, MapT<T> mc,
                    T* __restrict__ sites) {
     __shared__ float sO[3][kFieldTile];
     __shared__ T sA[9][kFieldTile];  // H1, H2, M' (x,y,z each)
@@ -47,14 +46,16 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
     const int nmol = natoms / natom_mol;
     const int N = nmol * nchrom_mol;
     const float* x = xyz + size_t(f) * natoms * 3;
-    const float bxf = box[3 * f + 0], byf = box[3 * f + 1], bzf = box[3 * f + 2];
-    const T bx = T(bxf), by = T(byf), bz = T(bzf);
+    const PBox<float> Bf = load_box<float>(box, boxk, f);
+    const PBox<T> Bt = load_box<T>(box, boxk, f);
     const bool have_m = natom_mol >= 4;
 
     const int chrom = blockIdx.x * kFieldBlock + threadIdx.x;
     const bool active = chrom < N;
     const int n = active ? chrom / nchrom_mol : 0;
-    const int hatom = (chrom % 2 == 0) ? 1 : 2;  // calcIR.cu:1061-1070
+    // calcIR.cu:1061-1070 picks H1/H2 by chrom%2; with one OH per molecule
+    // (nchrom_mol 1) every chromophore is H1.  Identical for nchrom_mol 2.
+    const int hatom = 1 + chrom % nchrom_mol;
     float hxf = 0, hyf = 0, hzf = 0;
     if (active) {
         hxf = x[3 * (n * natom_mol + hatom) + 0];
@@ -78,32 +79,47 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
                 sA[0 + c][t] = T(mx[3 + c]);
                 sA[3 + c][t] = T(mx[6 + c]);
             }
+            const T o[3] = {T(ox), T(oy), T(oz)};
+            T m[3];
             if (have_m) {
-                T m[3] = {T(mx[9]), T(mx[10]), T(mx[11])};
+                m[0] = T(mx[9]);
+                m[1] = T(mx[10]);
+                m[2] = T(mx[11]);
                 if (imodel != 0) {
                     // calcIR.cu:1124-1140: M moved to 0.015 nm along O->M
-                    const T o[3] = {T(ox), T(oy), T(oz)};
-                    const T bb[3] = {bx, by, bz};
-                    T om[3];
-                    for (int c = 0; c < 3; ++c) om[c] = minimage<T>(m[c] - o[c], bb[c]);
+                    T om[3] = {m[0] - o[0], m[1] - o[1], m[2] - o[2]};
+                    minimage3<T>(om[0], om[1], om[2], Bt);
                     const T rom = sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
                     for (int c = 0; c < 3; ++c) m[c] = o[c] + T(kOMTip4p) * om[c] / rom;
                 }
-                for (int c = 0; c < 3; ++c) sA[6 + c][t] = m[c];
+            } else {
+                // three-site water: the negative site is built on the HOH
+                // bisector, 0.015 nm from O (the TIP4P geometry)
+                T h1[3] = {T(mx[3]) - o[0], T(mx[4]) - o[1], T(mx[5]) - o[2]};
+                T h2[3] = {T(mx[6]) - o[0], T(mx[7]) - o[1], T(mx[8]) - o[2]};
+                minimage3<T>(h1[0], h1[1], h1[2], Bt);
+                minimage3<T>(h2[0], h2[1], h2[2], Bt);
+                const T r1 = sqrt(h1[0] * h1[0] + h1[1] * h1[1] + h1[2] * h1[2]);
+                const T r2 = sqrt(h2[0] * h2[0] + h2[1] * h2[1] + h2[2] * h2[2]);
+                T bi[3];
+                for (int c = 0; c < 3; ++c) bi[c] = h1[c] / r1 + h2[c] / r2;
+                const T rb = sqrt(bi[0] * bi[0] + bi[1] * bi[1] + bi[2] * bi[2]);
+                for (int c = 0; c < 3; ++c) m[c] = o[c] + T(kOMTip4p) * bi[c] / rb;
             }
+            for (int c = 0; c < 3; ++c) sA[6 + c][t] = m[c];
         }
         __syncthreads();
         if (!active) continue;
         for (int t = 0; t < cnt; ++t) {
             const int m = m0 + t;
             if (m == n) continue;  // calcIR.cu:1099
-            if (!within_cutoff(sO[0][t], sO[1][t], sO[2][t], hxf, hyf, hzf, bxf, byf, bzf, cutoff))
-                continue;
-            const int nsite = have_m ? 3 : 2;
-            for (int s = 0; s < nsite; ++s) {
-                const T dx = minimage<T>(hx - sA[3 * s + 0][t], bx) * bohr;
-                const T dy = minimage<T>(hy - sA[3 * s + 1][t], by) * bohr;
-                const T dz = minimage<T>(hz - sA[3 * s + 2][t], bz) * bohr;
+            if (!within_cutoff(sO[0][t], sO[1][t], sO[2][t], hxf, hyf, hzf, Bf, cutoff)) continue;
+            for (int s = 0; s < 3; ++s) {
+                T dx = hx - sA[3 * s + 0][t], dy = hy - sA[3 * s + 1][t], dz = hz - sA[3 * s + 2][t];
+                minimage3<T>(dx, dy, dz, Bt);
+                dx = dx * bohr;
+                dy = dy * bohr;
+                dz = dz * bohr;
                 const T r = sqrt(dx * dx + dy * dy + dz * dz);
                 const T q = (s < 2) ? T(kQH) : T(-kQM);
                 const T qr3 = q / (r * r * r);
@@ -118,7 +134,8 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
     // own OH unit vector, TDM position and maps (calcIR.cu:1047-1080, 1253-1304)
     const float* no = x + 3 * size_t(n * natom_mol);
     const T ox = T(no[0]), oy = T(no[1]), oz = T(no[2]);
-    T ux = minimage<T>(hx - ox, bx), uy = minimage<T>(hy - oy, by), uz = minimage<T>(hz - oz, bz);
+    T ux = hx - ox, uy = hy - oy, uz = hz - oz;
+    minimage3<T>(ux, uy, uz, Bt);
     const T r = sqrt(ux * ux + uy * uy + uz * uz);
     ux /= r;
     uy /= r;
@@ -138,9 +155,11 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
     S[(CIR_SITE_U + 0) * N + chrom] = ux;
     S[(CIR_SITE_U + 1) * N + chrom] = uy;
     S[(CIR_SITE_U + 2) * N + chrom] = uz;
-    S[(CIR_SITE_R + 0) * N + chrom] = minimage<T>(ox + T(kTdmOffset) * ux, bx);
-    S[(CIR_SITE_R + 1) * N + chrom] = minimage<T>(oy + T(kTdmOffset) * uy, by);
-    S[(CIR_SITE_R + 2) * N + chrom] = minimage<T>(oz + T(kTdmOffset) * uz, bz);
+    T rx = ox + T(kTdmOffset) * ux, ry = oy + T(kTdmOffset) * uy, rz = oz + T(kTdmOffset) * uz;
+    minimage3<T>(rx, ry, rz, Bt);  // calcIR.cu:1289-1291 (wrapped; only differences are used)
+    S[(CIR_SITE_R + 0) * N + chrom] = rx;
+    S[(CIR_SITE_R + 1) * N + chrom] = ry;
+    S[(CIR_SITE_R + 2) * N + chrom] = rz;
     S[(CIR_SITE_MU + 0) * N + chrom] = ux * mup * xx;
     S[(CIR_SITE_MU + 1) * N + chrom] = uy * mup * xx;
     S[(CIR_SITE_MU + 2) * N + chrom] = uz * mup * xx;
@@ -157,7 +176,7 @@ constexpr int kHTile = 64;
 template <typename T>
 __global__ void __launch_bounds__(256)
 hamiltonian_kernel(int N, int ld, int nchrom_mol, int hod, T avef, T wi0, T wi1, T wi2,
-                   const float* __restrict__ box, const T* __restrict__ sites, T* __restrict__ H) {
+                   const float* __restrict__ box, int boxk, const T* __restrict__ sites, T* __restrict__ H) {
     // per-tile site data: E, w, x, p, s = x*mup, u(3), r(3)
     __shared__ T rs[11][kHTile];
     __shared__ T cs[11][kHTile];
@@ -185,7 +204,7 @@ hamiltonian_kernel(int N, int ld, int nchrom_mol, int hod, T avef, T wi0, T wi1,
         }
     }
     __syncthreads();
-    const T bb[3] = {T(box[3 * f]), T(box[3 * f + 1]), T(box[3 * f + 2])};
+    const PBox<T> bb = load_box<T>(box, boxk, f);
     const int tx = tid % kHTile, ty = tid / kHTile;
     const int i = i0 + tx;
     if (i >= ld) return;
@@ -217,49 +236,51 @@ hamiltonian_kernel(int N, int ld, int nchrom_mol, int hod, T avef, T wi0, T wi1,
 
 template <typename T>
 int launch_eproj_sites(int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel,
-                       float cutoff, const float* xyz, const float* box, const MapCoef& mc,
+                       float cutoff, const float* xyz, const float* box, int boxk, const MapCoef& mc,
                        void* sites, hipStream_t s) {
     const int N = (natoms / natom_mol) * nchrom_mol;
     dim3 grid(unsigned(ceil_div(N, kFieldBlock)), unsigned(nframes));
     hipLaunchKernelGGL(eproj_sites_kernel<T>, grid, dim3(kFieldBlock), 0, s, natoms, natom_mol,
-                       nchrom_mol, imodel, cutoff, xyz, box, map_t<T>(mc), static_cast<T*>(sites));
+                       nchrom_mol, imodel, cutoff, xyz, box, boxk, map_t<T>(mc), static_cast<T*>(sites));
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
 
 template <typename T>
 int launch_hamiltonian(int nframes, int N, int ld, int nchrom_mol, int ispecies, const MapCoef& mc,
-                       float avef, const float* box, const void* sites, void* H, hipStream_t s) {
+                       float avef, const float* box, int boxk, const void* sites, void* H, hipStream_t s) {
     dim3 grid(unsigned(ceil_div(ld, kHTile)), unsigned(ceil_div(N, kHTile)), unsigned(nframes));
     const int hod = (ispecies == 1 || ispecies == 2) ? 1 : 0;
     hipLaunchKernelGGL(hamiltonian_kernel<T>, grid, dim3(256), 0, s, N, ld, nchrom_mol, hod, T(avef),
-                       T(mc.wi[0]), T(mc.wi[1]), T(mc.wi[2]), box, static_cast<const T*>(sites),
+                       T(mc.wi[0]), T(mc.wi[1]), T(mc.wi[2]), box, boxk, static_cast<const T*>(sites),
                        static_cast<T*>(H));
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
 
 int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, const MapCoef& mc,
-                float cutoff, const float* xyz_d, const float* box_d, void* sites_d, hipStream_t s) {
-    if (nframes <= 0 || natom_mol < 3 || nchrom_mol <= 0 || natoms <= 0 || natoms % natom_mol || !xyz_d ||
-        !box_d || !sites_d)
+                float cutoff, const float* xyz_d, const float* box_d, int boxk, void* sites_d, hipStream_t s) {
+    if (nframes <= 0 || natom_mol < 3 || nchrom_mol < 1 || nchrom_mol > 2 || natoms <= 0 || natoms % natom_mol ||
+        !xyz_d || !box_d || !sites_d || (boxk != 0 && boxk != 1))
         return fail(CIR_ERR_ARG, "cir_eproj_sites: bad sizes or null pointer");
     if (int64_t(nframes) > 65535) return fail(CIR_ERR_ARG, "cir_eproj_sites: nframes > 65535");
     if (prec == CIR_PREC_FP64)
-        return launch_eproj_sites<double>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, mc,
-                                          sites_d, s);
-    return launch_eproj_sites<float>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, mc,
-                                     sites_d, s);
+        return launch_eproj_sites<double>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, boxk,
+                                          mc, sites_d, s);
+    return launch_eproj_sites<float>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, boxk,
+                                     mc, sites_d, s);
 }
 
 int hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies, const MapCoef& mc, float avef,
-                const float* box_d, const void* sites_d, void* H_d, hipStream_t s) {
-    if (nframes <= 0 || n <= 0 || ld < n || nchrom_mol <= 0 || !box_d || !sites_d || !H_d)
+                const float* box_d, int boxk, const void* sites_d, void* H_d, hipStream_t s) {
+    if (nframes <= 0 || n <= 0 || ld < n || nchrom_mol <= 0 || !box_d || !sites_d || !H_d ||
+        (boxk != 0 && boxk != 1))
         return fail(CIR_ERR_ARG, "cir_hamiltonian: bad sizes or null pointer");
     if (nframes > 65535) return fail(CIR_ERR_ARG, "cir_hamiltonian: nframes > 65535");
     if (prec == CIR_PREC_FP64)
-        return launch_hamiltonian<double>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d, H_d, s);
-    return launch_hamiltonian<float>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d, H_d, s);
+        return launch_hamiltonian<double>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, boxk, sites_d, H_d,
+                                          s);
+    return launch_hamiltonian<float>(nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, boxk, sites_d, H_d, s);
 }
 
 }  // namespace cir
@@ -272,7 +293,7 @@ extern "C" int cir_eproj_sites(int prec, int nframes, int natoms, int natom_mol,
                                void* stream) {
     MapCoef mc;
     if (!map_coef(imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map/species not in the map table");
-    return eproj_sites(prec, nframes, natoms, natom_mol, nchrom_mol, imodel, mc, cutoff, xyz_d, box_d, sites_d,
+    return eproj_sites(prec, nframes, natoms, natom_mol, nchrom_mol, imodel, mc, cutoff, xyz_d, box_d, 0, sites_d,
                        as_stream(stream));
 }
 
@@ -281,6 +302,6 @@ extern "C" int cir_hamiltonian(int prec, int nframes, int n, int ld, int nchrom_
                                void* H_d, void* stream) {
     MapCoef mc;
     if (!map_coef(imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map/species not in the map table");
-    return hamiltonian(prec, nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, sites_d, H_d,
+    return hamiltonian(prec, nframes, n, ld, nchrom_mol, ispecies, mc, avef, box_d, 0, sites_d, H_d,
                        as_stream(stream));
 }

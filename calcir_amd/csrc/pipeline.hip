@@ -166,12 +166,38 @@ int finalize(cir_ctx* c, const std::vector<double>& out /* ns*nt*6 */, const std
 // Gathers trajectory frames frames[0..nb) into contiguous device buffers.
 int gather_frames(cir_ctx* c, const float* xyz_d, const float* box_d, const int* frames, int nb, float* xyzc, float* boxc) {
     const size_t fb = size_t(c->natoms) * 3 * sizeof(float);
+    const int bs = box_floats(c);
     for (int q = 0; q < nb; ++q) {
         CIR_HIP(hipMemcpyAsync(reinterpret_cast<char*>(xyzc) + q * fb,
                                reinterpret_cast<const char*>(xyz_d) + size_t(frames[q]) * fb, fb, hipMemcpyDeviceToDevice,
                                c->stream));
-        CIR_HIP(hipMemcpyAsync(boxc + 3 * q, box_d + 3 * size_t(frames[q]), 3 * sizeof(float), hipMemcpyDeviceToDevice,
-                               c->stream));
+        CIR_HIP(hipMemcpyAsync(boxc + size_t(bs) * q, box_d + size_t(bs) * frames[q], bs * sizeof(float),
+                               hipMemcpyDeviceToDevice, c->stream));
+    }
+    return CIR_OK;
+}
+
+// Host check of the boxes of a trajectory: positive finite lengths; for
+// triclinic boxes the GROMACS form (a along x, b in the xy plane) reduced so
+// that |b_x|, |c_x| <= a_x/2 and |c_y| <= b_y/2, which the minimum image needs.
+int check_boxes(const cir_ctx* c, int nframes, const float* box) {
+    const int bs = box_floats(c);
+    for (int f = 0; f < nframes; ++f) {
+        const float* q = box + size_t(bs) * f;
+        for (int i = 0; i < bs; ++i)
+            if (!std::isfinite(q[i])) return fail(CIR_ERR_NONFINITE, "non-finite box");
+        if (bs == 3) {
+            if (!(q[0] > 0) || !(q[1] > 0) || !(q[2] > 0)) return fail(CIR_ERR_NONFINITE, "non-positive box length");
+            continue;
+        }
+        if (!(q[0] > 0) || !(q[4] > 0) || !(q[8] > 0)) return fail(CIR_ERR_NONFINITE, "non-positive box length");
+        if (q[1] != 0.0f || q[2] != 0.0f || q[5] != 0.0f)
+            return fail(CIR_ERR_ARG, "triclinic box must have a along x and b in the xy plane (frame " +
+                                         std::to_string(f) + ")");
+        const float tol = 1.0f + 1e-5f;
+        if (std::fabs(q[3]) > 0.5f * q[0] * tol || std::fabs(q[6]) > 0.5f * q[0] * tol ||
+            std::fabs(q[7]) > 0.5f * q[4] * tol)
+            return fail(CIR_ERR_ARG, "triclinic box is not reduced (frame " + std::to_string(f) + ")");
     }
     return CIR_OK;
 }
@@ -240,6 +266,7 @@ int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d
     f.val(p.omegaStart);
     f.val(p.omegaStop);
     f.val(p.cutoff);
+    f.val(p.triclinic);
     f.val(c->mc);
     f.val(S.nslot);
     f.val(int64_t(S.uniq.size()));
@@ -321,7 +348,7 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     const int set = 0;
     hipStream_t sp = st;
     CIR_TRY(xyzc.alloc(size_t(B) * c->natoms * 3 * sizeof(float)));
-    CIR_TRY(boxc.alloc(size_t(B) * 3 * sizeof(float)));
+    CIR_TRY(boxc.alloc(size_t(B) * box_floats(c) * sizeof(float)));
     CIR_TRY(sites[set].alloc(size_t(B) * CIR_NSITE * N * esz));
     CIR_TRY(V[set].alloc(size_t(B) * N * ld * esz));
     CIR_TRY(w[set].alloc(size_t(B) * N * sizeof(double)));
@@ -379,11 +406,11 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
         {
             ProfScope ps(pf, CIR_PH_SITES, st);
             CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc,
-                                    p.cutoff, xyzc.as<float>(), boxc.as<float>(), sites[set].p, st));
+                                    p.cutoff, xyzc.as<float>(), boxc.as<float>(), c->boxk, sites[set].p, st));
         }
         {
             ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
-            CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxc.as<float>(),
+            CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxc.as<float>(), c->boxk,
                                     sites[set].p, H.p, st));
         }
         CIR_TRY(eigh_batched(c->prop_prec, nb, N, ld, H.p, V[set].p, w[set].as<double>(), work.p, st, pf));
@@ -463,7 +490,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
     const int ldf = 2 * N;
     DevBuf xyz1, box1, sites1, sites0, H, V, w, work, m2, F, G, Y, out, acc, skip;
     CIR_TRY(xyz1.alloc(size_t(c->natoms) * 3 * sizeof(float)));
-    CIR_TRY(box1.alloc(3 * sizeof(float)));
+    CIR_TRY(box1.alloc(box_floats(c) * sizeof(float)));
     CIR_TRY(sites1.alloc(size_t(CIR_NSITE) * N * esz));
     CIR_TRY(sites0.alloc(size_t(CIR_NSITE) * N * esz));
     CIR_TRY(H.alloc(size_t(N) * ld * esz));
@@ -489,11 +516,11 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
             {
                 ProfScope ps(pf, CIR_PH_SITES, st);
                 CIR_TRY(eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc,
-                                        p.cutoff, xyz1.as<float>(), box1.as<float>(), sites1.p, st));
+                                        p.cutoff, xyz1.as<float>(), box1.as<float>(), c->boxk, sites1.p, st));
             }
             {
                 ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
-                CIR_TRY(hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, box1.as<float>(),
+                CIR_TRY(hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, box1.as<float>(), c->boxk,
                                         sites1.p, H.p, st));
             }
             CIR_TRY(eigh_batched(c->prop_prec, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
@@ -570,6 +597,8 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     if (!map_coef(p->imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map must be 0 or 1");
     if (p->natom_mol < 3 || p->nchrom_mol < 1 || natoms <= 0 || natoms % p->natom_mol)
         return fail(CIR_ERR_ARG, "natoms must be a positive multiple of natom_mol (>= 3)");
+    if (p->nchrom_mol > 2) return fail(CIR_ERR_ARG, "nchrom_mol must be 1 or 2 (OH stretches of a water)");
+    if (p->triclinic != 0 && p->triclinic != 1) return fail(CIR_ERR_ARG, "triclinic must be 0 or 1");
     if (p->ntcfpoints <= 0 || p->nsamples <= 0 || p->omegaStep <= 0 || p->nzeros < 0 || !(p->t1 > 0) || !(p->dt > 0))
         return fail(CIR_ERR_ARG, "ntcfpoints, nsamples, omegaStep, t1 and dt must be positive, nzeros >= 0");
     if (cir_nomega(p) <= 0) return fail(CIR_ERR_ARG, "omegaStop must not be below omegaStart");
@@ -580,6 +609,7 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     c->imodel = imodel;
     c->ispecies = ispecies;
     c->mc = mc;
+    c->boxk = p->triclinic ? 1 : 0;
     c->natoms = natoms;
     c->nmol = natoms / p->natom_mol;
     c->N = c->nmol * p->nchrom_mol;
@@ -596,6 +626,13 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
 
 extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
     if (!c || !xyz_d || !box_d || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_run_device: bad arguments");
+    {
+        // the boxes are few bytes: check them on the host
+        std::vector<float> hb(size_t(nframes) * box_floats(c));
+        CIR_HIP(hipMemcpyAsync(hb.data(), box_d, hb.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        CIR_HIP(hipStreamSynchronize(c->stream));
+        CIR_TRY(check_boxes(c, nframes, hb.data()));
+    }
     c->have = false;
     for (int i = 0; i < CIR_NPHASE; ++i) {
         c->phase_sec[i] = 0.0;
@@ -637,13 +674,13 @@ extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* b
     const size_t nx = size_t(nframes) * c->natoms * 3;
     for (size_t i = 0; i < nx; ++i)
         if (!std::isfinite(xyz[i])) return fail(CIR_ERR_NONFINITE, "non-finite coordinate in the trajectory");
-    for (size_t i = 0; i < size_t(nframes) * 3; ++i)
-        if (!std::isfinite(box[i]) || !(box[i] > 0)) return fail(CIR_ERR_NONFINITE, "non-finite or non-positive box");
+    CIR_TRY(check_boxes(c, nframes, box));
+    const size_t nbx = size_t(nframes) * box_floats(c);
     DevBuf dx, db;
     CIR_TRY(dx.alloc(nx * sizeof(float)));
-    CIR_TRY(db.alloc(size_t(nframes) * 3 * sizeof(float)));
+    CIR_TRY(db.alloc(nbx * sizeof(float)));
     CIR_HIP(hipMemcpyAsync(dx.p, xyz, nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    CIR_HIP(hipMemcpyAsync(db.p, box, size_t(nframes) * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CIR_HIP(hipMemcpyAsync(db.p, box, nbx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     return cir_run_device(c, nframes, dx.as<float>(), db.as<float>(), times);
 }
 

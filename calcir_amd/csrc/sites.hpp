@@ -24,11 +24,9 @@ MapT<T> map_t(const MapCoef& c) {
 // operation order is symmetric in i and j, so H(i,j) == H(j,i) bit for bit.
 template <typename T>
 __device__ __forceinline__ T tdc_geometry(T uix, T uiy, T uiz, T rix, T riy, T riz, T ujx, T ujy, T ujz, T rjx,
-                                          T rjy, T rjz, const T* bb) {
-    T dr[3];
-    dr[0] = minimage<T>(rix - rjx, bb[0]);
-    dr[1] = minimage<T>(riy - rjy, bb[1]);
-    dr[2] = minimage<T>(riz - rjz, bb[2]);
+                                          T rjy, T rjz, const PBox<T>& bb) {
+    T dr[3] = {rix - rjx, riy - rjy, riz - rjz};
+    minimage3<T>(dr[0], dr[1], dr[2], bb);
     const T r = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
     const T inv = T(1) / r;
     const T d0 = dr[0] * inv, d1 = dr[1] * inv, d2 = dr[2] * inv;

@@ -9,10 +9,12 @@ namespace cir {
 // Field projection + site records and the Hamiltonian with an explicit map
 // coefficient set (k_sites.hip); the cir_eproj_sites / cir_hamiltonian entry
 // points resolve the set from the map table.
+// boxk 0: box_d holds 3 floats (rectangular diagonal) per frame; 1: 9 floats
+// (triclinic a, b, c rows).
 int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, const MapCoef& mc,
-                float cutoff, const float* xyz_d, const float* box_d, void* sites_d, hipStream_t s);
+                float cutoff, const float* xyz_d, const float* box_d, int boxk, void* sites_d, hipStream_t s);
 int hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies, const MapCoef& mc, float avef,
-                const float* box_d, const void* sites_d, void* H_d, hipStream_t s);
+                const float* box_d, int boxk, const void* sites_d, void* H_d, hipStream_t s);
 
 size_t eigh_workspace(int prec, int batch, int n, int ld);
 int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w, void* work, hipStream_t s,

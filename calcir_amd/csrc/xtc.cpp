@@ -504,8 +504,12 @@ extern "C" int cir_xtc_scan(const char* path, int* natoms, int* nframes) {
     return CIR_OK;
 }
 
-extern "C" int cir_xtc_read(const char* path, int first, int count, int natoms, float* xyz, float* box,
-                            float* times, int* steps) {
+namespace {
+
+// Frames first..first+count-1; box gets bs floats per frame: 3 (the
+// diagonal) or 9 (the full matrix, rows a, b, c).
+int read_frames(const char* path, int first, int count, int natoms, float* xyz, float* box, int bs, float* times,
+                int* steps) {
     if (!path || first < 0 || count < 0 || !xyz) return fail(CIR_ERR_ARG, "cir_xtc_read: bad argument");
     FILE* f = std::fopen(path, "rb");
     if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
@@ -529,8 +533,10 @@ extern "C" int cir_xtc_read(const char* path, int first, int count, int natoms, 
             return fail(CIR_ERR_IO, "xtc: corrupt coordinates in frame " + std::to_string(fr));
         }
         if (fr >= first) {
-            if (box)  // the diagonal, as the reference uses it (calcIR.cu:404-405)
+            if (box && bs == 3)  // the diagonal, as the reference uses it (calcIR.cu:404-405)
                 for (int k = 0; k < 3; ++k) box[3 * size_t(got) + k] = bx[4 * k];
+            if (box && bs == 9)
+                for (int k = 0; k < 9; ++k) box[9 * size_t(got) + k] = bx[k];
             if (times) times[got] = t;
             if (steps) steps[got] = step;
             ++got;
@@ -540,8 +546,8 @@ extern "C" int cir_xtc_read(const char* path, int first, int count, int natoms, 
     return CIR_OK;
 }
 
-extern "C" int cir_xtc_write(const char* path, int natoms, int nframes, const float* xyz, const float* box,
-                             const float* times, float precision, int append) {
+int write_frames(const char* path, int natoms, int nframes, const float* xyz, const float* box, int bs,
+                 const float* times, float precision, int append) {
     if (!path || natoms < 0 || nframes < 0 || (nframes > 0 && !xyz) || !(precision > 0.0f))
         return fail(CIR_ERR_ARG, "cir_xtc_write: bad argument");
     FILE* f = std::fopen(path, append ? "ab" : "wb");
@@ -553,7 +559,12 @@ extern "C" int cir_xtc_write(const char* path, int natoms, int nframes, const fl
         x.i32(fr);
         x.f32(times ? times[fr] : float(fr));
         for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) x.f32((r == c && box) ? box[3 * size_t(fr) + r] : 0.0f);
+            for (int c = 0; c < 3; ++c) {
+                float v = 0.0f;
+                if (box && bs == 3 && r == c) v = box[3 * size_t(fr) + r];
+                if (box && bs == 9) v = box[9 * size_t(fr) + 3 * r + c];
+                x.f32(v);
+            }
         if (!write_coords(x, natoms, xyz + size_t(fr) * 3 * natoms, precision)) {
             std::fclose(f);
             return fail(CIR_ERR_ARG, "xtc: coordinates out of range at this precision");
@@ -565,4 +576,26 @@ extern "C" int cir_xtc_write(const char* path, int natoms, int nframes, const fl
     }
     std::fclose(f);
     return CIR_OK;
+}
+
+}  // namespace
+
+extern "C" int cir_xtc_read(const char* path, int first, int count, int natoms, float* xyz, float* box,
+                            float* times, int* steps) {
+    return read_frames(path, first, count, natoms, xyz, box, 3, times, steps);
+}
+
+extern "C" int cir_xtc_read_full(const char* path, int first, int count, int natoms, float* xyz, float* box9,
+                                 float* times, int* steps) {
+    return read_frames(path, first, count, natoms, xyz, box9, 9, times, steps);
+}
+
+extern "C" int cir_xtc_write(const char* path, int natoms, int nframes, const float* xyz, const float* box,
+                             const float* times, float precision, int append) {
+    return write_frames(path, natoms, nframes, xyz, box, 3, times, precision, append);
+}
+
+extern "C" int cir_xtc_write_full(const char* path, int natoms, int nframes, const float* xyz, const float* box9,
+                                  const float* times, float precision, int append) {
+    return write_frames(path, natoms, nframes, xyz, box9, 9, times, precision, append);
 }

@@ -116,6 +116,12 @@ typedef struct cir_params {
                                   when it fits in device memory, else recompute them chunk
                                   by chunk in the reverse sweep; 1 = always keep; 2 = always
                                   recompute                                            */
+    int   triclinic;           /* key "triclinic": 1 = the box arrays given to cir_run,
+                                  cir_run_device and cir_forward_ls hold 9 floats per frame,
+                                  the rows a, b, c of a GROMACS box (a along x, b in the xy
+                                  plane, reduced); 0 = 3 floats, the rectangular diagonal as
+                                  the reference (calcIR.cu:404-405). Set by the CLI when the
+                                  trajectory's boxes have off-diagonal elements          */
 } cir_params;
 
 /* Results of a run, all arrays caller-allocated (host).  NULL members are
@@ -187,6 +193,12 @@ int   cir_xtc_read(const char* path, int first, int count, int natoms, float* xy
                    int* steps);
 int   cir_xtc_write(const char* path, int natoms, int nframes, const float* xyz, const float* box,
                     const float* times, float precision, int append);
+/* The same with the full box matrix, 9 floats per frame (rows a, b, c; the
+ * layout of the triclinic key's box arrays). */
+int   cir_xtc_read_full(const char* path, int first, int count, int natoms, float* xyz, float* box9,
+                        float* times, int* steps);
+int   cir_xtc_write_full(const char* path, int natoms, int nframes, const float* xyz, const float* box9,
+                         const float* times, float precision, int append);
 /* The drop-in CLI: argv as calcIR.exe (calcIR.cu:9-18). */
 int   cir_main(int argc, char** argv);
 

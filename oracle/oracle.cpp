@@ -88,17 +88,75 @@ inline T minimage(T dx, T boxl) {
     return dx - boxl * std::round(dx / boxl);
 }
 
+// Periodic box of a frame.  Rectangular (the reference, calcIR.cu:404-405):
+// the diagonal, 3 floats per frame.  With cfg->triclinic: 9 floats per frame,
+// the rows a = (a0,0,0), b = (b0,b1,0), c = (c0,c1,c2) of a GROMACS box.
+template <typename T>
+struct Box {
+    T a0, b0, b1, c0, c1, c2;
+    bool tri;
+};
+
+template <typename T>
+Box<T> make_box(const orc_config* cfg, const float* q) {
+    Box<T> B;
+    if (!cfg->triclinic) {
+        B.a0 = T(q[0]);
+        B.b1 = T(q[1]);
+        B.c2 = T(q[2]);
+        B.b0 = B.c0 = B.c1 = T(0);
+        B.tri = false;
+    } else {
+        B.a0 = T(q[0]);
+        B.b0 = T(q[3]);
+        B.b1 = T(q[4]);
+        B.c0 = T(q[6]);
+        B.c1 = T(q[7]);
+        B.c2 = T(q[8]);
+        B.tri = q[3] != 0.0f || q[6] != 0.0f || q[7] != 0.0f;
+    }
+    return B;
+}
+
+inline int box_floats(const orc_config* cfg) { return cfg->triclinic ? 9 : 3; }
+
+// Minimum image of d.  Rectangular: per axis as calcIR.cu:1457-1460.
+// Triclinic: remove c, then b, then a, each by the rounded coordinate along
+// its own axis (the GROMACS form for reduced boxes).
+template <typename T>
+void mi3(T* d, const Box<T>& B) {
+    if (!B.tri) {
+        d[0] = minimage<T>(d[0], B.a0);
+        d[1] = minimage<T>(d[1], B.b1);
+        d[2] = minimage<T>(d[2], B.c2);
+        return;
+    }
+    T s = std::round(d[2] / B.c2);
+    d[0] = d[0] - s * B.c0;
+    d[1] = d[1] - s * B.c1;
+    d[2] = d[2] - s * B.c2;
+    s = std::round(d[1] / B.b1);
+    d[0] = d[0] - s * B.b0;
+    d[1] = d[1] - s * B.b1;
+    s = std::round(d[0] / B.a0);
+    d[0] = d[0] - s * B.a0;
+}
+
 // The cutoff decision of calcIR.cu:1106-1113 is a discrete selection.  It is
 // always evaluated in FP32 from the FP32 input coordinates, with the
 // reference's operation order, so that every precision form selects the same
 // molecules and precision comparisons are not polluted by selection flips.
-inline bool within_cutoff_f32(const float* mo, const float* nh, const float* box, float cutoff) {
-    float d0 = minimage<float>(mo[0] - nh[0], box[0]);
-    float d1 = minimage<float>(mo[1] - nh[1], box[1]);
-    float d2 = minimage<float>(mo[2] - nh[2], box[2]);
-    float r = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+inline bool within_cutoff_f32(const float* mo, const float* nh, const Box<float>& B, float cutoff) {
+    float d[3] = {mo[0] - nh[0], mo[1] - nh[1], mo[2] - nh[2]};
+    mi3<float>(d, B);
+    float r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     return !(r > cutoff);
 }
+
+// The H atom of a chromophore: calcIR.cu:1061-1070 takes H1 or H2 by
+// chrom%2; with one OH per molecule (nchrom_mol 1) it is always H1.  The two
+// agree for nchrom_mol 2.
+inline int h_atom(const orc_config* cfg, int chrom) { return 1 + chrom % cfg->nchrom_mol; }
 
 int threads_of(const orc_config* cfg) {
 #ifdef _OPENMP
@@ -116,17 +174,18 @@ template <typename T>
 void eproj_t(const orc_config* cfg, const float* x, int natoms, const float* boxf, double* out) {
     const int nam = cfg->natom_mol, ncm = cfg->nchrom_mol;
     const int nmol = natoms / nam, nchrom = nmol * ncm;
-    const T box[3] = {T(boxf[0]), T(boxf[1]), T(boxf[2])};
+    const Box<T> box = make_box<T>(cfg, boxf);
+    const Box<float> boxs = make_box<float>(cfg, boxf);
     const T bohr = T(kBohrPerNm);
 #pragma omp parallel for schedule(static) num_threads(threads_of(cfg))
     for (int chrom = 0; chrom < nchrom; ++chrom) {
         const int n = chrom / ncm;
-        // calcIR.cu:1061-1070: chrom%2 picks H1 or H2
-        const float* nhf = &x[3 * (n * nam + ((chrom % 2 == 0) ? 1 : 2))];
+        const float* nhf = &x[3 * (n * nam + h_atom(cfg, chrom))];
         const float* nof = &x[3 * (n * nam)];
         T nh[3] = {T(nhf[0]), T(nhf[1]), T(nhf[2])};
         T oh[3];
-        for (int c = 0; c < 3; ++c) oh[c] = minimage<T>(nh[c] - T(nof[c]), box[c]);
+        for (int c = 0; c < 3; ++c) oh[c] = nh[c] - T(nof[c]);
+        mi3<T>(oh, box);
         T r = std::sqrt(oh[0] * oh[0] + oh[1] * oh[1] + oh[2] * oh[2]);
         for (int c = 0; c < 3; ++c) oh[c] /= r;
 
@@ -134,20 +193,41 @@ void eproj_t(const orc_config* cfg, const float* x, int natoms, const float* box
         for (int m = 0; m < nmol; ++m) {
             if (m == n) continue;  // calcIR.cu:1099
             const float* mof = &x[3 * (m * nam)];
-            if (!within_cutoff_f32(mof, nhf, boxf, cfg->cutoff)) continue;
+            if (!within_cutoff_f32(mof, nhf, boxs, cfg->cutoff)) continue;
             T mo[3] = {T(mof[0]), T(mof[1]), T(mof[2])};
-            for (int i = 1; i < nam; ++i) {
-                const float* maf = &x[3 * (m * nam + i)];
-                T mx[3] = {T(maf[0]), T(maf[1]), T(maf[2])};
-                if (i == 3 && cfg->imodel != 0) {
+            // sites 1, 2 (H) and 3 (M); a three-site water gets its M built
+            // on the HOH bisector 0.015 nm from O (the TIP4P geometry)
+            for (int i = 1; i <= 3; ++i) {
+                T mx[3] = {T(0), T(0), T(0)};
+                if (i < nam) {
+                    const float* maf = &x[3 * (m * nam + i)];
+                    for (int c = 0; c < 3; ++c) mx[c] = T(maf[c]);
+                }
+                if (i == 3 && nam >= 4 && cfg->imodel != 0) {
                     // calcIR.cu:1124-1140: M moved to 0.015 nm along O->M
                     T om[3];
-                    for (int c = 0; c < 3; ++c) om[c] = minimage<T>(mx[c] - mo[c], box[c]);
+                    for (int c = 0; c < 3; ++c) om[c] = mx[c] - mo[c];
+                    mi3<T>(om, box);
                     T rom = std::sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
                     for (int c = 0; c < 3; ++c) mx[c] = mo[c] + T(kOMTip4p) * om[c] / rom;
+                } else if (i == 3 && nam == 3) {
+                    T h1[3], h2[3], bi[3];
+                    for (int c = 0; c < 3; ++c) {
+                        h1[c] = T(x[3 * (m * nam + 1) + c]) - mo[c];
+                        h2[c] = T(x[3 * (m * nam + 2) + c]) - mo[c];
+                    }
+                    mi3<T>(h1, box);
+                    mi3<T>(h2, box);
+                    const T r1 = std::sqrt(h1[0] * h1[0] + h1[1] * h1[1] + h1[2] * h1[2]);
+                    const T r2 = std::sqrt(h2[0] * h2[0] + h2[1] * h2[1] + h2[2] * h2[2]);
+                    for (int c = 0; c < 3; ++c) bi[c] = h1[c] / r1 + h2[c] / r2;
+                    const T rb = std::sqrt(bi[0] * bi[0] + bi[1] * bi[1] + bi[2] * bi[2]);
+                    for (int c = 0; c < 3; ++c) mx[c] = mo[c] + T(kOMTip4p) * bi[c] / rb;
                 }
                 T dr[3];
-                for (int c = 0; c < 3; ++c) dr[c] = minimage<T>(nh[c] - mx[c], box[c]) * bohr;
+                for (int c = 0; c < 3; ++c) dr[c] = nh[c] - mx[c];
+                mi3<T>(dr, box);
+                for (int c = 0; c < 3; ++c) dr[c] = dr[c] * bohr;
                 T rr = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
                 T r3 = rr * rr * rr;
                 if (i < 3) {
@@ -176,7 +256,7 @@ void sites_t(const orc_config* cfg, const MapCoef& mc, const float* x, int natom
              const double* eproj, std::vector<Site<T>>& s) {
     const int nam = cfg->natom_mol, ncm = cfg->nchrom_mol;
     const int nmol = natoms / nam, nchrom = nmol * ncm;
-    const T box[3] = {T(boxf[0]), T(boxf[1]), T(boxf[2])};
+    const Box<T> box = make_box<T>(cfg, boxf);
     s.resize(nchrom);
     for (int chrom = 0; chrom < nchrom; ++chrom) {
         const int n = chrom / ncm;
@@ -188,16 +268,18 @@ void sites_t(const orc_config* cfg, const MapCoef& mc, const float* x, int natom
         S.p = T(mc.p[0]) + T(mc.p[1]) * S.w;
         S.mup = T(mc.mup[0]) + T(mc.mup[1]) * E + T(mc.mup[2]) * E * E;
         const float* nof = &x[3 * (n * nam)];
-        const float* nhf = &x[3 * (n * nam + ((chrom % 2 == 0) ? 1 : 2))];
+        const float* nhf = &x[3 * (n * nam + h_atom(cfg, chrom))];
         T u[3];
-        for (int c = 0; c < 3; ++c) u[c] = minimage<T>(T(nhf[c]) - T(nof[c]), box[c]);
+        for (int c = 0; c < 3; ++c) u[c] = T(nhf[c]) - T(nof[c]);
+        mi3<T>(u, box);
         T r = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
         for (int c = 0; c < 3; ++c) {
             u[c] /= r;
             S.u[c] = u[c];
-            S.r[c] = minimage<T>(T(nof[c]) + T(kTdmOffset) * u[c], box[c]);  // calcIR.cu:1289-1291
-            S.mu[c] = u[c] * S.mup * S.x;                                      // calcIR.cu:1294-1296
+            S.r[c] = T(nof[c]) + T(kTdmOffset) * u[c];  // calcIR.cu:1289-1291
+            S.mu[c] = u[c] * S.mup * S.x;               // calcIR.cu:1294-1296
         }
+        mi3<T>(S.r, box);
         const T A = T(kRamanAniso);
         S.a[0] = (A * u[0] * u[0] + T(1)) * S.x;  // calcIR.cu:1299-1304
         S.a[1] = (A * u[1] * u[1] + T(1)) * S.x;
@@ -211,7 +293,7 @@ void sites_t(const orc_config* cfg, const MapCoef& mc, const float* x, int natom
 // Hamiltonian element (n, m), calcIR.cu:1322-1401.
 template <typename T>
 inline T h_element(const orc_config* cfg, const MapCoef& mc, const Site<T>& Sn, const Site<T>& Sm,
-                   int n, int m, const T* box) {
+                   int n, int m, const Box<T>& box) {
     const int ncm = cfg->nchrom_mol;
     if (n == m) return Sm.w - T(cfg->avef);
     const bool hod = (cfg->ispecies == 1 || cfg->ispecies == 2);  // calcIR.cu:1334, 1348
@@ -220,7 +302,8 @@ inline T h_element(const orc_config* cfg, const MapCoef& mc, const Site<T>& Sn, 
         return (T(mc.wi[0]) + T(mc.wi[1]) * (Sn.E + Sm.E)) * Sn.x * Sm.x + T(mc.wi[2]) * Sn.p * Sm.p;
     }
     T dr[3];
-    for (int c = 0; c < 3; ++c) dr[c] = minimage<T>(Sn.r[c] - Sm.r[c], box[c]);
+    for (int c = 0; c < 3; ++c) dr[c] = Sn.r[c] - Sm.r[c];
+    mi3<T>(dr, box);
     T r = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
     for (int c = 0; c < 3; ++c) dr[c] /= r;
     r *= T(kBohrPerNm);
@@ -234,7 +317,7 @@ template <typename T>
 void hamiltonian_t(const orc_config* cfg, const MapCoef& mc, const std::vector<Site<T>>& s,
                    const float* boxf, T* H /* row-major N*N */) {
     const int N = int(s.size());
-    const T box[3] = {T(boxf[0]), T(boxf[1]), T(boxf[2])};
+    const Box<T> box = make_box<T>(cfg, boxf);
 #pragma omp parallel for schedule(static) num_threads(threads_of(cfg))
     for (int n = 0; n < N; ++n)
         for (int m = 0; m < N; ++m) H[size_t(n) * N + m] = h_element<T>(cfg, mc, s[n], s[m], n, m, box);
@@ -565,7 +648,7 @@ int run_t(const orc_config* cfg, int natoms, int nframes, const float* xyz, cons
             if (f < 0 || f >= nframes) return 3;
             if (std::fabs(desired - times[f]) > frame_dt * 1E-1) return 3;  // calcIR.cu:391
             const float* x = &xyz[size_t(f) * natoms * 3];
-            const float* bx = &box[size_t(f) * 3];
+            const float* bx = &box[size_t(f) * box_floats(cfg)];
 
             double t0 = now_s<T>();
             eproj_t<T>(cfg, x, natoms, bx, eproj.data());

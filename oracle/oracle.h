@@ -88,6 +88,9 @@ typedef struct orc_config {
      * use_coef != 0: w0 w1 w2, x0 x1, p0 p1, mu0 mu1 mu2, wi0 wi1 wi2 */
     int    use_coef;
     double coef[13];
+    /* box arrays hold 9 floats per frame (rows a, b, c of a GROMACS box, a
+     * along x, b in the xy plane) instead of the 3-float diagonal */
+    int    triclinic;
 } orc_config;
 
 /* Outputs of orc_run.  All arrays are caller-allocated. */

@@ -30,7 +30,7 @@ class Config(ctypes.Structure):
         ("sampleEvery", ctypes.c_float), ("beginTime", ctypes.c_float),
         ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float), ("omegaStep", ctypes.c_int),
         ("propagation", ctypes.c_int), ("nthreads", ctypes.c_int),
-        ("use_coef", ctypes.c_int), ("coef", ctypes.c_double * 13),
+        ("use_coef", ctypes.c_int), ("coef", ctypes.c_double * 13), ("triclinic", ctypes.c_int),
     ]
 
 
@@ -86,10 +86,11 @@ def lib():
 def config(imodel=1, ispecies=0, imap=0, cutoff=0.7831, avef=3415.2, dt=0.010, t1=0.260,
            ntcfpoints=150, nsamples=1, nzeros=25600, sampleEvery=10.0, beginTime=0.0,
            omegaStart=2000.0, omegaStop=5000.0, omegaStep=5, propagation=PROP_VECTOR, nthreads=0,
-           natom_mol=4, nchrom_mol=2, coef=None):
+           natom_mol=4, nchrom_mol=2, coef=None, triclinic=0):
     c = Config(natom_mol, nchrom_mol, imodel, ispecies, imap, cutoff, avef, dt, t1, ntcfpoints,
                nsamples, nzeros, sampleEvery, beginTime, omegaStart, omegaStop, omegaStep,
                propagation, nthreads)
+    c.triclinic = int(triclinic)
     if coef is not None:
         set_coef(c, coef)
     return c
@@ -221,4 +222,4 @@ def config_from_params(p, coef=None):
                   t1=p.t1, ntcfpoints=p.ntcfpoints, nsamples=p.nsamples, nzeros=p.nzeros,
                   sampleEvery=p.sampleEvery, beginTime=p.beginTime, omegaStart=p.omegaStart,
                   omegaStop=p.omegaStop, omegaStep=p.omegaStep, natom_mol=p.natom_mol,
-                  nchrom_mol=p.nchrom_mol, coef=coef)
+                  nchrom_mol=p.nchrom_mol, coef=coef, triclinic=p.triclinic)
