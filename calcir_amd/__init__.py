@@ -110,6 +110,7 @@ SIGNATURES = {
     "cir_run": (_i, [_vp, _i, _vp, _vp, _vp]),
     "cir_run_device": (_i, [_vp, _i, _vp, _vp, _vp]),
     "cir_get_results": (_i, [_vp, ctypes.POINTER(Results)]),
+    "cir_run_many": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _i]),
     "cir_set_profiling": (_i, [_vp, _i]),
     "cir_phase_times": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll), _i]),
     "cir_destroy": (None, [_vp]),
@@ -333,6 +334,45 @@ def run(p, xyz, box, times, stream=None, profile=False, coefs=None):
     out.update(nsteps=r.nsteps, nframes_used=r.nframes_used, hist_skipped=r.hist_skipped,
                seconds=r.seconds, phases=phases)
     return out
+
+
+def run_many(jobs, nthreads=8):
+    """Independent jobs at once on one GPU (cir_run_many): jobs is a list of
+    (params, xyz, box, times) as for run(); returns their result dicts."""
+    L = lib()
+    n = len(jobs)
+    ctxs = (ctypes.c_void_p * n)()
+    keep = []
+    try:
+        for j, (p, xyz, box, times) in enumerate(jobs):
+            xyz = np.ascontiguousarray(xyz, np.float32)
+            box = np.ascontiguousarray(box, np.float32)
+            times = np.ascontiguousarray(times, np.float32)
+            keep.append((p, xyz, box, times))
+            c = ctypes.c_void_p()
+            check(L.cir_create(ctypes.byref(p), xyz.shape[1], None, ctypes.byref(c)))
+            ctxs[j] = c.value
+        nfr = (ctypes.c_int * n)(*[k[1].shape[0] for k in keep])
+        px = (ctypes.c_void_p * n)(*[k[1].ctypes.data for k in keep])
+        pb = (ctypes.c_void_p * n)(*[k[2].ctypes.data for k in keep])
+        pt = (ctypes.c_void_p * n)(*[k[3].ctypes.data for k in keep])
+        check(L.cir_run_many(n, ctxs, nfr, px, pb, pt, int(nthreads)))
+        outs = []
+        for j in range(n):
+            r, bufs = _results_buffers(keep[j][0])
+            check(L.cir_get_results(ctypes.c_void_p(ctxs[j]), ctypes.byref(r)))
+            out = {}
+            for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+                out[k] = bufs[k][0::2] + 1j * bufs[k][1::2]
+            for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw", "rw", "rmw"):
+                out[k] = bufs[k]
+            out.update(nsteps=r.nsteps, nframes_used=r.nframes_used, hist_skipped=r.hist_skipped, seconds=r.seconds)
+            outs.append(out)
+        return outs
+    finally:
+        for j in range(n):
+            if ctxs[j]:
+                L.cir_destroy(ctypes.c_void_p(ctxs[j]))
 
 
 def run_device(p, xyz, box, times, profile=False):

@@ -15,6 +15,8 @@
 //     forms of calcIR.cu:628-745.
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cstdint>
 #include <cmath>
 #include <cstdio>
@@ -682,6 +684,55 @@ extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* b
     CIR_HIP(hipMemcpyAsync(dx.p, xyz, nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     CIR_HIP(hipMemcpyAsync(db.p, box, nbx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     return cir_run_device(c, nframes, dx.as<float>(), db.as<float>(), times);
+}
+
+// Independent jobs (boxes of any size) at once on one GPU: each job runs on
+// its own stream, issued by a pool of host threads; the largest jobs start
+// first.  Kernels of different jobs overlap on the device, which fills the
+// GPU when single small boxes cannot.
+extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, const float* const* xyz,
+                            const float* const* box, const float* const* times, int nthreads) {
+    if (njob < 0 || (njob > 0 && (!ctxs || !nframes || !xyz || !box || !times)))
+        return fail(CIR_ERR_ARG, "cir_run_many: bad arguments");
+    for (int j = 0; j < njob; ++j)
+        for (int k = 0; k < j; ++k)
+            if (ctxs[j] == ctxs[k] || !ctxs[j]) return fail(CIR_ERR_ARG, "cir_run_many: contexts must be distinct");
+    if (njob == 0) return CIR_OK;
+    std::vector<int> order(njob);
+    for (int j = 0; j < njob; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return double(ctxs[a]->N) * ctxs[a]->N * nframes[a] > double(ctxs[b]->N) * ctxs[b]->N * nframes[b];
+    });
+    // jobs bound to the null stream get a stream of their own for the call
+    std::vector<hipStream_t> made(njob, nullptr), saved(njob, nullptr);
+    for (int j = 0; j < njob; ++j) {
+        saved[j] = ctxs[j]->stream;
+        if (!saved[j]) {
+            CIR_HIP(hipStreamCreateWithFlags(&made[j], hipStreamNonBlocking));
+            ctxs[j]->stream = made[j];
+        }
+    }
+    const int nt = std::max(1, std::min(nthreads > 0 ? nthreads : 8, njob));
+    std::vector<int> status(njob, CIR_OK);
+    std::vector<std::string> detail(njob);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (int q = next++; q < njob; q = next++) {
+            const int j = order[q];
+            status[j] = cir_run(ctxs[j], nframes[j], xyz[j], box[j], times[j]);
+            if (status[j] != CIR_OK) detail[j] = cir_last_error();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    for (int j = 0; j < njob; ++j) {
+        ctxs[j]->stream = saved[j];
+        if (made[j]) (void)hipStreamDestroy(made[j]);
+    }
+    for (int j = 0; j < njob; ++j)
+        if (status[j] != CIR_OK) return fail(status[j], "job " + std::to_string(j) + ": " + detail[j]);
+    return CIR_OK;
 }
 
 extern "C" int cir_get_results(cir_ctx* c, cir_results* r) {

@@ -211,6 +211,13 @@ int   cir_run(cir_ctx* ctx, int nframes, const float* xyz, const float* box, con
 /* Same with the trajectory already on the device. */
 int   cir_run_device(cir_ctx* ctx, int nframes, const float* xyz_d, const float* box_d,
                      const float* times);
+/* Independent jobs at once on one GPU (for example boxes of uneven size):
+ * job j is cir_run(ctxs[j], nframes[j], xyz[j], box[j], times[j]); each job
+ * runs on its own stream (a context bound to the null stream gets one for the
+ * call), issued by nthreads host threads (0 = 8), largest jobs first.  The
+ * contexts must be distinct.  Returns the first failing job's status. */
+int   cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, const float* const* xyz,
+                   const float* const* box, const float* const* times, int nthreads);
 int   cir_get_results(cir_ctx* ctx, cir_results* r);
 /* Enables (on != 0) per-phase device timing of subsequent runs. */
 int   cir_set_profiling(cir_ctx* ctx, int on);

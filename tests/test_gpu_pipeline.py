@@ -281,3 +281,23 @@ def test_cli_halts_and_resumes(tmp_path):
         a = np.loadtxt(tmp_path / ("two_%s.dat" % suffix))
         b = np.loadtxt(tmp_path / ("one_%s.dat" % suffix))
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max(), err_msg=suffix)
+
+
+def test_run_many_uneven_boxes_equal_single_runs():
+    # independent boxes of uneven size (ragged N = 100, 128, 256, 512, 666),
+    # mixed precisions, at once on one GPU: each job's result is its own run's
+    sizes = [(64, "fp64"), (50, "fp32"), (256, "fp32"), (128, "fp64"), (333, "bf16x3")]
+    jobs = []
+    for j, (nw, prec) in enumerate(sizes):
+        xyz, box, times = frames(nw, 24, 4000 + j)
+        p = C.default_params(precision=prec, mode="batched", **dict(KW, nsamples=2, ntcfpoints=8, sampleEvery=0.05))
+        jobs.append((p, xyz, box, times))
+    announce("run_many %d jobs" % len(jobs))
+    outs = C.run_many(jobs, nthreads=4)
+    for (p, xyz, box, times), g, (nw, prec) in zip(jobs, outs, sizes):
+        announce("run_many single check nwater=%d %s" % (nw, prec))
+        ref = C.run(p, xyz, box, times)
+        tol = 1e-12 if prec == "fp64" else 1e-5
+        for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+            assert rel(g[k], ref[k]) < tol, (nw, prec, k, rel(g[k], ref[k]))
+        assert g["nsteps"] == ref["nsteps"] == 16
