@@ -127,6 +127,69 @@ def cpu_baseline(xyz, box, times, nthreads):
                       "literal F-matrix propagation (3 complex products per step), %.1f s" % (k, k - 1, dt)}
 
 
+def adjoint_bench(C):
+    """Lineshape adjoint on the 1024-OH differentiation workload: 512 waters
+    (L = 2.483 nm), 400 frames, origins every 10 frames (26 samples x 150
+    points); forward + backward time per precision and the FP32 gradient's
+    deviation from FP64 (relative sensitivities coef * dL/dcoef)."""
+    from calcir_amd.adjoint import LineshapeSession, lineshape_map
+
+    xyz, box, times = C.synth_trajectory(512, 400, 1004, box_nm=2.483)
+    kw = dict(WORK, ntcfpoints=150, nsamples=26, sampleEvery=0.1)
+    coef = C.map_coefs(0, 0)
+    pert = coef * (1 + 0.02 * np.random.default_rng(1004).choice([-1.0, 1.0], coef.size))
+    out, grads, target = {}, {}, None
+    for prec in ("fp64", "fp32"):
+        print("[bench] adjoint %s" % prec, file=sys.stderr, flush=True)
+        p = C.default_params(precision=prec, **kw)
+        idx, _, scale = lineshape_map(p)
+        sess = LineshapeSession(p, xyz, box, times)
+        if target is None:
+            target = sess.forward(pert)[idx] * scale
+        sess.forward(coef)  # warm
+        t0 = time.perf_counter()
+        ft = sess.forward(coef)
+        t1 = time.perf_counter()
+        r = ft[idx] * scale - target
+        dL = np.zeros_like(ft)
+        np.add.at(dL, idx, 2.0 * r * scale)
+        grads[prec] = sess.backward(dL)
+        t2 = time.perf_counter()
+        sess.close()
+        out[prec] = {"forward_s": t1 - t0, "backward_s": t2 - t1}
+    s64, s32 = coef * grads["fp64"], coef * grads["fp32"]
+    out["fp32_vs_fp64_sensitivity_err"] = float(np.abs(s32 - s64).max() / np.abs(s64).max())
+    out["workload"] = "512 waters, 400 frames, 26 samples x 150 points, 13 map coefficients"
+    return out
+
+
+def uneven_boxes_bench(C, nbox=16, nframes=200):
+    """Independent boxes of uneven size (water counts drawn from
+    {64,128,256,512,768,1024}, seeded), one after another and at once
+    (cir_run_many); BF16x3 batched path, origins every 10 frames."""
+    rng = np.random.default_rng(3000)
+    counts = rng.choice([64, 128, 256, 512, 768, 1024], nbox)
+    kw = dict(WORK, precision="bf16x3", sampleEvery=0.1, nsamples=(nframes - 150) // 10 + 1)
+    jobs = []
+    for b, nw in enumerate(counts):
+        xyz, box, times = C.synth_trajectory(int(nw), nframes, 3000 + b, box_nm=C.synth_box_nm(int(nw)))
+        jobs.append((C.default_params(**kw), xyz, box, times))
+    print("[bench] uneven boxes", file=sys.stderr, flush=True)
+    C.run(*jobs[-1])  # warm
+    t0 = time.perf_counter()
+    for j in jobs:
+        C.run(*j)
+    tseq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = C.run_many(jobs, nthreads=8)
+    tmany = time.perf_counter() - t0
+    steps = sum(int(r["nsteps"]) for r in res)
+    return {"boxes": nbox, "frames_per_box": nframes, "waters": [int(c) for c in counts],
+            "sequential_frames_per_s": steps / tseq, "concurrent_frames_per_s": steps / tmany,
+            "concurrent_speedup": tseq / tmany, "sample": "%d boxes x %d frames (host arrays, "
+            "transfers included)" % (nbox, nframes)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +298,9 @@ def main():
                              "sample": "1 origin x %d tcf points, reference control flow" % args.plain_points}
         out["speedup_vs_plain_fp32"] = value / plain_fs
         out["speedup_vs_batched_fp32"] = value * t32 / steps_per_run
+        del r32, r64, rp
+        out["adjoint_1024oh"] = adjoint_bench(C)
+        out["uneven_boxes"] = uneven_boxes_bench(C)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(xyz, box, times, min(16, os.cpu_count() or 1))
