@@ -143,7 +143,10 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
 // into hi = bf16(v) and lo = bf16(v - hi), stored k-contiguous per row of
 // m (or n) so that an MFMA operand (8 consecutive k of one row) is a single
 // 16-byte LDS read.  The product is lo*hi + hi*lo + hi*hi in FP32.
-template <int BM, int BN, int BK, bool AMC, bool BNC>
+// EARLYC (short K with beta != 0): the C tile is read into registers before
+// the K loop, so its latency hides behind the few K steps instead of
+// trailing them.
+template <int BM, int BN, int BK, bool AMC, bool BNC, bool EARLYC>
 __global__ void __launch_bounds__(256, 2) gemm_bf16x3_kernel(GemmArgs g) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 32, FN = WN / 32;
@@ -172,6 +175,22 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x3_kernel(GemmArgs g) {
         for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    const float alpha = float(g.alpha), beta = float(g.beta);
+    float cpre[EARLYC ? FM : 1][EARLYC ? FN : 1][16];
+    if constexpr (EARLYC) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int n = n0 + wn + j * 32 + l32;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    cpre[i][j][r] = (m < M && n < N) ? C[int64_t(m) * g.c_m + int64_t(n) * g.c_n] : 0.f;
+                }
+            }
+    }
 
     TileLoad<BM, BK, AMC> la;
     TileLoad<BN, BK, BNC> lb;
@@ -228,7 +247,6 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x3_kernel(GemmArgs g) {
     }
 
     // epilogue: a fragment's 16 values of C are read together before any store
-    const float alpha = float(g.alpha), beta = float(g.beta);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -238,7 +256,10 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x3_kernel(GemmArgs g) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                cv[r] = (beta != 0.f && m < M && n < N) ? C[int64_t(m) * g.c_m + int64_t(n) * g.c_n] : 0.f;
+                if constexpr (EARLYC)
+                    cv[r] = cpre[i][j][r];
+                else
+                    cv[r] = (beta != 0.f && m < M && n < N) ? C[int64_t(m) * g.c_m + int64_t(n) * g.c_n] : 0.f;
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
@@ -310,24 +331,24 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
         }
 }
 
-template <int BM, int BN, int BK, bool SPLIT, bool AMC, bool BNC>
+template <int BM, int BN, int BK, bool SPLIT, bool AMC, bool BNC, bool EARLYC>
 static int launch_tile(const GemmArgs& g, hipStream_t s) {
     dim3 grid(unsigned(ceil_div(g.M, BM)), unsigned(ceil_div(g.N, BN)), unsigned(g.batch));
     if constexpr (SPLIT)
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, BK, AMC, BNC>), grid, dim3(256), 0, s, g);
+        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, BK, AMC, BNC, EARLYC>), grid, dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, BK, AMC, BNC>), grid, dim3(256), 0, s, g);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
 
-template <int BM, int BN, int BK, bool SPLIT>
+template <int BM, int BN, int BK, bool SPLIT, bool EARLYC = false>
 static int launch_layout(const GemmArgs& g, hipStream_t s) {
     const bool amc = g.a_m == 1, bnc = g.b_n == 1;
-    if (amc && bnc) return launch_tile<BM, BN, BK, SPLIT, true, true>(g, s);
-    if (amc) return launch_tile<BM, BN, BK, SPLIT, true, false>(g, s);
-    if (bnc) return launch_tile<BM, BN, BK, SPLIT, false, true>(g, s);
-    return launch_tile<BM, BN, BK, SPLIT, false, false>(g, s);
+    if (amc && bnc) return launch_tile<BM, BN, BK, SPLIT, true, true, EARLYC>(g, s);
+    if (amc) return launch_tile<BM, BN, BK, SPLIT, true, false, EARLYC>(g, s);
+    if (bnc) return launch_tile<BM, BN, BK, SPLIT, false, true, EARLYC>(g, s);
+    return launch_tile<BM, BN, BK, SPLIT, false, false, EARLYC>(g, s);
 }
 
 // Largest in-matrix element offset of a strided operand.
@@ -351,8 +372,13 @@ int gemm(int prec, const GemmArgs& g, hipStream_t s) {
         max_offset(g.K, g.b_k, g.N, g.b_n) >= (int64_t(1) << 31))
         return fail(CIR_ERR_ARG, "gemm: a matrix spans more than 2^31 elements");
     const bool big = g.M >= 128 && g.N >= 128;
-    if (prec == CIR_PREC_BF16X3)
+    if (prec == CIR_PREC_BF16X3) {
+        // short-K updates C += A B (the back-transform's Z -= Y W): 64x64
+        // tiles with the C tile read ahead of the K loop (the 128x128 tile
+        // has no registers left for it)
+        if (g.beta != 0.0 && g.K <= 128) return launch_layout<64, 64, 32, true, true>(g, s);
         return big ? launch_layout<128, 128, 32, true>(g, s) : launch_layout<64, 64, 32, true>(g, s);
+    }
     return big ? launch_layout<128, 128, 16, false>(g, s) : launch_layout<64, 64, 16, false>(g, s);
 }
 
