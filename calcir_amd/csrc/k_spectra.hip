@@ -374,7 +374,7 @@ constexpr int kPrMaxCols = 512;  // wider products (the F-matrix form) use the t
 template <bool TRANS, bool PHASE, bool SPLIT3>
 __global__ void __launch_bounds__(kPrNT)
 prop_mfma_kernel(int n, int ncols, int nmt_pad, int ld, const float* __restrict__ V, const float* __restrict__ B,
-                 int ldb, float* __restrict__ C, int ldc, double dt, const double* __restrict__ w) {
+                 int ldb, float* __restrict__ C, int ldc, double dt, const double* __restrict__ w, int kseg) {
     __shared__ float red[3][16][64];
     __shared__ float ph[2][32];
     const int mt = blockIdx.x % nmt_pad, nt = blockIdx.x / nmt_pad;
@@ -387,8 +387,12 @@ prop_mfma_kernel(int n, int ncols, int nmt_pad, int ld, const float* __restrict_
     const int bcol = (l32 < 16) ? cc : ncols + cc;
     const int m = m0 + l32;
     const bool mvalid = m < n;
-    const int kq = ((n + 63) / 64) * 16;
-    const int kbeg = wave * kq, kend = min(n, kbeg + kq);
+    // K segment of this workgroup (blockIdx.y; kseg is a multiple of 64),
+    // split in quarters over the 4 waves.  With more than one segment the
+    // partial results are added into C (zeroed beforehand).
+    const bool split_k = gridDim.y > 1;
+    const int kq = kseg / 4;
+    const int kbeg = blockIdx.y * kseg + wave * kq, kend = min(n, kbeg + kq);
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
@@ -452,30 +456,59 @@ prop_mfma_kernel(int n, int ncols, int nmt_pad, int ld, const float* __restrict_
                 v = (l32 < 16) ? co * v - sn * o : sn * o + co * v;
             }
             const int mm = m0 + i;
-            if (mm < n && cvalid) C[size_t(mm) * ldc + bcol] = v;
+            if (mm < n && cvalid) {
+                if (split_k)
+                    atomicAdd(&C[size_t(mm) * ldc + bcol], v);
+                else
+                    C[size_t(mm) * ldc + bcol] = v;
+            }
         }
     }
+}
+
+// C(0:n, 0:w2) = 0 for the split-K accumulation (row pitch ldc).
+__global__ void __launch_bounds__(256) zero_rows_kernel(int n, int w2, int ldc, float* __restrict__ C) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= int64_t(n) * w2) return;
+    C[size_t(e / w2) * ldc + size_t(e % w2)] = 0.f;
 }
 
 static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
                          const float* Yin, float* Yout, float* G, hipStream_t s) {
     const int nmt = int(ceil_div(n, 32));
     const int nmt_pad = int(ceil_div(nmt, 8) * 8);
-    const dim3 grid(unsigned(nmt_pad * ceil_div(ncols, 16)));
+    const int tiles = nmt_pad * int(ceil_div(ncols, 16));
+    // split K over workgroups until there are about 4 per CU (each segment
+    // at least 256 long, a multiple of 64); the partial products are then
+    // accumulated with atomics into zeroed outputs
+    int ks = std::max(1, std::min<int>(kPropMaxSplit, int(ceil_div(1024, tiles))));
+    ks = std::max(1, std::min(ks, n / 256));
+    const int kseg = int(ceil_div(ceil_div(n, ks), 64) * 64);
+    ks = int(ceil_div(n, kseg));
+    const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(ks));
     const int w2 = 2 * ncols;
-    if (split3) {
+    auto zero = [&](float* C, int ldc) -> int {
+        if (ks == 1) return CIR_OK;
+        hipLaunchKernelGGL(zero_rows_kernel, dim3(unsigned(ceil_div(int64_t(n) * w2, 256))), dim3(256), 0, s, n, w2,
+                           ldc, C);
+        CIR_CHECK_LAUNCH();
+        return CIR_OK;
+    };
+    CIR_TRY(zero(G, w2));
+    if (split3)
         hipLaunchKernelGGL((prop_mfma_kernel<true, true, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           Yin, ldy, G, w2, dt, w);
-        CIR_CHECK_LAUNCH();
-        hipLaunchKernelGGL((prop_mfma_kernel<false, false, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           G, w2, Yout, ldy, dt, w);
-    } else {
+                           Yin, ldy, G, w2, dt, w, kseg);
+    else
         hipLaunchKernelGGL((prop_mfma_kernel<true, true, false>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           Yin, ldy, G, w2, dt, w);
-        CIR_CHECK_LAUNCH();
+                           Yin, ldy, G, w2, dt, w, kseg);
+    CIR_CHECK_LAUNCH();
+    CIR_TRY(zero(Yout, ldy));
+    if (split3)
+        hipLaunchKernelGGL((prop_mfma_kernel<false, false, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
+                           G, w2, Yout, ldy, dt, w, kseg);
+    else
         hipLaunchKernelGGL((prop_mfma_kernel<false, false, false>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           G, w2, Yout, ldy, dt, w);
-    }
+                           G, w2, Yout, ldy, dt, w, kseg);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
