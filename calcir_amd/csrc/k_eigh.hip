@@ -1047,15 +1047,23 @@ __device__ __forceinline__ double hash_uniform(uint32_t a, uint32_t b) {
     return double(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-__host__ __device__ inline size_t invit_scratch_bytes(int n) { return 32 * size_t(n) * n; }
+// 3 FP64 arrays and a byte array of n x n, rounded up to 256 bytes so that
+// the next matrix's arrays stay aligned
+__host__ __device__ inline size_t invit_scratch_bytes(int n) { return (25 * size_t(n) * n + 255) & ~size_t(255); }
 
-// Scratch of one matrix (invit_scratch_bytes, FP64): the vectors X[k*n + c],
-// then U[(a*n + k)*n + c], a = 0..2 the three diagonals of U (LU of
-// T - lambda_c I with partial pivoting; a = 0 holds the reciprocal pivots).
-// Factors and vectors stay FP64 even for an FP32-derived T: eigenvectors of
-// eigenvalues closer than ~1e-7 ||T|| (but not clustered) are only told apart
-// by factors accurate beyond FP32, and near-degenerate pivots grow the vector
-// past the FP32 range before it is normalised.
+// Scratch of one matrix (invit_scratch_bytes): FP64 arrays X (the vectors),
+// R (reciprocal pivots) and L (multipliers), then the pivot flags F (bytes),
+// element (k, c) at [k*n + c], c the eigenvalue index (lanes run along c, so
+// every access is coalesced).  The LU of T - lambda_c I with partial pivoting
+// has U = (u0, u1, u2) per row; only 1/u0 is kept: u1 and u2 are rebuilt from
+// T, the flag and the previous row's multiplier (a pivoted row has u1 =
+// d(i+1) - lambda, u2 = e(i+1); an unpivoted row u2 = 0 and u1 the carried
+// e(i), times -l(i-1) after a pivoted row), which cuts the scratch traffic by
+// a third.  The second iteration reuses the factorisation.  Factors and
+// vectors are FP64 even for an FP32-derived T: eigenvectors of eigenvalues
+// closer than ~1e-7 ||T|| (but not clustered) are only told apart by factors
+// accurate beyond FP32, and near-degenerate pivots grow the vector past the
+// FP32 range before it is normalised.
 template <typename T>
 __global__ void __launch_bounds__(kEigNT)
 invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ eall,
@@ -1070,7 +1078,9 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
     T* Z = Zall + size_t(b) * n * ld;
     const size_t nn = size_t(n) * n;
     double* const SX = reinterpret_cast<double*>(scr + size_t(bl) * invit_scratch_bytes(n));
-    double* const S = SX + nn;
+    double* const SR = SX + nn;
+    double* const SL = SR + nn;
+    unsigned char* const SF = reinterpret_cast<unsigned char*>(SL + nn);
     for (int k = tid; k < n; k += kEigNT) {
         sd[k] = double(d[k]);
         se[k] = (k < n - 1) ? double(e[k]) : 0.0;
@@ -1087,67 +1097,102 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
     if ((c - cs) % kMaxCluster != 0) return;  // solved by its group's first lane
     int cend = c;
     while (cend + 1 < n && cend + 1 - c < kMaxCluster && joins_cluster(w, cend + 1)) ++cend;
+    const bool single = cend == c;
 
     double lam_prev = 0.0;
     for (int mb = c; mb <= cend; ++mb) {
         double lam = w[mb];
         if (mb > c) lam = fmax(lam, lam_prev + 10.0 * DBL_EPSILON * fmax(fabs(lam_prev), 1.0));
         lam_prev = lam;
-        double* U0 = S + mb;
-        double* U1 = S + nn + mb;
-        double* U2 = S + 2 * nn + mb;
         double* X = SX + mb;
-        for (int k = 0; k < n; ++k) X[size_t(k) * n] = hash_uniform(uint32_t(mb), uint32_t(k));
-        double scale = 1.0;
-        for (int it = 0; it < 2; ++it) {
-            // forward: factor T - lam I = P L U and apply P L^{-1} to scale*X
-            double a = sd[0] - lam;
-            double bs = (n > 1) ? se[0] : 0.0;
-            double yi = X[0] * scale;
-            for (int i = 0; i < n - 1; ++i) {
-                const double sub = se[i];
-                const double dn = sd[i + 1] - lam;
-                const double en = (i + 1 < n - 1) ? se[i + 1] : 0.0;
-                const double yn = X[size_t(i + 1) * n] * scale;
-                double u0, u1, u2, yo;
-                if (fabs(a) >= fabs(sub)) {
-                    if (a == 0.0) a = pivmin;
-                    const double l = sub / a;
-                    u0 = a;
-                    u1 = bs;
-                    u2 = 0.0;
-                    yo = yi;
-                    a = dn - l * bs;
-                    bs = en;
-                    yi = yn - l * yi;
-                } else {
-                    const double l = a / sub;
-                    u0 = sub;
-                    u1 = dn;
-                    u2 = en;
-                    yo = yn;
-                    a = bs - l * dn;
-                    bs = -l * en;
-                    yi = yi - l * yn;
-                }
-                if (fabs(u0) < pivmin) u0 = (u0 < 0.0) ? -pivmin : pivmin;
-                U0[size_t(i) * n] = 1.0 / u0;  // reciprocal pivot: the back substitution multiplies
-                U1[size_t(i) * n] = u1;
-                U2[size_t(i) * n] = u2;
-                X[size_t(i) * n] = yo;
+        double* R = SR + mb;
+        double* Lm = SL + mb;
+        unsigned char* F = SF + mb;
+        // iteration 1, forward: factor T - lam I = P L U, apply P L^{-1} to a
+        // seeded start vector
+        double a = sd[0] - lam;
+        double bs = (n > 1) ? se[0] : 0.0;
+        double yi = hash_uniform(uint32_t(mb), 0u);
+        for (int i = 0; i < n - 1; ++i) {
+            const double sub = se[i];
+            const double dn = sd[i + 1] - lam;
+            const double en = (i + 1 < n - 1) ? se[i + 1] : 0.0;
+            const double yn = hash_uniform(uint32_t(mb), uint32_t(i + 1));
+            double u0, l, yo;
+            unsigned char fl;
+            if (fabs(a) >= fabs(sub)) {
+                if (a == 0.0) a = pivmin;
+                l = sub / a;
+                u0 = a;
+                yo = yi;
+                a = dn - l * bs;
+                bs = en;
+                yi = yn - l * yi;
+                fl = 0;
+            } else {
+                l = a / sub;
+                u0 = sub;
+                yo = yn;
+                a = bs - l * dn;
+                bs = -l * en;
+                yi = yi - l * yn;
+                fl = 1;
             }
-            if (fabs(a) < pivmin) a = (a < 0.0) ? -pivmin : pivmin;
-            // backward: U x = y
-            double x1 = yi / a, x2 = 0.0;
+            if (fabs(u0) < pivmin) u0 = (u0 < 0.0) ? -pivmin : pivmin;
+            R[size_t(i) * n] = 1.0 / u0;  // reciprocal pivot: the back substitution multiplies
+            Lm[size_t(i) * n] = l;
+            F[size_t(i) * n] = fl;
+            X[size_t(i) * n] = yo;
+        }
+        if (fabs(a) < pivmin) a = (a < 0.0) ? -pivmin : pivmin;
+        const double alast = a;
+        double scale = 1.0, big = 0.0;
+        for (int it = 0; it < 2; ++it) {
+            if (it == 1) {
+                // forward with the kept factorisation: P L^{-1} (scale x)
+                yi = X[0] * scale;
+                for (int i = 0; i < n - 1; ++i) {
+                    const double yn = X[size_t(i + 1) * n] * scale;
+                    const double l = Lm[size_t(i) * n];
+                    double yo;
+                    if (F[size_t(i) * n] == 0) {
+                        yo = yi;
+                        yi = yn - l * yi;
+                    } else {
+                        yo = yn;
+                        yi = yi - l * yn;
+                    }
+                    X[size_t(i) * n] = yo;
+                }
+            }
+            // backward: U x = y, u1/u2 rebuilt from T and the flags
+            double x1 = yi / alast, x2 = 0.0;
             X[size_t(n - 1) * n] = x1;
             double nrm2 = x1 * x1;
+            big = x1;
+            unsigned char fi = (n > 1) ? F[size_t(n - 2) * n] : 0;
             for (int i = n - 2; i >= 0; --i) {
-                const double xi = (X[size_t(i) * n] - U1[size_t(i) * n] * x1 - U2[size_t(i) * n] * x2) *
-                                  U0[size_t(i) * n];
+                unsigned char fp = 0;
+                double lp = 0.0;
+                if (i > 0) {
+                    fp = F[size_t(i - 1) * n];
+                    lp = Lm[size_t(i - 1) * n];
+                }
+                double u1, u2;
+                if (fi) {
+                    u1 = sd[i + 1] - lam;
+                    u2 = (i + 1 < n - 1) ? se[i + 1] : 0.0;
+                } else {
+                    u1 = (i > 0 && fp) ? -lp * se[i] : se[i];
+                    u2 = 0.0;
+                }
+                const double xi = (X[size_t(i) * n] - u1 * x1 - u2 * x2) * R[size_t(i) * n];
                 X[size_t(i) * n] = xi;
                 nrm2 += xi * xi;
+                if (fabs(xi) >= fabs(big)) big = xi;  // first largest component in ascending order
                 x2 = x1;
                 x1 = xi;
+                fi = fp;
             }
             // orthogonalise against the cluster's earlier vectors
             if (mb > c) {
@@ -1165,8 +1210,14 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
             }
             scale = 1.0 / sqrt(nrm2);
         }
-        // normalise in place, largest component positive
-        double big = 0.0;
+        if (single) {
+            // normalise, largest component positive, straight into Z
+            const double f = (big < 0.0) ? -scale : scale;
+            for (int k = 0; k < n; ++k) Z[size_t(k) * ld + mb] = T(X[size_t(k) * n] * f);
+            continue;
+        }
+        // cluster members stay in scratch, normalised, for the later members
+        big = 0.0;
         for (int k = 0; k < n; ++k) {
             const double v = X[size_t(k) * n] * scale;
             X[size_t(k) * n] = v;
@@ -1175,6 +1226,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
         if (big < 0.0)
             for (int k = 0; k < n; ++k) X[size_t(k) * n] = -X[size_t(k) * n];
     }
+    if (single) return;
     for (int mb = c; mb <= cend; ++mb) {
         const double* X = SX + mb;
         for (int k = 0; k < n; ++k) Z[size_t(k) * ld + mb] = T(X[size_t(k) * n]);
