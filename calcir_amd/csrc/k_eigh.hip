@@ -44,9 +44,9 @@ constexpr int kBtNB = 128;   // reflectors per compact-WY block in the back-tran
 constexpr int kEigNT = 256;  // threads per block of bisection / inverse iteration
 constexpr double kClusterRel = 1e-7;
 constexpr int kMaxCluster = 128;
-constexpr int kSingleMinN = 512;     // matrices at least this large may run the whole-GPU tridiagonalisation:
-constexpr int kSingleMaxBatch = 8;   //   batches of at most this many,
-constexpr int kSingleLargeN = 6144;  //   or matrices at least this large
+constexpr int kSingleMinN = 512;     // matrices at least this large run the whole-GPU tridiagonalisation
+constexpr int kSingleMaxBatch = 16;  //   in batches of at most this many (one workgroup per
+                                     //   matrix needs more matrices to fill the GPU)
 
 // Eigenvalue k joins the cluster of k-1 (ascending w).
 __device__ __forceinline__ bool joins_cluster(const double* w, int k) {
@@ -1324,10 +1324,10 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     unsigned char* scr = work + L.off_scr;
 
     // 1. tridiagonalisation: a batch runs one workgroup per matrix; a few
-    // matrices, or very large ones (whose batches are small and whose
-    // one-workgroup product is slow), run one at a time across the whole GPU
+    // large matrices (too few workgroups to fill the GPU) run one at a time
+    // across the whole GPU
     if (ld % 4 != 0) return fail(CIR_ERR_ARG, "eigh: the leading dimension must be a multiple of 4");
-    if (n >= kSingleMinN && (batch <= kSingleMaxBatch || n >= kSingleLargeN)) {
+    if (n >= kSingleMinN && batch <= kSingleMaxBatch) {
         ProfScope ps(prof, CIR_PH_TRIDIAG, s);
         T* s1 = reinterpret_cast<T*>(work + L.off_s1);
         for (int b = 0; b < batch; ++b)

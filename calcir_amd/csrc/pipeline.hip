@@ -206,8 +206,15 @@ int check_boxes(const cir_ctx* c, int nframes, const float* box) {
 
 int auto_chunk(const cir_ctx* c, int nuniq) {
     if (c->p.chunk > 0) return std::min(c->p.chunk, nuniq);
-    const double per = 2.0 * double(c->N) * c->ld * c->esz;  // H + V
-    int B = int(std::max(1.0, (24.0 * (1 << 30)) / per));
+    // H and V of every frame of a chunk plus the eigensolver's per-matrix
+    // panels and back-transform blocks, within half of the free device memory
+    // (large boxes need large batches: one workgroup per matrix fills the GPU
+    // only with hundreds of matrices)
+    const double per = 2.0 * double(c->N) * c->ld * c->esz + 3.0 * 128.0 * c->N * c->esz;
+    size_t fr = 0, tot = 0;
+    double budget = 24.0 * (1 << 30);
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::max(budget, 0.5 * double(fr));
+    int B = int(std::max(1.0, budget / per));
     B = std::min(B, 512);
     return std::max(1, std::min(B, nuniq));
 }
