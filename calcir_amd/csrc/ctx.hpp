@@ -78,6 +78,7 @@ struct cir_ctx {
     hipStream_t stream = nullptr;
     cir::MapCoef mc{};              // map coefficients in use (the table row unless overridden)
     int boxk = 0;                   // 0: 3 box floats per frame (rectangular), 1: 9 (triclinic)
+    int mem_share = 1;              // jobs sharing the device memory (cir_run_many)
     std::vector<double> tcf[3], ftcf[3], sw, pw, rw, rmw;
     int64_t nsteps = 0, nframes_used = 0, hist_skipped = 0;
     double seconds = 0.0;

@@ -214,6 +214,7 @@ int auto_chunk(const cir_ctx* c, int nuniq) {
     size_t fr = 0, tot = 0;
     double budget = 24.0 * (1 << 30);
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::max(budget, 0.5 * double(fr));
+    budget /= std::max(1, c->mem_share);
     int B = int(std::max(1.0, budget / per));
     B = std::min(B, 512);
     return std::max(1, std::min(B, nuniq));
@@ -712,14 +713,26 @@ extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, 
     });
     // jobs bound to the null stream get a stream of their own for the call
     std::vector<hipStream_t> made(njob, nullptr), saved(njob, nullptr);
+    for (int j = 0; j < njob; ++j) saved[j] = ctxs[j]->stream;
     for (int j = 0; j < njob; ++j) {
-        saved[j] = ctxs[j]->stream;
-        if (!saved[j]) {
-            CIR_HIP(hipStreamCreateWithFlags(&made[j], hipStreamNonBlocking));
-            ctxs[j]->stream = made[j];
+        if (saved[j]) continue;
+        const hipError_t e = hipStreamCreateWithFlags(&made[j], hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (int k = 0; k < j; ++k) {
+                ctxs[k]->stream = saved[k];
+                if (made[k]) (void)hipStreamDestroy(made[k]);
+            }
+            return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
         }
+        ctxs[j]->stream = made[j];
     }
     const int nt = std::max(1, std::min(nthreads > 0 ? nthreads : 8, njob));
+    // concurrent jobs split the device memory their chunks may take
+    std::vector<int> saved_share(njob);
+    for (int j = 0; j < njob; ++j) {
+        saved_share[j] = ctxs[j]->mem_share;
+        ctxs[j]->mem_share = nt;
+    }
     std::vector<int> status(njob, CIR_OK);
     std::vector<std::string> detail(njob);
     std::atomic<int> next{0};
@@ -735,6 +748,7 @@ extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, 
     for (auto& t : pool) t.join();
     for (int j = 0; j < njob; ++j) {
         ctxs[j]->stream = saved[j];
+        ctxs[j]->mem_share = saved_share[j];
         if (made[j]) (void)hipStreamDestroy(made[j]);
     }
     for (int j = 0; j < njob; ++j)
