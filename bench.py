@@ -322,6 +322,15 @@ def main():
         out["batched_fp32"] = {"value": steps_per_run / t32, "unit": "frames/s",
                                "error_vs_fp64": spectrum_error(p32, r32, r64)}
         out["batched_bf16x3_error_vs_fp64"] = spectrum_error(p, res, r64)
+        print("[bench] bf16x3 with re-unitarisation every 25 points", file=sys.stderr, flush=True)
+        pr = C.default_params(precision="bf16x3", mode="batched", nsamples=ns, reunitarise_every=25, **WORK)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rr = C.run_device(pr, xyz_d, box_d, times)
+        torch.cuda.synchronize()
+        out["batched_bf16x3_reunit25"] = {"value": steps_per_run / (time.perf_counter() - t1), "unit": "frames/s",
+                                          "error_vs_fp64": spectrum_error(pr, rr, r64)}
+        del rr
         out["batched_fp64"] = {"value": steps_per_run / t64, "unit": "frames/s"}
         out["plain_fp32"] = {"value": plain_fs, "unit": "frames/s",
                              "sample": "1 origin x %d tcf points, reference control flow" % args.plain_points}

@@ -60,6 +60,7 @@ class Params(ctypes.Structure):
         ("cutoff", ctypes.c_float), ("precision", ctypes.c_int), ("mode", ctypes.c_int),
         ("chunk", ctypes.c_int), ("ckpt_every", ctypes.c_int), ("resume", ctypes.c_int),
         ("max_frames", ctypes.c_int), ("adj_store", ctypes.c_int), ("triclinic", ctypes.c_int),
+        ("reunit_every", ctypes.c_int),
     ]
 
 
@@ -111,6 +112,8 @@ SIGNATURES = {
     "cir_run_device": (_i, [_vp, _i, _vp, _vp, _vp]),
     "cir_get_results": (_i, [_vp, ctypes.POINTER(Results)]),
     "cir_run_many": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _i]),
+    "cir_reunit_workspace": (ctypes.c_size_t, [_i]),
+    "cir_reunitarise": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "cir_set_profiling": (_i, [_vp, _i]),
     "cir_phase_times": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll), _i]),
     "cir_destroy": (None, [_vp]),
@@ -176,6 +179,8 @@ def default_params(**kw):
             k = "ckpt_every"
         if k == "adjoint_store":
             k = "adj_store"
+        if k == "reunitarise_every":
+            k = "reunit_every"
         if k == "precision" and isinstance(v, str):
             v = {"fp32": PREC_FP32, "fp64": PREC_FP64, "bf16x3": PREC_BF16X3}[v]
         if k == "mode" and isinstance(v, str):

@@ -691,6 +691,8 @@ extern "C" int cir_forward_ls(cir_ctx* c, int nframes, const float* xyz, const f
     if (!c || !xyz || !box || !times || !ftcf_ir || nframes <= 0)
         return fail(CIR_ERR_ARG, "cir_forward_ls: bad arguments");
     if (c->p.adj_store < 0 || c->p.adj_store > 2) return fail(CIR_ERR_ARG, "adjoint_store must be 0, 1 or 2");
+    if (c->p.reunit_every > 0)
+        return fail(CIR_ERR_ARG, "the lineshape adjoint propagates without re-unitarisation: set reunitarise_every 0");
     c->adj.reset();
     const size_t fb = size_t(nframes) * c->natoms * 3 * sizeof(float);
     for (size_t q = 0; q < size_t(nframes) * c->natoms * 3; ++q)

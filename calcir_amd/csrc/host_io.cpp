@@ -58,6 +58,7 @@ extern "C" void cir_params_default(cir_params* p) {
     p->max_frames = 0;
     p->adj_store = 0;
     p->triclinic = 0;
+    p->reunit_every = 0;
 }
 
 extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
@@ -101,6 +102,7 @@ extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
         else if (!std::strcmp(para, "max_frames")) std::sscanf(value, "%d", &p->max_frames);
         else if (!std::strcmp(para, "adjoint_store")) std::sscanf(value, "%d", &p->adj_store);
         else if (!std::strcmp(para, "triclinic")) std::sscanf(value, "%d", &p->triclinic);
+        else if (!std::strcmp(para, "reunitarise_every")) std::sscanf(value, "%d", &p->reunit_every);
         else if (!std::strcmp(para, "precision")) {
             if (!std::strcmp(value, "fp64")) p->precision = CIR_PREC_FP64;
             else if (!std::strcmp(value, "bf16x3")) p->precision = CIR_PREC_BF16X3;

@@ -673,6 +673,151 @@ int correlate(int prec, int n, int nslot, int ldy, const int* action, const int*
     return CIR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Re-unitarisation of the propagated origins.  An origin's nine columns obey
+// Y_s = F M(0) with F unitary, so its Gram matrix Y_s^H Y_s stays M(0)^T M(0)
+// = G0_s; rounding in the products lets it drift.  With G = Y_s^H Y_s =
+// G0 + D (D small, Hermitian), Y_s <- Y_s (I - G0^{-1} D / 2) restores the
+// Gram matrix to G0 up to O(D^2).  mode[s]: 0 nothing, 1 record G0_s from Y_s
+// (an origin, Y_s = M(0) real), 2 correct Y_s.
+// ---------------------------------------------------------------------------
+constexpr int kGramRows = 512;  // rows per workgroup of the Gram sums
+
+// G[s] += sum over this workgroup's rows of conj(y_a) y_b, thread (a, b)
+template <typename T>
+__global__ void __launch_bounds__(128)
+gram_kernel(int n, int nslot, int ldy, const int* __restrict__ mode, const T* __restrict__ Y,
+            double* __restrict__ G) {
+    const int s = blockIdx.y, t = threadIdx.x;
+    if (mode[s] == 0 || t >= 81) return;
+    const int a = t / 9, b = t % 9, ncols = 9 * nslot;
+    const int ca = 9 * s + a, cb = 9 * s + b;
+    const int i0 = blockIdx.x * kGramRows, i1 = min(n, i0 + kGramRows);
+    double re = 0.0, im = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const T* y = Y + size_t(i) * ldy;
+        const double ar = double(y[ca]), ai = double(y[ncols + ca]);
+        const double br = double(y[cb]), bi = double(y[ncols + cb]);
+        re += ar * br + ai * bi;  // conj(y_a) y_b
+        im += ar * bi - ai * br;
+    }
+    atomicAdd(&G[(size_t(s) * 81 + t) * 2], re);
+    atomicAdd(&G[(size_t(s) * 81 + t) * 2 + 1], im);
+}
+
+// one thread per slot: record G0 (mode 1) or form C = I - G0^{-1} (G - G0)/2
+// (mode 2) through the Cholesky factor of the real G0
+__global__ void __launch_bounds__(64)
+reunit_coef_kernel(int nslot, const int* __restrict__ mode, const double* __restrict__ G, double* __restrict__ G0,
+                   double* __restrict__ C) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nslot) return;
+    const double* g = G + size_t(s) * 162;
+    double* g0 = G0 + size_t(s) * 81;
+    if (mode[s] == 1) {
+        for (int e = 0; e < 81; ++e) g0[e] = g[2 * e];
+        return;
+    }
+    if (mode[s] != 2) return;
+    double L[81];
+    for (int e = 0; e < 81; ++e) L[e] = g0[e];
+    for (int k = 0; k < 9; ++k) {  // lower Cholesky factor in place
+        double d = L[k * 9 + k];
+        for (int q = 0; q < k; ++q) d -= L[k * 9 + q] * L[k * 9 + q];
+        d = sqrt(fmax(d, 1e-300));
+        L[k * 9 + k] = d;
+        for (int r = k + 1; r < 9; ++r) {
+            double v = L[r * 9 + k];
+            for (int q = 0; q < k; ++q) v -= L[r * 9 + q] * L[k * 9 + q];
+            L[r * 9 + k] = v / d;
+        }
+    }
+    double* c = C + size_t(s) * 162;
+    for (int b = 0; b < 9; ++b)
+        for (int part = 0; part < 2; ++part) {
+            double x[9];
+            for (int a = 0; a < 9; ++a) x[a] = g[(a * 9 + b) * 2 + part] - (part == 0 ? g0[a * 9 + b] : 0.0);
+            for (int r = 0; r < 9; ++r) {  // L y = d
+                double v = x[r];
+                for (int q = 0; q < r; ++q) v -= L[r * 9 + q] * x[q];
+                x[r] = v / L[r * 9 + r];
+            }
+            for (int r = 8; r >= 0; --r) {  // L^T x = y
+                double v = x[r];
+                for (int q = r + 1; q < 9; ++q) v -= L[q * 9 + r] * x[q];
+                x[r] = v / L[r * 9 + r];
+            }
+            for (int a = 0; a < 9; ++a) c[(a * 9 + b) * 2 + part] = ((a == b && part == 0) ? 1.0 : 0.0) - 0.5 * x[a];
+        }
+}
+
+// Y_s(i, :) <- Y_s(i, :) C_s for the slots with mode 2
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+reunit_apply_kernel(int n, int nslot, int ldy, const int* __restrict__ mode, const double* __restrict__ C,
+                    T* __restrict__ Y) {
+    const int64_t idx = int64_t(blockIdx.x) * kSpNT + threadIdx.x;
+    if (idx >= int64_t(n) * nslot) return;
+    const int i = int(idx / nslot), s = int(idx % nslot);
+    if (mode[s] != 2) return;
+    const int ncols = 9 * nslot;
+    T* yr = Y + size_t(i) * ldy + 9 * s;
+    T* yi = yr + ncols;
+    double ar[9], ai[9];
+    for (int a = 0; a < 9; ++a) {
+        ar[a] = double(yr[a]);
+        ai[a] = double(yi[a]);
+    }
+    const double* c = C + size_t(s) * 162;
+    for (int b = 0; b < 9; ++b) {
+        double re = 0.0, im = 0.0;
+        for (int a = 0; a < 9; ++a) {
+            const double cr = c[(a * 9 + b) * 2], ci = c[(a * 9 + b) * 2 + 1];
+            re += ar[a] * cr - ai[a] * ci;
+            im += ar[a] * ci + ai[a] * cr;
+        }
+        yr[b] = T(re);
+        yi[b] = T(im);
+    }
+}
+
+__global__ void __launch_bounds__(256) zero_doubles_kernel(int64_t cnt, double* __restrict__ p) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e < cnt) p[e] = 0.0;
+}
+
+size_t reunit_work_doubles(int nslot) { return size_t(nslot) * (162 + 162); }
+
+int reunitarise(int prec, int n, int nslot, int ldy, const int* mode, double* G0, void* Y, double* work,
+                hipStream_t s) {
+    if (n <= 0 || nslot <= 0 || ldy < 18 * nslot || !mode || !G0 || !Y || !work)
+        return fail(CIR_ERR_ARG, "reunitarise: bad arguments");
+    double* G = work;
+    double* C = work + size_t(nslot) * 162;
+    const int64_t ng = int64_t(nslot) * 162;
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3(unsigned(ceil_div(ng, 256))), dim3(256), 0, s, ng, G);
+    CIR_CHECK_LAUNCH();
+    const dim3 gg(unsigned(ceil_div(n, kGramRows)), unsigned(nslot));
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(gram_kernel<double>, gg, dim3(128), 0, s, n, nslot, ldy, mode,
+                           static_cast<const double*>(Y), G);
+    else
+        hipLaunchKernelGGL(gram_kernel<float>, gg, dim3(128), 0, s, n, nslot, ldy, mode, static_cast<const float*>(Y),
+                           G);
+    CIR_CHECK_LAUNCH();
+    hipLaunchKernelGGL(reunit_coef_kernel, dim3(unsigned(ceil_div(nslot, 64))), dim3(64), 0, s, nslot, mode, G, G0, C);
+    CIR_CHECK_LAUNCH();
+    const int64_t tot = int64_t(n) * nslot;
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(reunit_apply_kernel<double>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           nslot, ldy, mode, C, static_cast<double*>(Y));
+    else
+        hipLaunchKernelGGL(reunit_apply_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
+                           nslot, ldy, mode, C, static_cast<float*>(Y));
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
 int spectrum_c2r(int nspec, int nfft, int nx, const double* X, double* y, hipStream_t s) {
     if (nspec <= 0 || nfft <= 0 || nx < 0 || nspec > 65535) return fail(CIR_ERR_ARG, "c2r: bad sizes");
     hipLaunchKernelGGL(c2r_kernel, dim3(unsigned(ceil_div(nfft, kSpNT)), unsigned(nspec)), dim3(kSpNT), 0, s, nfft,

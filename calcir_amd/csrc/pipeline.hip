@@ -232,9 +232,9 @@ struct CkptHeader {
     char magic[8];
     uint64_t key;
     int64_t next;
-    int64_t ybytes, outbytes, accbytes;
+    int64_t ybytes, outbytes, accbytes, g0bytes;
 };
-constexpr char kCkptMagic[8] = {'C', 'I', 'R', 'C', 'K', 'P', 'T', '1'};
+constexpr char kCkptMagic[8] = {'C', 'I', 'R', 'C', 'K', 'P', 'T', '2'};
 
 struct Fnv {
     uint64_t h = 1469598103934665603ull;
@@ -277,6 +277,7 @@ int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d
     f.val(p.omegaStop);
     f.val(p.cutoff);
     f.val(p.triclinic);
+    f.val(p.reunit_every);
     f.val(c->mc);
     f.val(S.nslot);
     f.val(int64_t(S.uniq.size()));
@@ -297,7 +298,8 @@ int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d
 std::string ckpt_path(const cir_params& p) { return std::string(p.cptf) + "_ckpt.bin"; }
 
 int ckpt_write(const std::string& path, uint64_t key, int64_t next, const std::vector<unsigned char>& y,
-               const std::vector<double>& out, const std::vector<double>& acc, long long skip) {
+               const std::vector<double>& out, const std::vector<double>& acc, long long skip,
+               const std::vector<double>& g0) {
     CkptHeader h{};
     std::memcpy(h.magic, kCkptMagic, 8);
     h.key = key;
@@ -305,6 +307,7 @@ int ckpt_write(const std::string& path, uint64_t key, int64_t next, const std::v
     h.ybytes = int64_t(y.size());
     h.outbytes = int64_t(out.size() * sizeof(double));
     h.accbytes = int64_t(acc.size() * sizeof(double));
+    h.g0bytes = int64_t(g0.size() * sizeof(double));
     const std::string tmp = path + ".tmp";
     FILE* f = std::fopen(tmp.c_str(), "wb");
     if (!f) return fail(CIR_ERR_IO, "could not open " + tmp + " for the checkpoint");
@@ -313,13 +316,14 @@ int ckpt_write(const std::string& path, uint64_t key, int64_t next, const std::v
     ok = ok && std::fwrite(out.data(), sizeof(double), out.size(), f) == out.size();
     ok = ok && std::fwrite(acc.data(), sizeof(double), acc.size(), f) == acc.size();
     ok = ok && std::fwrite(&skip, sizeof(skip), 1, f) == 1;
+    ok = ok && std::fwrite(g0.data(), sizeof(double), g0.size(), f) == g0.size();
     ok = (std::fclose(f) == 0) && ok;
     if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) return fail(CIR_ERR_IO, "checkpoint write to " + path + " failed");
     return CIR_OK;
 }
 
 int ckpt_read(const std::string& path, uint64_t key, int64_t* next, std::vector<unsigned char>& y,
-              std::vector<double>& out, std::vector<double>& acc, long long* skip) {
+              std::vector<double>& out, std::vector<double>& acc, long long* skip, std::vector<double>& g0) {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return fail(CIR_ERR_IO, "resume: no checkpoint " + path);
     CkptHeader h{};
@@ -329,11 +333,12 @@ int ckpt_read(const std::string& path, uint64_t key, int64_t* next, std::vector<
         return fail(CIR_ERR_ARG, "resume: " + path + " was written for another job (parameters or frames differ)");
     }
     ok = ok && h.ybytes == int64_t(y.size()) && h.outbytes == int64_t(out.size() * sizeof(double)) &&
-         h.accbytes == int64_t(acc.size() * sizeof(double));
+         h.accbytes == int64_t(acc.size() * sizeof(double)) && h.g0bytes == int64_t(g0.size() * sizeof(double));
     ok = ok && std::fread(y.data(), 1, y.size(), f) == y.size();
     ok = ok && std::fread(out.data(), sizeof(double), out.size(), f) == out.size();
     ok = ok && std::fread(acc.data(), sizeof(double), acc.size(), f) == acc.size();
     ok = ok && std::fread(skip, sizeof(*skip), 1, f) == 1;
+    ok = ok && std::fread(g0.data(), sizeof(double), g0.size(), f) == g0.size();
     std::fclose(f);
     if (!ok) return fail(CIR_ERR_IO, "resume: " + path + " is not a complete checkpoint");
     *next = h.next;
@@ -377,6 +382,29 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
     CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
     CIR_TRY(skip.alloc(sizeof(long long)));
+    // re-unitarisation: mode per (distinct frame, slot): 1 at an origin (record
+    // G0), 2 every reunit_every-th propagated point of a sample
+    const int R = p.reunit_every;
+    DevBuf rmode, rg0, rwork;
+    std::vector<int> hmode;
+    std::vector<char> rany;
+    if (R > 0) {
+        hmode.assign(size_t(nu) * nslot, 0);
+        rany.assign(nu, 0);
+        for (int u = 0; u < nu; ++u)
+            for (int q = 0; q < nslot; ++q) {
+                const size_t e = size_t(u) * nslot + q;
+                const int k = S.target[e] % nt;
+                if (S.action[e] == 1) hmode[e] = 1;
+                if (S.action[e] == 2 && k % R == 0) hmode[e] = 2;
+                rany[u] |= hmode[e] != 0;
+            }
+        CIR_TRY(rmode.alloc(hmode.size() * sizeof(int)));
+        CIR_TRY(rg0.alloc(size_t(nslot) * 81 * sizeof(double)));
+        CIR_TRY(rwork.alloc(reunit_work_doubles(nslot) * sizeof(double)));
+        CIR_HIP(hipMemcpyAsync(rmode.p, hmode.data(), rmode.bytes, hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemsetAsync(rg0.p, 0, rg0.bytes, st));
+    }
     CIR_HIP(hipMemcpyAsync(wmult.p, S.mult.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
     CIR_HIP(hipMemcpyAsync(worig.p, S.orig.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
     CIR_HIP(hipMemcpyAsync(act.p, S.action.data(), S.action.size() * sizeof(int), hipMemcpyHostToDevice, st));
@@ -397,10 +425,11 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
     int start = 0;
     if (p.resume) {
         std::vector<unsigned char> hy(Y0.bytes);
-        std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom));
+        std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom)), hg(R > 0 ? size_t(nslot) * 81 : 0);
         long long hs = 0;
         int64_t next = 0;
-        CIR_TRY(ckpt_read(ckpt_path(p), key, &next, hy, ho, ha, &hs));
+        CIR_TRY(ckpt_read(ckpt_path(p), key, &next, hy, ho, ha, &hs, hg));
+        if (R > 0) CIR_HIP(hipMemcpyAsync(rg0.p, hg.data(), rg0.bytes, hipMemcpyHostToDevice, st));
         if (next < 0 || next > nu) return fail(CIR_ERR_IO, "resume: checkpoint position out of range");
         CIR_HIP(hipMemcpyAsync(Y0.p, hy.data(), hy.size(), hipMemcpyHostToDevice, st));
         CIR_HIP(hipMemcpyAsync(out.p, ho.data(), out.bytes, hipMemcpyHostToDevice, st));
@@ -451,6 +480,9 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
             {
                 ProfScope ps(pf, CIR_PH_CORRELATE, sp);
                 CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, sp));
+                if (R > 0 && rany[u])
+                    CIR_TRY(reunitarise(rp, N, nslot, ldy, rmode.as<int>() + size_t(u) * nslot, rg0.as<double>(),
+                                        Ynext, rwork.as<double>(), sp));
                 CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot,
                                   tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), sp));
             }
@@ -462,14 +494,15 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
         const bool halt = more && p.max_frames > 0 && done_here >= p.max_frames;
         if (more && (halt || (p.ckpt_every > 0 && since_ckpt >= p.ckpt_every))) {
             std::vector<unsigned char> hy(Y0.bytes);
-            std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom));
+            std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom)), hg(R > 0 ? size_t(nslot) * 81 : 0);
             long long hs = 0;
+            if (R > 0) CIR_HIP(hipMemcpyAsync(hg.data(), rg0.p, rg0.bytes, hipMemcpyDeviceToHost, st));
             CIR_HIP(hipMemcpyAsync(hy.data(), Ycur, hy.size(), hipMemcpyDeviceToHost, st));
             CIR_HIP(hipMemcpyAsync(ho.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
             CIR_HIP(hipMemcpyAsync(ha.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
             CIR_HIP(hipMemcpyAsync(&hs, skip.p, sizeof(hs), hipMemcpyDeviceToHost, st));
             CIR_HIP(hipStreamSynchronize(st));
-            CIR_TRY(ckpt_write(ckpt_path(p), key, c0 + nb, hy, ho, ha, hs));
+            CIR_TRY(ckpt_write(ckpt_path(p), key, c0 + nb, hy, ho, ha, hs, hg));
             since_ckpt = 0;
         }
         if (halt) {
@@ -491,6 +524,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
     const cir_params& p = c->p;
     if (p.ckpt_every > 0 || p.resume != 0 || p.max_frames > 0)
         return fail(CIR_ERR_ARG, "checkpoint_every, resume and max_frames need mode batched");
+    if (p.reunit_every > 0) return fail(CIR_ERR_ARG, "reunitarise_every needs mode batched");
     Schedule S;
     CIR_TRY(build_schedule(p, nframes, times, S));
     const int N = c->N, ld = c->ld, ns = p.nsamples, nt = p.ntcfpoints, nom = c->nomega;
@@ -609,6 +643,7 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
         return fail(CIR_ERR_ARG, "natoms must be a positive multiple of natom_mol (>= 3)");
     if (p->nchrom_mol > 2) return fail(CIR_ERR_ARG, "nchrom_mol must be 1 or 2 (OH stretches of a water)");
     if (p->triclinic != 0 && p->triclinic != 1) return fail(CIR_ERR_ARG, "triclinic must be 0 or 1");
+    if (p->reunit_every < 0) return fail(CIR_ERR_ARG, "reunitarise_every must be >= 0");
     if (p->ntcfpoints <= 0 || p->nsamples <= 0 || p->omegaStep <= 0 || p->nzeros < 0 || !(p->t1 > 0) || !(p->dt > 0))
         return fail(CIR_ERR_ARG, "ntcfpoints, nsamples, omegaStep, t1 and dt must be positive, nzeros >= 0");
     if (cir_nomega(p) <= 0) return fail(CIR_ERR_ARG, "omegaStop must not be below omegaStart");
@@ -816,6 +851,14 @@ extern "C" int cir_correlate(int prec, int n, int nslot, int ldy, const int* act
                              const void* sites_d, const void* Y_d, double* out_d, void* stream) {
     return correlate(prec == CIR_PREC_FP64 ? CIR_PREC_FP64 : CIR_PREC_FP32, n, nslot, ldy, action_d, target_d, sites_d,
                      Y_d, out_d, as_stream(stream));
+}
+
+extern "C" size_t cir_reunit_workspace(int nslot) { return reunit_work_doubles(nslot) * sizeof(double); }
+
+extern "C" int cir_reunitarise(int prec, int n, int nslot, int ldy, const int* mode_d, double* G0_d, void* Y_d,
+                               void* work_d, void* stream) {
+    const int rp = (prec == CIR_PREC_FP64) ? CIR_PREC_FP64 : CIR_PREC_FP32;
+    return reunitarise(rp, n, nslot, ldy, mode_d, G0_d, Y_d, static_cast<double*>(work_d), as_stream(stream));
 }
 
 extern "C" int cir_spectrum_c2r(int nspec, int nfft, int nx, const double* X_d, double* y_d, void* stream) {

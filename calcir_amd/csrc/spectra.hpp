@@ -38,5 +38,10 @@ int slot_fixup(int prec, int n, int nslot, int ldy, const int* action, const voi
 int correlate(int prec, int n, int nslot, int ldy, const int* action, const int* target, const void* sites,
               const void* Y, double* out, hipStream_t s);
 int spectrum_c2r(int nspec, int nfft, int nx, const double* X, double* y, hipStream_t s);
+// Re-unitarisation of the origins' propagated columns (mode per slot: 0 none,
+// 1 record G0 = Y^H Y, 2 restore Y^H Y to G0); work: reunit_work_doubles().
+size_t reunit_work_doubles(int nslot);
+int reunitarise(int prec, int n, int nslot, int ldy, const int* mode, double* G0, void* Y, double* work,
+                hipStream_t s);
 
 }  // namespace cir

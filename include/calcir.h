@@ -122,6 +122,11 @@ typedef struct cir_params {
                                   plane, reduced); 0 = 3 floats, the rectangular diagonal as
                                   the reference (calcIR.cu:404-405). Set by the CLI when the
                                   trajectory's boxes have off-diagonal elements          */
+    int   reunit_every;        /* key "reunitarise_every" (batched mode): every this many
+                                  propagated points of a sample, restore the Gram matrix of
+                                  its nine propagated columns, Y^H Y, to its value at the
+                                  origin (F unitary keeps it there; rounding lets it drift);
+                                  0 = never                                             */
 } cir_params;
 
 /* Results of a run, all arrays caller-allocated (host).  NULL members are
@@ -316,6 +321,15 @@ int cir_correlate(int prec, int n, int nslot, int ldy, const int* action_d, cons
 /* cuFFT C2R contract (calcIR.cu:790-791) for nspec spectra: X_d nspec*nx
  * complex doubles (re, im), y_d nspec*nfft doubles. */
 int cir_spectrum_c2r(int nspec, int nfft, int nx, const double* X_d, double* y_d, void* stream);
+/* Re-unitarisation of propagated origins (no reference counterpart: the
+ * reference's F-matrix is never corrected).  Y_d as cir_propagate's vector
+ * form (n rows of [Re | Im], 9 complex columns per slot, ldy >= 18 nslot);
+ * mode_d[nslot]: 0 leave the slot, 1 record G0_d[s] = Y_s^H Y_s (real, 81
+ * doubles per slot; Y_s = M(0) at an origin), 2 set Y_s <- Y_s (I - G0^{-1}
+ * (Y_s^H Y_s - G0)/2), which restores Y_s^H Y_s to G0 to second order. */
+size_t cir_reunit_workspace(int nslot);
+int cir_reunitarise(int prec, int n, int nslot, int ldy, const int* mode_d, double* G0_d, void* Y_d,
+                    void* work_d, void* stream);
 /* Batched dense product C = alpha A B + beta C (the library GEMMs of
  * calcIR.cu:462-467, 601-613): element (m,k) of A at A[m*a_m + k*a_k] (one of
  * the two strides 1), B (k,n) at B[k*b_k + n*b_n], C (m,n) at C[m*c_m + n*c_n],

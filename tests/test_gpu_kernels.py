@@ -1,6 +1,8 @@
 """Per-kernel parity of the HIP path (through the C ABI) against the CPU oracle,
 numpy and a plain torch float64 reference."""
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -340,3 +342,60 @@ def test_eigh_outlier_does_not_merge_bulk(dtype):
     # ~ eps * 1e6 / gap(~1); FP32 stays at its own rounding level here
     orth = np.abs(V.T @ V - np.eye(n)).max()
     assert orth < (2e-3 if dtype == torch.float32 else 1e-9), orth
+
+
+@pytest.mark.parametrize("prec", [C.PREC_FP64, C.PREC_FP32])
+def test_reunitarise_restores_the_gram_matrix(prec):
+    # Y = F M(0) with F = V e^{i w} V^T unitary; a non-unitary perturbation
+    # Y (I + E), |E| ~ 1e-3, has its Gram matrix restored to M(0)^T M(0) to
+    # second order, and a unitary Y is left alone
+    announce("reunitarise prec=%d" % prec)
+    g = torch.Generator().manual_seed(11)
+    n, nslot = 96, 2
+    Hs = torch.randn(n, n, generator=g, dtype=torch.float64)
+    w, V = torch.linalg.eigh(Hs + Hs.T)
+    F = (V.to(torch.complex128) * torch.exp(0.3j * w)) @ V.T.to(torch.complex128)
+    M0 = torch.randn(nslot, n, 9, generator=g, dtype=torch.float64)
+    E = 1e-3 * (torch.randn(nslot, 9, 9, generator=g, dtype=torch.float64)
+                + 1j * torch.randn(nslot, 9, 9, generator=g, dtype=torch.float64))
+    Yu = [F @ M0[q].to(torch.complex128) for q in range(nslot)]
+    Yp = [Yu[q] @ (torch.eye(9, dtype=torch.complex128) + E[q]) for q in range(nslot)]
+    dtype = torch.float64 if prec == C.PREC_FP64 else torch.float32
+
+    def pack(blocks):
+        re = torch.cat([b.real for b in blocks], 1)
+        im = torch.cat([b.imag for b in blocks], 1)
+        return torch.cat([re, im], 1).to(dtype).contiguous().to(DEV)
+
+    def unpack(Y):
+        Y = Y.cpu().double()
+        return [torch.complex(Y[:, 9 * q:9 * q + 9], Y[:, 9 * nslot + 9 * q:9 * nslot + 9 * q + 9]) for q in range(nslot)]
+
+    L = C.lib()
+    G0 = torch.zeros(nslot * 81, dtype=torch.float64, device=DEV)
+    work = torch.zeros(L.cir_reunit_workspace(nslot) // 8, dtype=torch.float64, device=DEV)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    Y = pack([M0[q].to(torch.complex128) for q in range(nslot)])
+    mode = torch.tensor([1, 1], dtype=torch.int32, device=DEV)
+    C.check(L.cir_reunitarise(prec, n, nslot, 18 * nslot, ptr(mode), ptr(G0), ptr(Y), ptr(work), stream))
+    torch.cuda.synchronize()
+    g0 = G0.cpu().view(nslot, 9, 9)
+    for q in range(nslot):
+        assert torch.allclose(g0[q], M0[q].T @ M0[q], rtol=1e-5 if prec == C.PREC_FP32 else 1e-12)
+    mode = torch.tensor([2, 2], dtype=torch.int32, device=DEV)
+    for blocks, before_tol in ((Yp, 1e-4), (Yu, 0.0)):
+        Y = pack(blocks)
+        C.check(L.cir_reunitarise(prec, n, nslot, 18 * nslot, ptr(mode), ptr(G0), ptr(Y), ptr(work), stream))
+        torch.cuda.synchronize()
+        out = unpack(Y)
+        for q in range(nslot):
+            ref = M0[q].T @ M0[q]
+            before = (blocks[q].conj().T @ blocks[q] - ref).abs().max() / ref.abs().max()
+            after = (out[q].conj().T @ out[q] - ref).abs().max() / ref.abs().max()
+            assert before >= before_tol
+            # second order in the deviation (plus the rounding of Y)
+            floor = 1e-6 if prec == C.PREC_FP32 else 1e-13
+            assert after <= 4 * float(before) ** 2 + floor, (before, after)
+            if before_tol == 0.0:
+                assert (out[q] - blocks[q]).abs().max() <= (1e-5 if prec == C.PREC_FP32 else 1e-12) * blocks[q].abs().max()

@@ -301,3 +301,38 @@ def test_run_many_uneven_boxes_equal_single_runs():
         for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
             assert rel(g[k], ref[k]) < tol, (nw, prec, k, rel(g[k], ref[k]))
         assert g["nsteps"] == ref["nsteps"] == 16
+
+
+def test_reunitarisation_in_the_pipeline():
+    # FP64: F is unitary to rounding, so restoring the Gram matrices changes
+    # nothing beyond 1e-12; BF16x3 with it stays within 1e-3 of FP64 and close
+    # to BF16x3 without it; a halted and resumed run keeps the recorded G0
+    xyz, box, times = frames(128, 60, 1001)
+    announce("reunitarise fp64")
+    a = C.run(C.default_params(precision="fp64", **KW), xyz, box, times)
+    b = C.run(C.default_params(precision="fp64", reunitarise_every=5, **KW), xyz, box, times)
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        assert rel(b[k], a[k]) < 1e-12, (k, rel(b[k], a[k]))
+    announce("reunitarise bf16x3")
+    c = C.run(C.default_params(precision="bf16x3", reunitarise_every=3, **KW), xyz, box, times)
+    d = C.run(C.default_params(precision="bf16x3", **KW), xyz, box, times)
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
+        assert rel_l2(c[k], a[k]) < 1e-3 and rel_l2(c[k], d[k]) < 1e-4, k
+    with pytest.raises(C.CalcirError):
+        C.run(C.default_params(precision="fp64", mode="plain", reunitarise_every=5, **KW), xyz, box, times)
+
+
+def test_reunitarisation_survives_checkpoint_resume(tmp_path):
+    xyz, box, times = frames(128, 60, 1001)
+    ck = str(tmp_path / "ru")
+    kw = dict(KW, precision="fp64", chunk=8, reunitarise_every=4)
+    announce("reunitarise + checkpoint: uninterrupted")
+    ref = C.run(C.default_params(**kw), xyz, box, times)
+    announce("reunitarise + checkpoint: halt")
+    with pytest.raises(C.CalcirError) as e:
+        C.run(C.default_params(cptf=ck, max_frames=24, **kw), xyz, box, times)
+    assert e.value.status == C.INCOMPLETE
+    announce("reunitarise + checkpoint: resume")
+    g = C.run(C.default_params(cptf=ck, resume=1, **kw), xyz, box, times)
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        assert rel(g[k], ref[k]) < 1e-12, (k, rel(g[k], ref[k]))

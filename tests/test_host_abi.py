@@ -78,7 +78,8 @@ def test_parser_keys_comments_and_quirks(tmp_path):
         "resume          1\n"
         "max_frames      200\n"
         "adjoint_store   2\n"
-        "triclinic       1\n")
+        "triclinic       1\n"
+        "reunitarise_every 25\n")
     p = C.read_params(str(f))
     assert p.xtcf == b"nvt.xtc" and p.outf == b"test" and p.model == b"tip4p" and p.species == b"HOD/D2O"
     assert p.ntcfpoints == 400 and p.nsamples == 2 and p.imap == 1
@@ -86,7 +87,8 @@ def test_parser_keys_comments_and_quirks(tmp_path):
     assert p.beginTime == pytest.approx(1.5) and p.sampleEvery == pytest.approx(4.0)
     assert p.cutoff == pytest.approx(0.75) and p.precision == C.PREC_BF16X3 and p.mode == C.MODE_PLAIN
     assert (p.ckpt_every, p.resume, p.max_frames) == (40, 1, 200)
-    assert (p.adj_store, p.triclinic) == (2, 1)
+    assert (p.adj_store, p.triclinic, p.reunit_every) == (2, 1, 25)
+    assert C.default_params(reunitarise_every=7).reunit_every == 7
     assert (C.default_params().adj_store, C.default_params().triclinic) == (0, 0)
     assert (C.default_params().ckpt_every, C.default_params().resume, C.default_params().max_frames) == (0, 0, 0)
     assert C.default_params(checkpoint_every=5).ckpt_every == 5
