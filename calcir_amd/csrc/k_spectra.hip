@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "gemm.hpp"
 #include "spectra.hpp"
@@ -374,7 +377,8 @@ constexpr int kPrMaxCols = 512;  // wider products (the F-matrix form) use the t
 template <bool TRANS, bool PHASE, bool SPLIT3>
 __global__ void __launch_bounds__(kPrNT)
 prop_mfma_kernel(int n, int ncols, int nmt_pad, int ld, const float* __restrict__ V, const float* __restrict__ B,
-                 int ldb, float* __restrict__ C, int ldc, double dt, const double* __restrict__ w, int kseg) {
+                 int ldb, float* __restrict__ C, int ldc, double dt, const double* __restrict__ w, int kseg,
+                 int vecA) {
     __shared__ float red[3][16][64];
     __shared__ float ph[2][32];
     const int mt = blockIdx.x % nmt_pad, nt = blockIdx.x / nmt_pad;
@@ -402,12 +406,31 @@ prop_mfma_kernel(int n, int ncols, int nmt_pad, int ld, const float* __restrict_
         float av[kU][8], bv[kU][8];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
+            const int kb = k0 + 16 * u + 8 * h;
+            if (!TRANS && vecA && mvalid && kb + 8 <= kend) {
+                // V(m, kb..kb+7) is 32 contiguous, 16-byte aligned bytes of
+                // row m (V aligned, ld and kb multiples of 4): two vector loads
+                const float4* src = reinterpret_cast<const float4*>(V + size_t(m) * ld + kb);
+                const float4 x0 = src[0], x1 = src[1];
+                av[u][0] = x0.x;
+                av[u][1] = x0.y;
+                av[u][2] = x0.z;
+                av[u][3] = x0.w;
+                av[u][4] = x1.x;
+                av[u][5] = x1.y;
+                av[u][6] = x1.z;
+                av[u][7] = x1.w;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int k = kb + q;
+                    av[u][q] = (mvalid && k < kend) ? (TRANS ? V[size_t(k) * ld + m] : V[size_t(m) * ld + k]) : 0.f;
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int k = k0 + 16 * u + 8 * h + q;
-                const bool kv = k < kend;
-                av[u][q] = (mvalid && kv) ? (TRANS ? V[size_t(k) * ld + m] : V[size_t(m) * ld + k]) : 0.f;
-                bv[u][q] = (cvalid && kv) ? B[size_t(k) * ldb + bcol] : 0.f;
+                const int k = kb + q;
+                bv[u][q] = (cvalid && k < kend) ? B[size_t(k) * ldb + bcol] : 0.f;
             }
         }
 #pragma unroll
@@ -473,20 +496,77 @@ __global__ void __launch_bounds__(256) zero_rows_kernel(int n, int w2, int ldc, 
     C[size_t(e / w2) * ldc + size_t(e % w2)] = 0.f;
 }
 
+// K split of the propagation products, picked per (n, ncols, product form)
+// by timing the candidates once per process (the first call of a shape whose
+// output does not alias its input); the default guess aims at about 4
+// workgroups per CU with K segments of at least 256.
+struct PropTune {
+    std::mutex mu;
+    std::map<std::tuple<int, int, bool>, int> best;
+};
+static PropTune& prop_tune() {
+    static PropTune t;
+    return t;
+}
+
+static int prop_pair(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
+                     const float* Yin, float* Yout, float* G, int ks, hipStream_t s);
+
 static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
                          const float* Yin, float* Yout, float* G, hipStream_t s) {
-    const int nmt = int(ceil_div(n, 32));
-    const int nmt_pad = int(ceil_div(nmt, 8) * 8);
+    const int nmt_pad = int(ceil_div(ceil_div(n, 32), 8) * 8);
     const int tiles = nmt_pad * int(ceil_div(ncols, 16));
-    // split K over workgroups until there are about 4 per CU (each segment
-    // at least 256 long, a multiple of 64); the partial products are then
-    // accumulated with atomics into zeroed outputs
-    int ks = std::max(1, std::min<int>(kPropMaxSplit, int(ceil_div(1024, tiles))));
-    ks = std::max(1, std::min(ks, n / 256));
+    const int guess = std::max(1, std::min({kPropMaxSplit, int(ceil_div(1024, tiles)), std::max(1, n / 256)}));
+    const auto key = std::make_tuple(n, ncols, split3);
+    int ks = guess;
+    bool tune = false;
+    {
+        std::lock_guard<std::mutex> lk(prop_tune().mu);
+        auto it = prop_tune().best.find(key);
+        if (it != prop_tune().best.end())
+            ks = it->second;
+        else
+            tune = Yin != Yout;
+    }
+    if (tune) {
+        hipEvent_t e0, e1;
+        CIR_HIP(hipEventCreate(&e0));
+        CIR_HIP(hipEventCreate(&e1));
+        float best_ms = 0.f;
+        int best_ks = guess;
+        for (int cand = 1; cand <= std::min(kPropMaxSplit, std::max(1, n / 256)); cand *= 2) {
+            CIR_TRY(prop_pair(split3, n, ld, ncols, ldy, dt, V, w, Yin, Yout, G, cand, s));  // warm
+            CIR_HIP(hipEventRecord(e0, s));
+            CIR_TRY(prop_pair(split3, n, ld, ncols, ldy, dt, V, w, Yin, Yout, G, cand, s));
+            CIR_HIP(hipEventRecord(e1, s));
+            CIR_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (cand == 1 || ms < best_ms) {
+                best_ms = ms;
+                best_ks = cand;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ks = best_ks;
+        std::lock_guard<std::mutex> lk(prop_tune().mu);
+        prop_tune().best.emplace(key, ks);
+    }
+    return prop_pair(split3, n, ld, ncols, ldy, dt, V, w, Yin, Yout, G, ks, s);
+}
+
+// The two products with ks K segments (ks > 1: atomic accumulation into
+// zeroed outputs).
+static int prop_pair(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
+                     const float* Yin, float* Yout, float* G, int ks, hipStream_t s) {
+    const int nmt_pad = int(ceil_div(ceil_div(n, 32), 8) * 8);
+    const int tiles = nmt_pad * int(ceil_div(ncols, 16));
     const int kseg = int(ceil_div(ceil_div(n, ks), 64) * 64);
     ks = int(ceil_div(n, kseg));
     const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(ks));
     const int w2 = 2 * ncols;
+    const int vecA = (ld % 4 == 0 && (reinterpret_cast<uintptr_t>(V) & 15) == 0) ? 1 : 0;
     auto zero = [&](float* C, int ldc) -> int {
         if (ks == 1) return CIR_OK;
         hipLaunchKernelGGL(zero_rows_kernel, dim3(unsigned(ceil_div(int64_t(n) * w2, 256))), dim3(256), 0, s, n, w2,
@@ -497,18 +577,18 @@ static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double 
     CIR_TRY(zero(G, w2));
     if (split3)
         hipLaunchKernelGGL((prop_mfma_kernel<true, true, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           Yin, ldy, G, w2, dt, w, kseg);
+                           Yin, ldy, G, w2, dt, w, kseg, vecA);
     else
         hipLaunchKernelGGL((prop_mfma_kernel<true, true, false>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           Yin, ldy, G, w2, dt, w, kseg);
+                           Yin, ldy, G, w2, dt, w, kseg, vecA);
     CIR_CHECK_LAUNCH();
     CIR_TRY(zero(Yout, ldy));
     if (split3)
         hipLaunchKernelGGL((prop_mfma_kernel<false, false, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           G, w2, Yout, ldy, dt, w, kseg);
+                           G, w2, Yout, ldy, dt, w, kseg, vecA);
     else
         hipLaunchKernelGGL((prop_mfma_kernel<false, false, false>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
-                           G, w2, Yout, ldy, dt, w, kseg);
+                           G, w2, Yout, ldy, dt, w, kseg, vecA);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
