@@ -89,6 +89,48 @@ constexpr int strip_cols() {
     return sizeof(T) == 4 ? 16 : 8;
 }
 
+// Sum over the 16 lanes of a row (lanes 16 R .. 16 R + 15) in every lane of
+// the row, by row rotations (DPP row_ror 8, 4, 2, 1).  After the rotation by
+// 8 the partial sums repeat with period 8, and so on, so every lane holds the
+// same bits.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float row_sum16(float x) {
+    x += dpp_f32<0x128>(x);
+    x += dpp_f32<0x124>(x);
+    x += dpp_f32<0x122>(x);
+    x += dpp_f32<0x121>(x);
+    return x;
+}
+
+// Adds the wave sums of the 16 per-lane column partials of a symmetric-product
+// strip to sp[c + k], k < nc.  Two lane swaps halve the values each lane
+// carries (v_permlane32_swap moves lanes 32-63 of the first operand and lanes
+// 0-31 of the second into each other, v_permlane16_swap the odd 16-lane rows of
+// the first and the even rows of the second), so that 16-lane row R ends with
+// the 4-lane partials of columns 4 R .. 4 R + 3; row rotations finish them and
+// lane 16 R + i adds column 4 R + i.
+__device__ __forceinline__ void strip_sums16(const float (&dotc)[16], int lane, int c, int nc, float* sp) {
+    float s1[8], s2[4];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const auto sw =
+            __builtin_amdgcn_permlane32_swap(__float_as_uint(dotc[k]), __float_as_uint(dotc[k + 8]), false, false);
+        s1[k] = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);  // column k + 8 (lane >> 5)
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const auto sw =
+            __builtin_amdgcn_permlane16_swap(__float_as_uint(s1[k]), __float_as_uint(s1[k + 4]), false, false);
+        s2[k] = row_sum16(__uint_as_float(sw[0]) + __uint_as_float(sw[1]));  // column k + 4 (lane >> 4)
+    }
+    const int i = lane & 15, col = i + 4 * (lane >> 4);
+    const float v = i == 0 ? s2[0] : i == 1 ? s2[1] : i == 2 ? s2[2] : s2[3];
+    if (i < 4 && col < nc) atomicAdd(&sp[c + col], v);
+}
+
 template <typename T>
 __device__ T block_sum1(T v, T* sred) {
     v = wave_sum(v);
@@ -249,9 +291,13 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
         T tau_prev = T(0), K_prev = T(0);
         // group sum over the LPR lanes of one row
         auto row_sum = [&](T x) {
+            if constexpr (sizeof(T) == 4 && LPR == 16) {
+                return row_sum16(x);
+            } else {
 #pragma unroll
-            for (int off = 1; off < LPR; off <<= 1) x += __shfl_xor(x, off, 64);
-            return x;
+                for (int off = 1; off < LPR; off <<= 1) x += __shfl_xor(x, off, 64);
+                return x;
+            }
         };
         // W(r, i-1) from the lane's panel-row vector xr (entries updated in
         // place); returns nothing, writes the finished entry to P
@@ -378,88 +424,115 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 const __amdgpu_buffer_rsrc_t arsrc = matrix_rsrc(A, uint32_t(size_t(n) * ld * sizeof(T)));
                 for (int r = j + 1 + tid; r < n; r += kTriNT) sp[r] = T(0);
                 __syncthreads();
-                for (int c = j + 1 + wave * kStrip; c < n; c += kTriNW * kStrip) {
+                // Strips are dealt to the waves in snake order (wave w takes
+                // strips w, 2 kTriNW - 1 - w, 2 kTriNW + w, ...), which evens
+                // out the rows each wave reads.  A wave runs through its
+                // strips' tiles as one stream: the loads of the next tile, of
+                // this strip or of the next one, are in flight while a tile is
+                // consumed.
+                constexpr int kTile = 64 * VW;
+                const int nstrip = (n - j - 1 + kStrip - 1) / kStrip;
+                auto strip_col = [&](int m) {
+                    const int s = kTriNW * m + ((m & 1) ? kTriNW - 1 - wave : wave);
+                    return s < nstrip ? j + 1 + s * kStrip : n;
+                };
+                T dotc[kStrip];
+#pragma unroll
+                for (int k = 0; k < kStrip; ++k) dotc[k] = T(0);
+                // column offsets are wave-uniform (scalar); columns past n
+                // are clamped to n - 1 and enter with zero weight
+                auto load = [&](VT(&a)[kStrip], int c, int rt) {
+                    const int r = rt + lane * VW;
+                    const int loff = (r < n) ? r * int(sizeof(T)) : kOffNone;
+#pragma unroll
+                    for (int k = 0; k < kStrip; ++k)
+                        a[k] = buffer_load16<VT>(arsrc, loff, min(c + k, n - 1) * ld * int(sizeof(T)));
+                };
+                auto tile = [&](const VT(&a)[kStrip], int c, int rt) {
                     const int nc = min(kStrip, n - c);
-                    T dotc[kStrip];
+                    const bool diag = rt == (c & ~(VW - 1));  // the strip's first tile holds the diagonal
+                    const int r = rt + lane * VW;
+                    T vr[VW], rowp[VW], vcs[kStrip];
 #pragma unroll
-                    for (int k = 0; k < kStrip; ++k) dotc[k] = T(0);
-                    // column offsets are wave-uniform (scalar); columns past n
-                    // are clamped to n - 1 and enter with zero weight
-                    auto load = [&](VT(&a)[kStrip], int rt) {
-                        const int r = rt + lane * VW;
-                        const int loff = (r < n) ? r * int(sizeof(T)) : kOffNone;
+                    for (int q = 0; q < VW; ++q) {
+                        vr[q] = (r + q < n && r + q > j) ? sv[r + q] : T(0);  // rows <= j hold stale v
+                        rowp[q] = T(0);
+                    }
+                    // v over the strip's columns (zero past n), re-read from
+                    // LDS for each tile to save registers
 #pragma unroll
-                        for (int k = 0; k < kStrip; ++k)
-                            a[k] = buffer_load16<VT>(arsrc, loff, min(c + k, n - 1) * ld * int(sizeof(T)));
-                    };
-                    auto tile = [&](const VT(&a)[kStrip], int rt, bool diag) {
-                        const int r = rt + lane * VW;
-                        T vr[VW], rowp[VW], vcs[kStrip];
+                    for (int k = 0; k < kStrip; ++k) vcs[k] = (k < nc) ? sv[c + k] : T(0);
+                    if (diag) {
+                        // tile on the diagonal: row >= column for the row
+                        // part, row > column for the transposed part
 #pragma unroll
-                        for (int q = 0; q < VW; ++q) {
-                            vr[q] = (r + q < n && r + q > j) ? sv[r + q] : T(0);  // rows <= j hold stale v
-                            rowp[q] = T(0);
-                        }
-                        // v over the strip's columns (zero past n), re-read
-                        // from LDS for each tile to save registers
+                        for (int k = 0; k < kStrip; ++k) {
+                            const T* ak = reinterpret_cast<const T*>(&a[k]);
 #pragma unroll
-                        for (int k = 0; k < kStrip; ++k) vcs[k] = (k < nc) ? sv[c + k] : T(0);
-                        if (diag) {
-                            // tile on the diagonal: row >= column for the row
-                            // part, row > column for the transposed part
-#pragma unroll
-                            for (int k = 0; k < kStrip; ++k) {
-                                const T* ak = reinterpret_cast<const T*>(&a[k]);
-#pragma unroll
-                                for (int q = 0; q < VW; ++q) {
-                                    const int rr = r + q;
-                                    const T val = (rr >= c + k) ? ak[q] : T(0);
-                                    rowp[q] += val * vcs[k];
-                                    dotc[k] += (rr > c + k) ? val * vr[q] : T(0);
-                                }
-                            }
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < kStrip; ++k) {
-                                const T* ak = reinterpret_cast<const T*>(&a[k]);
-#pragma unroll
-                                for (int q = 0; q < VW; ++q) {
-                                    rowp[q] += ak[q] * vcs[k];
-                                    dotc[k] += ak[q] * vr[q];
-                                }
+                            for (int q = 0; q < VW; ++q) {
+                                const int rr = r + q;
+                                const T val = (rr >= c + k) ? ak[q] : T(0);
+                                rowp[q] += val * vcs[k];
+                                dotc[k] += (rr > c + k) ? val * vr[q] : T(0);
                             }
                         }
+                    } else {
 #pragma unroll
-                        for (int q = 0; q < VW; ++q) {
-                            vav += vr[q] * rowp[q];
-                            if (r + q < n && r + q >= c) atomicAdd(&sp[r + q], rowp[q]);
+                        for (int k = 0; k < kStrip; ++k) {
+                            const T* ak = reinterpret_cast<const T*>(&a[k]);
+#pragma unroll
+                            for (int q = 0; q < VW; ++q) {
+                                rowp[q] += ak[q] * vcs[k];
+                                dotc[k] += ak[q] * vr[q];
+                            }
                         }
-                    };
-                    // the first tile holds the diagonal; the loads of the next
-                    // tile are in flight while a tile is consumed
-                    constexpr int kTile = 64 * VW;
-                    VT a0[kStrip], a1[kStrip];
-                    int rt = c & ~(VW - 1);
-                    load(a0, rt);
-                    bool more = rt + kTile < n;
-                    if (more) load(a1, rt + kTile);
-                    tile(a0, rt, true);
-                    while (more) {
-                        rt += kTile;
-                        more = rt + kTile < n;
-                        if (more) load(a0, rt + kTile);
-                        tile(a1, rt, false);
-                        if (!more) break;
-                        rt += kTile;
-                        more = rt + kTile < n;
-                        if (more) load(a1, rt + kTile);
-                        tile(a0, rt, false);
                     }
 #pragma unroll
-                    for (int k = 0; k < kStrip; ++k) {
+                    for (int q = 0; q < VW; ++q) {
+                        vav += vr[q] * rowp[q];
+                        if (r + q < n && r + q >= c) atomicAdd(&sp[r + q], rowp[q]);
+                    }
+                };
+                // the column sums of a finished strip go to sp[c ..]
+                auto strip_end = [&](int c) {
+                    const int nc = min(kStrip, n - c);
+#pragma unroll
+                    for (int k = 0; k < kStrip; ++k)
                         if (k < nc) vav += sv[c + k] * dotc[k];
-                        const T dk = wave_sum(dotc[k]);
-                        if (lane == k && k < nc) atomicAdd(&sp[c + k], dk);
+                    if constexpr (sizeof(T) == 4 && kStrip == 16) {
+                        strip_sums16(dotc, lane, c, nc, sp);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < kStrip; ++k) {
+                            const T dk = wave_sum(dotc[k]);
+                            if (lane == k && k < nc) atomicAdd(&sp[c + k], dk);
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kStrip; ++k) dotc[k] = T(0);
+                };
+                // cursor: strip m at column c, tile rows from rt
+                int m = 0, c = strip_col(0);
+                int rt = c & ~(VW - 1);
+                // consume `cur` at the cursor while `nxt` loads the next tile
+                auto step = [&](const VT(&cur)[kStrip], VT(&nxt)[kStrip]) {
+                    int m2 = m, c2 = c, r2 = rt + kTile;
+                    if (r2 >= n) {
+                        c2 = strip_col(++m2);
+                        r2 = c2 & ~(VW - 1);
+                    }
+                    if (c2 < n) load(nxt, c2, r2);
+                    tile(cur, c, rt);
+                    if (c2 != c) strip_end(c);
+                    m = m2;
+                    c = c2;
+                    rt = r2;
+                    return c < n;
+                };
+                VT a0[kStrip], a1[kStrip];
+                if (c < n) {
+                    load(a0, c, rt);
+                    while (step(a0, a1) && step(a1, a0)) {
                     }
                 }
             }
