@@ -28,8 +28,11 @@
 //      bisection tolerance are relative to the eigenvalue itself, so a few
 //      outlying diagonal entries (strong-field chromophores) neither merge the
 //      bulk of the spectrum into one cluster nor coarsen its eigenvalues.
-//   4. back-transform V = Q Z in compact-WY blocks of kBtNB reflectors on
-//      the MFMA product (gemm.hpp): Z <- Z - Y (T (Y^T Z)), last block first.
+//   4. back-transform V = Q Z in compact-WY passes of kBtPass reflectors on
+//      the MFMA product (gemm.hpp): Z <- Z - Y (T (Y^T Z)), last pass first.
+//      A pass joins two blocks of kBtNB reflectors, T = [T1 T12; 0 T2] with
+//      T12 = -T1 (Y1^T Y2) T2, so Z is read and written once per 256
+//      reflectors.
 
 #include <cfloat>
 
@@ -40,7 +43,8 @@ namespace cir {
 constexpr int kTriNT = 512;  // threads per matrix in the tridiagonalisation
 constexpr int kTriNW = kTriNT / 64;
 constexpr int kTriNB = 32;   // panel width
-constexpr int kBtNB = 128;   // reflectors per compact-WY block in the back-transform
+constexpr int kBtNB = 128;   // reflectors per compact-WY block (T factor) in the back-transform
+constexpr int kBtPass = 2 * kBtNB;  // reflectors per back-transform pass over Z
 constexpr int kEigNT = 256;  // threads per block of bisection / inverse iteration
 constexpr double kClusterRel = 1e-7;
 constexpr int kMaxCluster = 128;
@@ -1306,12 +1310,14 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
     }
 }
 
-// T factor of a compact-WY block from its Gram matrix G = Y^T Y:
-// T(l,l) = tau_l, T(0:l, l) = -tau_l T(0:l, 0:l) G(0:l, l).
+// T factor of a compact-WY block of kb <= kBtNB reflectors from its Gram
+// matrix G = Y^T Y: T(l,l) = tau_l, T(0:l, l) = -tau_l T(0:l, 0:l) G(0:l, l).
+// G and T are kBtNB x kBtNB blocks with row pitch ldt inside per-matrix
+// arrays of bstride elements.
 template <typename T>
 __global__ void __launch_bounds__(128)
 larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __restrict__ Gall,
-             T* __restrict__ Tall) {
+             T* __restrict__ Tall, int ldt, int bstride) {
     // T is built in LDS, transposed with a padded pitch (Tt[c*kLtP + r] =
     // T(r, c)): the column step reads a row of T with lanes along r and the
     // final copy reads it with lanes along c, both without bank conflicts
@@ -1321,13 +1327,13 @@ larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __res
     T* const col = Tt + kBtNB * kLtP;
     const int b = blockIdx.x, tid = threadIdx.x;
     const T* tau = tauall + size_t(b) * n + j0;
-    const T* G = Gall + size_t(b) * kBtNB * kBtNB;
-    T* Tm = Tall + size_t(b) * kBtNB * kBtNB;
+    const T* G = Gall + size_t(b) * bstride;
+    T* Tm = Tall + size_t(b) * bstride;
     for (int q = tid; q < kBtNB * kLtP; q += 128) Tt[q] = T(0);
     __syncthreads();
     for (int l = 0; l < kb; ++l) {
         const T tl = tau[l];
-        if (tid < l) col[tid] = -tl * G[tid * kBtNB + l];
+        if (tid < l) col[tid] = -tl * G[tid * ldt + l];
         __syncthreads();
         if (tid < l) {
             T acc = T(0);
@@ -1339,7 +1345,7 @@ larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __res
     }
     for (int q = tid; q < kBtNB * kBtNB; q += 128) {
         const int r = q / kBtNB, c = q % kBtNB;
-        Tm[q] = Tt[c * kLtP + r];
+        Tm[r * ldt + c] = Tt[c * kLtP + r];
     }
 }
 
@@ -1363,13 +1369,13 @@ inline EighWork eigh_layout(size_t esz, int batch, int n, int ld) {
     L.off_vw = o;
     o = al(o + esz * batch * 2 * kTriNB * size_t(ld));
     L.off_g = o;
-    o = al(o + esz * batch * kBtNB * kBtNB);
+    o = al(o + esz * batch * kBtPass * kBtPass);
     L.off_t = o;
-    o = al(o + esz * batch * kBtNB * kBtNB);
+    o = al(o + esz * batch * kBtPass * kBtPass);
     L.off_w1 = o;
-    o = al(o + esz * batch * kBtNB * size_t(n));
+    o = al(o + esz * batch * kBtPass * size_t(n));
     L.off_w2 = o;
-    o = al(o + esz * batch * kBtNB * size_t(n));
+    o = al(o + esz * batch * kBtPass * size_t(n));
     L.off_s1 = o;  // single-matrix tridiagonalisation state
     o = al(o + esz * tri1_elems(n));
     const size_t per = invit_scratch_bytes(n);
@@ -1442,95 +1448,68 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(larft_smem_bytes(sizeof(T)))));
     ProfScope pbt(prof, CIR_PH_BACKTRANSFORM, s);
     const int nref = n - 1;
-    const int nblk = int(ceil_div(nref, kBtNB));
-    for (int blk = nblk - 1; blk >= 0; --blk) {
-        const int j0 = blk * kBtNB;
-        const int kb = std::min(kBtNB, nref - j0);
+    const int npass = int(ceil_div(nref, kBtPass));
+    constexpr int P = kBtPass;
+    const int64_t pp = int64_t(P) * P, pn = int64_t(P) * n;
+    for (int pass = npass - 1; pass >= 0; --pass) {
+        const int j0 = pass * P;
+        const int kb = std::min(P, nref - j0);
+        const int kb1 = std::min(kBtNB, kb), kb2 = kb - kb1;
         const int mrows = n - j0 - 1;
         if (mrows <= 0 || kb <= 0) continue;
         const T* Y = A + size_t(j0) * ld + (j0 + 1);  // Y(r, l) = Y[l*ld + r]
         T* Zs = V + size_t(j0 + 1) * ld;                // rows j0+1.. of V
-        GemmArgs g;
-        // G = Y^T Y
-        g.M = kb;
-        g.N = kb;
-        g.K = mrows;
-        g.batch = batch;
-        g.A = Y;
-        g.a_m = ld;
-        g.a_k = 1;
-        g.a_b = int64_t(n) * ld;
-        g.B = Y;
-        g.b_k = 1;
-        g.b_n = ld;
-        g.b_b = int64_t(n) * ld;
-        g.C = G;
-        g.c_m = kBtNB;
-        g.c_n = 1;
-        g.c_b = kBtNB * kBtNB;
-        CIR_TRY(gemm(gemm_prec, g, s));
-        hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(128), larft_smem_bytes(sizeof(T)), s, n, kb, j0, tau,
-                           G, Tm);
+        // C(m, n) = alpha op(A) B + beta C on the per-matrix arrays; the
+        // strides are (m, k) of A, (k, n) of B, (m, n) of C
+        auto mm = [&](int M, int N, int K, const T* a, int64_t a_m, int64_t a_k, int64_t a_b, const T* b,
+                      int64_t b_k, int64_t b_n, int64_t b_b, T* c, int64_t c_m, int64_t c_n, int64_t c_b,
+                      double alpha, double beta) {
+            GemmArgs g;
+            g.M = M;
+            g.N = N;
+            g.K = K;
+            g.batch = batch;
+            g.A = a;
+            g.a_m = a_m;
+            g.a_k = a_k;
+            g.a_b = a_b;
+            g.B = b;
+            g.b_k = b_k;
+            g.b_n = b_n;
+            g.b_b = b_b;
+            g.C = c;
+            g.c_m = c_m;
+            g.c_n = c_n;
+            g.c_b = c_b;
+            g.alpha = alpha;
+            g.beta = beta;
+            return gemm(gemm_prec, g, s);
+        };
+        const int64_t mat = int64_t(n) * ld;
+        // G = Y^T Y (kb x kb, pitch P)
+        CIR_TRY(mm(kb, kb, mrows, Y, ld, 1, mat, Y, 1, ld, mat, G, P, 1, pp, 1.0, 0.0));
+        // T1, T2 from the diagonal blocks of G; T12 = -T1 (G12 T2), with
+        // G12 T2 staged in W2
+        hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(128), larft_smem_bytes(sizeof(T)), s, n, kb1, j0, tau,
+                           G, Tm, P, int(pp));
         CIR_CHECK_LAUNCH();
-        // W1 = Y^T Z
-        g = GemmArgs{};
-        g.M = kb;
-        g.N = n;
-        g.K = mrows;
-        g.batch = batch;
-        g.A = Y;
-        g.a_m = ld;
-        g.a_k = 1;
-        g.a_b = int64_t(n) * ld;
-        g.B = Zs;
-        g.b_k = ld;
-        g.b_n = 1;
-        g.b_b = int64_t(n) * ld;
-        g.C = W1;
-        g.c_m = n;
-        g.c_n = 1;
-        g.c_b = int64_t(kBtNB) * n;
-        CIR_TRY(gemm(gemm_prec, g, s));
-        // W2 = T W1
-        g = GemmArgs{};
-        g.M = kb;
-        g.N = n;
-        g.K = kb;
-        g.batch = batch;
-        g.A = Tm;
-        g.a_m = kBtNB;
-        g.a_k = 1;
-        g.a_b = kBtNB * kBtNB;
-        g.B = W1;
-        g.b_k = n;
-        g.b_n = 1;
-        g.b_b = int64_t(kBtNB) * n;
-        g.C = W2;
-        g.c_m = n;
-        g.c_n = 1;
-        g.c_b = int64_t(kBtNB) * n;
-        CIR_TRY(gemm(gemm_prec, g, s));
+        if (kb2 > 0) {
+            hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(128), larft_smem_bytes(sizeof(T)), s, n, kb2,
+                               j0 + kb1, tau, G + int64_t(kb1) * P + kb1, Tm + int64_t(kb1) * P + kb1, P, int(pp));
+            CIR_CHECK_LAUNCH();
+            CIR_TRY(mm(kb1, kb2, kb2, G + kb1, P, 1, pp, Tm + int64_t(kb1) * P + kb1, P, 1, pp, W2, P, 1, pn, 1.0,
+                       0.0));
+            CIR_TRY(mm(kb1, kb2, kb1, Tm, P, 1, pp, W2, P, 1, pn, Tm + kb1, P, 1, pp, -1.0, 0.0));
+        }
+        // W1 = Y^T Z (kb x n)
+        CIR_TRY(mm(kb, n, mrows, Y, ld, 1, mat, Zs, ld, 1, mat, W1, n, 1, pn, 1.0, 0.0));
+        // W2 = T W1 by block rows (T21 = 0 is never formed)
+        CIR_TRY(mm(kb1, n, kb, Tm, P, 1, pp, W1, n, 1, pn, W2, n, 1, pn, 1.0, 0.0));
+        if (kb2 > 0)
+            CIR_TRY(mm(kb2, n, kb2, Tm + int64_t(kb1) * P + kb1, P, 1, pp, W1 + int64_t(kb1) * n, n, 1, pn,
+                       W2 + int64_t(kb1) * n, n, 1, pn, 1.0, 0.0));
         // Z -= Y W2
-        g = GemmArgs{};
-        g.M = mrows;
-        g.N = n;
-        g.K = kb;
-        g.batch = batch;
-        g.A = Y;
-        g.a_m = 1;
-        g.a_k = ld;
-        g.a_b = int64_t(n) * ld;
-        g.B = W2;
-        g.b_k = n;
-        g.b_n = 1;
-        g.b_b = int64_t(kBtNB) * n;
-        g.C = Zs;
-        g.c_m = ld;
-        g.c_n = 1;
-        g.c_b = int64_t(n) * ld;
-        g.alpha = -1.0;
-        g.beta = 1.0;
-        CIR_TRY(gemm(gemm_prec, g, s));
+        CIR_TRY(mm(mrows, n, kb, Y, 1, ld, mat, W2, n, 1, pn, Zs, ld, 1, mat, -1.0, 1.0));
     }
     return CIR_OK;
 }

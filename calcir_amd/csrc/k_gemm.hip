@@ -373,9 +373,10 @@ int gemm(int prec, const GemmArgs& g, hipStream_t s) {
         return fail(CIR_ERR_ARG, "gemm: a matrix spans more than 2^31 elements");
     const bool big = g.M >= 128 && g.N >= 128;
     if (prec == CIR_PREC_BF16X3) {
-        // short-K updates C += A B (the back-transform's Z -= Y W): 64x64
-        // tiles with the C tile read ahead of the K loop (the 128x128 tile
-        // has no registers left for it)
+        // short-K updates C += A B: 64x64 tiles with the C tile read ahead
+        // of the K loop (the 128x128 tile has no registers left for it).  At
+        // K = 256 (the back-transform's Z -= Y W) the 128x128 tile's operand
+        // reuse wins instead.
         if (g.beta != 0.0 && g.K <= 128) return launch_layout<64, 64, 32, true, true>(g, s);
         return big ? launch_layout<128, 128, 32, true>(g, s) : launch_layout<64, 64, 32, true>(g, s);
     }
