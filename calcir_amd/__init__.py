@@ -104,12 +104,19 @@ SIGNATURES = {
     "cir_xtc_read": (_i, [ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "cir_xtc_write": (_i, [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _f, _i]),
     "cir_xtc_read_full": (_i, [ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "cir_xtc_times": (_i, [ctypes.c_char_p, _i, _vp, _vp]),
+    "cir_xtc_open": (_i, [ctypes.c_char_p, _i, ctypes.POINTER(_vp)]),
+    "cir_xtc_next": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "cir_xtc_close": (None, [_vp]),
     "cir_xtc_write_full": (_i, [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _f, _i]),
     "cir_main": (_i, [_i, ctypes.POINTER(ctypes.c_char_p)]),
     "cir_device_count": (_i, [ctypes.POINTER(_i)]),
     "cir_create": (_i, [ctypes.POINTER(Params), _i, _vp, ctypes.POINTER(_vp)]),
     "cir_run": (_i, [_vp, _i, _vp, _vp, _vp]),
     "cir_run_device": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "cir_stream_begin": (_i, [_vp, _i, _vp]),
+    "cir_stream_push": (_i, [_vp, _i, _i, _vp, _vp]),
+    "cir_stream_end": (_i, [_vp]),
     "cir_get_results": (_i, [_vp, ctypes.POINTER(Results)]),
     "cir_run_many": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _i]),
     "cir_reunit_workspace": (ctypes.c_size_t, [_i]),
@@ -262,6 +269,36 @@ def xtc_read(path, first=0, count=None, full_box=False):
     return xyz, box, times, steps
 
 
+def xtc_times(path):
+    """(times [nframes] ps, box matrices [nframes, 3, 3]) of an XTC file from
+    the frame headers alone (cir_xtc_times skips the coordinate blocks)."""
+    _, nframes = xtc_scan(path)
+    times = np.zeros(nframes, np.float32)
+    box = np.zeros((nframes, 3, 3), np.float32)
+    check(lib().cir_xtc_times(os.fsencode(path), nframes, times.ctypes.data, box.ctypes.data))
+    return times, box
+
+
+def xtc_blocks(path, block):
+    """Consecutive frames of an XTC file read sequentially (cir_xtc_open /
+    cir_xtc_next), yielded as (xyz [k, natoms, 3], box matrices [k, 3, 3],
+    times [k]) with k <= block."""
+    natoms, nframes = xtc_scan(path)
+    L = lib()
+    rd = ctypes.c_void_p()
+    check(L.cir_xtc_open(os.fsencode(path), natoms, ctypes.byref(rd)))
+    try:
+        for f0 in range(0, nframes, block):
+            k = min(block, nframes - f0)
+            xyz = np.zeros((k, natoms, 3), np.float32)
+            box = np.zeros((k, 3, 3), np.float32)
+            times = np.zeros(k, np.float32)
+            check(L.cir_xtc_next(rd, k, xyz.ctypes.data, box.ctypes.data, times.ctypes.data))
+            yield xyz, box, times
+    finally:
+        L.cir_xtc_close(rd)
+
+
 def xtc_write(path, xyz, box, times, precision=1000.0, append=False):
     """Writes frames (xyz [nframes, natoms, 3] nm, box diagonal [nframes, 3]
     or matrices [nframes, 3, 3], times [nframes] ps) as XTC with the given
@@ -322,15 +359,25 @@ def run(p, xyz, box, times, stream=None, profile=False, coefs=None):
         if profile:
             check(L.cir_set_profiling(ctx, 1))
         check(L.cir_run(ctx, nframes, xyz.ctypes.data, box.ctypes.data, times.ctypes.data))
-        r, bufs = _results_buffers(p)
-        check(L.cir_get_results(ctx, ctypes.byref(r)))
-        if profile:
-            secs = (ctypes.c_double * len(PHASES))()
-            cnts = (ctypes.c_longlong * len(PHASES))()
-            check(L.cir_phase_times(ctx, secs, cnts, len(PHASES)))
-            phases = {name: (secs[i], cnts[i]) for i, name in enumerate(PHASES)}
+        r, bufs, phases = _collect(L, ctx, p, profile)
     finally:
         L.cir_destroy(ctx)
+    return _results_dict(r, bufs, phases)
+
+
+def _collect(L, ctx, p, profile):
+    r, bufs = _results_buffers(p)
+    check(L.cir_get_results(ctx, ctypes.byref(r)))
+    phases = None
+    if profile:
+        secs = (ctypes.c_double * len(PHASES))()
+        cnts = (ctypes.c_longlong * len(PHASES))()
+        check(L.cir_phase_times(ctx, secs, cnts, len(PHASES)))
+        phases = {name: (secs[i], cnts[i]) for i, name in enumerate(PHASES)}
+    return r, bufs, phases
+
+
+def _results_dict(r, bufs, phases):
     out = {}
     for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
         out[k] = bufs[k][0::2] + 1j * bufs[k][1::2]
@@ -339,6 +386,40 @@ def run(p, xyz, box, times, stream=None, profile=False, coefs=None):
     out.update(nsteps=r.nsteps, nframes_used=r.nframes_used, hist_skipped=r.hist_skipped,
                seconds=r.seconds, phases=phases)
     return out
+
+
+def run_streamed(p, times, blocks, natoms, profile=False, coefs=None):
+    """cir_run fed block by block (cir_stream_begin / _push / _end): times
+    [nframes] ps of the whole trajectory, blocks an iterable of (xyz, box)
+    arrays of consecutive frames in trajectory order ([k, natoms, 3] nm and
+    [k, 3], or [k, 9] with the triclinic key).  Only the frames the samples
+    use are copied to the device; the trajectory is never resident.  Returns
+    what run() returns."""
+    times = np.ascontiguousarray(times, np.float32)
+    L = lib()
+    ctx = ctypes.c_void_p()
+    check(L.cir_create(ctypes.byref(p), int(natoms), None, ctypes.byref(ctx)))
+    try:
+        if coefs is not None:
+            cf = np.ascontiguousarray(coefs, np.float64)
+            if cf.shape != (NCOEF,):
+                raise ValueError("coefs must hold %d values" % NCOEF)
+            check(L.cir_set_coefs(ctx, cf.ctypes.data))
+        if profile:
+            check(L.cir_set_profiling(ctx, 1))
+        check(L.cir_stream_begin(ctx, times.size, times.ctypes.data))
+        first = 0
+        for xyz, box in blocks:
+            xyz = np.ascontiguousarray(xyz, np.float32)
+            box = np.ascontiguousarray(box, np.float32)
+            k = xyz.shape[0]
+            check(L.cir_stream_push(ctx, first, k, xyz.ctypes.data, box.ctypes.data))
+            first += k
+        check(L.cir_stream_end(ctx))
+        r, bufs, phases = _collect(L, ctx, p, profile)
+    finally:
+        L.cir_destroy(ctx)
+    return _results_dict(r, bufs, phases)
 
 
 def run_many(jobs, nthreads=8):

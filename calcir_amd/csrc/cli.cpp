@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,36 @@ constexpr int kExitIncomplete = 3;
 int fail_msg(int st) {
     std::printf("ERROR: %s: %s\n", cir_strerror(st), cir_last_error());
     return EXIT_FAILURE;
+}
+
+// The diagonals of full boxes, as calcIR.cu:404-405 uses them.
+std::vector<float> diagonal_boxes(const std::vector<float>& box9, int nframes) {
+    std::vector<float> box(size_t(nframes) * 3);
+    for (int f = 0; f < nframes; ++f)
+        for (int k = 0; k < 3; ++k) box[3 * size_t(f) + k] = box9[9 * size_t(f) + 4 * k];
+    return box;
+}
+
+// A batched run fed from the XTC file in blocks of frames, so that the
+// trajectory is never held whole (the reference reads each frame on demand,
+// calcIR.cu:380-386).
+int stream_xtc(cir_ctx* ctx, const cir_params& p, int natoms, int nframes, const std::vector<float>& times) {
+    constexpr int kBlock = 64;
+    int st = cir_stream_begin(ctx, nframes, times.data());
+    if (st != CIR_OK) return st;
+    cir_xtc* rd = nullptr;
+    st = cir_xtc_open(p.xtcf, natoms, &rd);
+    if (st != CIR_OK) return st;
+    std::vector<float> xyz(size_t(kBlock) * natoms * 3), box9(size_t(kBlock) * 9);
+    for (int f0 = 0; f0 < nframes && st == CIR_OK; f0 += kBlock) {
+        const int k = std::min(kBlock, nframes - f0);
+        st = cir_xtc_next(rd, k, xyz.data(), box9.data(), nullptr);
+        if (st != CIR_OK) break;
+        const std::vector<float> box = p.triclinic ? box9 : diagonal_boxes(box9, k);
+        st = cir_stream_push(ctx, f0, k, xyz.data(), box.data());
+    }
+    cir_xtc_close(rd);
+    return st == CIR_OK ? cir_stream_end(ctx) : st;
 }
 
 }  // namespace
@@ -71,6 +102,7 @@ extern "C" int cir_main(int argc, char** argv) {
     unsigned long long seed = 0;
     float boxl = 0.0f;
     std::vector<float> xyz, times, box;
+    bool streamed = false;
     if (parse_synth(p.xtcf, &nwater, &nframes, &seed, &boxl)) {
         natoms = 4 * nwater;
         if (p.natom_mol != 4) {
@@ -96,10 +128,10 @@ extern "C" int cir_main(int argc, char** argv) {
             std::printf("WARNING: %s holds fewer than two frames.\n", p.xtcf);
             return EXIT_FAILURE;
         }
-        xyz.resize(size_t(nframes) * natoms * 3);
+        // the frame times and boxes first (headers only)
         times.resize(nframes);
         std::vector<float> box9(size_t(nframes) * 9);
-        st = cir_xtc_read_full(p.xtcf, 0, nframes, natoms, xyz.data(), box9.data(), times.data(), nullptr);
+        st = cir_xtc_times(p.xtcf, nframes, times.data(), box9.data());
         if (st != CIR_OK) return fail_msg(st);
         // rectangular boxes go through as their diagonal (calcIR.cu:404-405);
         // any off-diagonal element switches to the triclinic minimum image
@@ -109,12 +141,16 @@ extern "C" int cir_main(int argc, char** argv) {
                 if (k % 4 != 0 && box9[9 * size_t(f) + k] != 0.0f) tri = true;
         if (tri) {
             p.triclinic = 1;
-            box = box9;
             std::printf(">>> Triclinic box: using the triclinic minimum image.\n");
-        } else {
-            box.resize(size_t(nframes) * 3);
-            for (int f = 0; f < nframes; ++f)
-                for (int k = 0; k < 3; ++k) box[3 * size_t(f) + k] = box9[9 * size_t(f) + 4 * k];
+        }
+        // batched runs read the coordinates block by block while they run, as
+        // the reference reads frames on demand; checkpointed runs load them
+        streamed = p.mode == CIR_MODE_BATCHED && p.ckpt_every == 0 && p.resume == 0 && p.max_frames == 0;
+        if (!streamed) {
+            xyz.resize(size_t(nframes) * natoms * 3);
+            st = cir_xtc_read_full(p.xtcf, 0, nframes, natoms, xyz.data(), box9.data(), nullptr, nullptr);
+            if (st != CIR_OK) return fail_msg(st);
+            box = tri ? box9 : diagonal_boxes(box9, nframes);
         }
     }
     const int nmol = natoms / p.natom_mol, nchrom = nmol * p.nchrom_mol;
@@ -129,7 +165,10 @@ extern "C" int cir_main(int argc, char** argv) {
     cir_ctx* ctx = nullptr;
     st = cir_create(&p, natoms, nullptr, &ctx);
     if (st != CIR_OK) return fail_msg(st);
-    st = cir_run(ctx, nframes, xyz.data(), box.data(), times.data());
+    if (streamed)
+        st = stream_xtc(ctx, p, natoms, nframes, times);
+    else
+        st = cir_run(ctx, nframes, xyz.data(), box.data(), times.data());
     if (st == CIR_INCOMPLETE) {
         // max_frames reached: the checkpoint holds the state; no output files
         std::printf("\n>>> %s\n>>> Run again with resume 1 to continue.\n", cir_last_error());

@@ -66,6 +66,13 @@ struct AdjDeleter {
     void operator()(AdjState* a) const { adj_release(a); }
 };
 
+// State of a streamed batched run between cir_stream_begin and cir_stream_end.
+struct StreamRun;
+void stream_release(StreamRun* s);
+struct StreamDeleter {
+    void operator()(StreamRun* s) const { stream_release(s); }
+};
+
 }  // namespace cir
 
 struct cir_ctx {
@@ -87,6 +94,7 @@ struct cir_ctx {
     double phase_sec[CIR_NPHASE] = {};
     long long phase_cnt[CIR_NPHASE] = {};
     std::unique_ptr<cir::AdjState, cir::AdjDeleter> adj;
+    std::unique_ptr<cir::StreamRun, cir::StreamDeleter> streamed;
 };
 
 namespace cir {

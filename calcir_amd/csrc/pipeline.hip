@@ -346,163 +346,218 @@ int ckpt_read(const std::string& path, uint64_t key, int64_t* next, std::vector<
     return CIR_OK;
 }
 
-int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
-    const cir_params& p = c->p;
+// A batched run between chunks of distinct frames: the schedule, the device
+// buffers, the propagation state Y and the accumulators.  run_batched drives
+// it over a trajectory resident in device memory; the streaming entry points
+// (cir_stream_*) fill its chunk buffers from host frames as they arrive.
+struct BatchRun {
+    cir_ctx* c = nullptr;
     Schedule S;
-    CIR_TRY(build_schedule(p, nframes, times, S));
-    const int nu = int(S.uniq.size());
-    const int N = c->N, ld = c->ld, ns = p.nsamples, nt = p.ntcfpoints, nom = c->nomega;
-    const int B = auto_chunk(c, nu);
-    const size_t esz = c->esz;
-    const int rp = c->real_prec;
-    hipStream_t st = c->stream;
+    int nu = 0, B = 0, nslot = 0, ldy = 0, R = 0;
     // One stream: the propagation of a chunk follows its eigensolve.  (Run
     // concurrently on a second stream it only competes with the
     // tridiagonalisation, which keeps every CU busy.)
-    DevBuf xyzc, boxc, H, work, wmult, worig, m2, Y0, Y1, G, act, tgt, out, acc, skip;
-    DevBuf sites[1], V[1], w[1];
-    const int set = 0;
-    hipStream_t sp = st;
-    CIR_TRY(xyzc.alloc(size_t(B) * c->natoms * 3 * sizeof(float)));
-    CIR_TRY(boxc.alloc(size_t(B) * box_floats(c) * sizeof(float)));
-    CIR_TRY(sites[set].alloc(size_t(B) * CIR_NSITE * N * esz));
-    CIR_TRY(V[set].alloc(size_t(B) * N * ld * esz));
-    CIR_TRY(w[set].alloc(size_t(B) * N * sizeof(double)));
-    CIR_TRY(H.alloc(size_t(B) * N * ld * esz));
-    CIR_TRY(work.alloc(eigh_workspace(rp, B, N, ld)));
-    CIR_TRY(wmult.alloc(size_t(nu) * sizeof(double)));
-    CIR_TRY(worig.alloc(size_t(nu) * sizeof(double)));
-    CIR_TRY(m2.alloc(size_t(B) * N * sizeof(double)));
-    const int nslot = S.nslot;
-    const int ldy = 18 * nslot;
-    CIR_TRY(Y0.alloc(size_t(N) * ldy * esz));
-    CIR_TRY(Y1.alloc(size_t(N) * ldy * esz));
-    CIR_TRY(G.alloc(propagate_work_elems(N, 9 * nslot) * esz));
-    CIR_TRY(act.alloc(S.action.size() * sizeof(int)));
-    CIR_TRY(tgt.alloc(S.target.size() * sizeof(int)));
-    CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
-    CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
-    CIR_TRY(skip.alloc(sizeof(long long)));
-    // re-unitarisation: mode per (distinct frame, slot): 1 at an origin (record
-    // G0), 2 every reunit_every-th propagated point of a sample
-    const int R = p.reunit_every;
+    DevBuf xyzc, boxc, H, work, wmult, worig, m2, Y0, Y1, G, act, tgt, out, acc, skip, sites, V, w;
+    // re-unitarisation: mode per (distinct frame, slot): 1 at an origin
+    // (record G0), 2 every reunit_every-th propagated point of a sample
     DevBuf rmode, rg0, rwork;
-    std::vector<int> hmode;
     std::vector<char> rany;
-    if (R > 0) {
-        hmode.assign(size_t(nu) * nslot, 0);
-        rany.assign(nu, 0);
-        for (int u = 0; u < nu; ++u)
-            for (int q = 0; q < nslot; ++q) {
-                const size_t e = size_t(u) * nslot + q;
-                const int k = S.target[e] % nt;
-                if (S.action[e] == 1) hmode[e] = 1;
-                if (S.action[e] == 2 && k % R == 0) hmode[e] = 2;
-                rany[u] |= hmode[e] != 0;
-            }
-        CIR_TRY(rmode.alloc(hmode.size() * sizeof(int)));
-        CIR_TRY(rg0.alloc(size_t(nslot) * 81 * sizeof(double)));
-        CIR_TRY(rwork.alloc(reunit_work_doubles(nslot) * sizeof(double)));
-        CIR_HIP(hipMemcpyAsync(rmode.p, hmode.data(), rmode.bytes, hipMemcpyHostToDevice, st));
-        CIR_HIP(hipMemsetAsync(rg0.p, 0, rg0.bytes, st));
-    }
-    CIR_HIP(hipMemcpyAsync(wmult.p, S.mult.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
-    CIR_HIP(hipMemcpyAsync(worig.p, S.orig.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
-    CIR_HIP(hipMemcpyAsync(act.p, S.action.data(), S.action.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    CIR_HIP(hipMemcpyAsync(tgt.p, S.target.data(), S.target.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    CIR_HIP(hipMemsetAsync(out.p, 0, out.bytes, st));
-    CIR_HIP(hipMemsetAsync(acc.p, 0, acc.bytes, st));
-    CIR_HIP(hipMemsetAsync(skip.p, 0, skip.bytes, st));
-    CIR_HIP(hipMemsetAsync(Y0.p, 0, Y0.bytes, st));
-    CIR_HIP(hipMemsetAsync(Y1.p, 0, Y1.bytes, st));
+    void* Ycur = nullptr;
+    void* Ynext = nullptr;
 
-    double* accp = acc.as<double>();
-    void* Ycur = Y0.p;
-    void* Ynext = Y1.p;
-    Prof* pf = &c->prof;
+    int init(cir_ctx* ctx, int nframes, const float* times) {
+        c = ctx;
+        const cir_params& p = c->p;
+        CIR_TRY(build_schedule(p, nframes, times, S));
+        nu = int(S.uniq.size());
+        const int N = c->N, ld = c->ld, ns = p.nsamples, nt = p.ntcfpoints, nom = c->nomega;
+        B = auto_chunk(c, nu);
+        const size_t esz = c->esz;
+        hipStream_t st = c->stream;
+        CIR_TRY(xyzc.alloc(size_t(B) * c->natoms * 3 * sizeof(float)));
+        CIR_TRY(boxc.alloc(size_t(B) * box_floats(c) * sizeof(float)));
+        CIR_TRY(sites.alloc(size_t(B) * CIR_NSITE * N * esz));
+        CIR_TRY(V.alloc(size_t(B) * N * ld * esz));
+        CIR_TRY(w.alloc(size_t(B) * N * sizeof(double)));
+        CIR_TRY(H.alloc(size_t(B) * N * ld * esz));
+        CIR_TRY(work.alloc(eigh_workspace(c->real_prec, B, N, ld)));
+        CIR_TRY(wmult.alloc(size_t(nu) * sizeof(double)));
+        CIR_TRY(worig.alloc(size_t(nu) * sizeof(double)));
+        CIR_TRY(m2.alloc(size_t(B) * N * sizeof(double)));
+        nslot = S.nslot;
+        ldy = 18 * nslot;
+        CIR_TRY(Y0.alloc(size_t(N) * ldy * esz));
+        CIR_TRY(Y1.alloc(size_t(N) * ldy * esz));
+        CIR_TRY(G.alloc(propagate_work_elems(N, 9 * nslot) * esz));
+        CIR_TRY(act.alloc(S.action.size() * sizeof(int)));
+        CIR_TRY(tgt.alloc(S.target.size() * sizeof(int)));
+        CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
+        CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
+        CIR_TRY(skip.alloc(sizeof(long long)));
+        R = p.reunit_every;
+        if (R > 0) {
+            std::vector<int> hmode(size_t(nu) * nslot, 0);
+            rany.assign(nu, 0);
+            for (int u = 0; u < nu; ++u)
+                for (int q = 0; q < nslot; ++q) {
+                    const size_t e = size_t(u) * nslot + q;
+                    const int k = S.target[e] % nt;
+                    if (S.action[e] == 1) hmode[e] = 1;
+                    if (S.action[e] == 2 && k % R == 0) hmode[e] = 2;
+                    rany[u] |= hmode[e] != 0;
+                }
+            CIR_TRY(rmode.alloc(hmode.size() * sizeof(int)));
+            CIR_TRY(rg0.alloc(size_t(nslot) * 81 * sizeof(double)));
+            CIR_TRY(rwork.alloc(reunit_work_doubles(nslot) * sizeof(double)));
+            CIR_HIP(hipMemcpyAsync(rmode.p, hmode.data(), rmode.bytes, hipMemcpyHostToDevice, st));
+            CIR_HIP(hipMemsetAsync(rg0.p, 0, rg0.bytes, st));
+            // the host copy must outlive the asynchronous copy
+            CIR_HIP(hipStreamSynchronize(st));
+        }
+        CIR_HIP(hipMemcpyAsync(wmult.p, S.mult.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(worig.p, S.orig.data(), nu * sizeof(double), hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(act.p, S.action.data(), S.action.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(tgt.p, S.target.data(), S.target.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemsetAsync(out.p, 0, out.bytes, st));
+        CIR_HIP(hipMemsetAsync(acc.p, 0, acc.bytes, st));
+        CIR_HIP(hipMemsetAsync(skip.p, 0, skip.bytes, st));
+        CIR_HIP(hipMemsetAsync(Y0.p, 0, Y0.bytes, st));
+        CIR_HIP(hipMemsetAsync(Y1.p, 0, Y1.bytes, st));
+        Ycur = Y0.p;
+        Ynext = Y1.p;
+        return CIR_OK;
+    }
+
+    // Distinct frames c0 .. c0+nb-1, whose coordinates and boxes are in
+    // xyzc / boxc: field, Hamiltonian, eigensystems, histograms and spectral
+    // density, then frame by frame propagation and correlations.
+    int chunk(int c0, int nb) {
+        const cir_params& p = c->p;
+        const int N = c->N, ld = c->ld, nom = c->nomega;
+        const size_t esz = c->esz;
+        const int rp = c->real_prec;
+        hipStream_t st = c->stream;
+        Prof* pf = &c->prof;
+        double* accp = acc.as<double>();
+        {
+            ProfScope ps(pf, CIR_PH_SITES, st);
+            CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc, p.cutoff,
+                                xyzc.as<float>(), boxc.as<float>(), c->boxk, sites.p, st));
+        }
+        {
+            ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
+            CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxc.as<float>(), c->boxk,
+                                sites.p, H.p, st));
+        }
+        CIR_TRY(eigh_batched(c->prop_prec, nb, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
+        {
+            ProfScope ps(pf, CIR_PH_IPR, st);
+            CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
+                             wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom,
+                             skip.as<long long>(), st));
+        }
+        {
+            ProfScope ps(pf, CIR_PH_SW, st);
+            CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
+                                     w.as<double>(), sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st));
+        }
+        for (int u = c0; u < c0 + nb; ++u) {
+            const int bl = u - c0;
+            const char* Vu = V.as<char>() + size_t(bl) * N * ld * esz;
+            const double* wu = w.as<double>() + size_t(bl) * N;
+            const char* su = sites.as<char>() + size_t(bl) * CIR_NSITE * N * esz;
+            bool has_prop = false;
+            for (int q = 0; q < nslot; ++q) has_prop |= S.action[size_t(u) * nslot + q] == 2;
+            if (has_prop) {
+                ProfScope ps(pf, CIR_PH_PROPAGATE, st);
+                CIR_TRY(propagate(rp, c->prop_prec, N, ld, 9 * nslot, ldy, double(p.dt), Vu, wu, Ycur, Ynext, G.p,
+                                  st));
+            }
+            {
+                ProfScope ps(pf, CIR_PH_CORRELATE, st);
+                CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, st));
+                if (R > 0 && rany[u])
+                    CIR_TRY(reunitarise(rp, N, nslot, ldy, rmode.as<int>() + size_t(u) * nslot, rg0.as<double>(),
+                                        Ynext, rwork.as<double>(), st));
+                CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot,
+                                  tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), st));
+            }
+            std::swap(Ycur, Ynext);
+        }
+        return CIR_OK;
+    }
+
+    // Host copies of the state a checkpoint holds.
+    int save(std::vector<unsigned char>& hy, std::vector<double>& ho, std::vector<double>& ha, long long& hs,
+             std::vector<double>& hg) {
+        hipStream_t st = c->stream;
+        hy.resize(Y0.bytes);
+        ho.resize(out.bytes / sizeof(double));
+        ha.resize(acc.bytes / sizeof(double));
+        hg.resize(R > 0 ? size_t(nslot) * 81 : 0);
+        if (R > 0) CIR_HIP(hipMemcpyAsync(hg.data(), rg0.p, rg0.bytes, hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipMemcpyAsync(hy.data(), Ycur, hy.size(), hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipMemcpyAsync(ho.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipMemcpyAsync(ha.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipMemcpyAsync(&hs, skip.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipStreamSynchronize(st));
+        return CIR_OK;
+    }
+
+    // The accumulators to the host and the outputs into the context.
+    int finish() {
+        hipStream_t st = c->stream;
+        std::vector<double> hout(out.bytes / sizeof(double)), hacc(acc.bytes / sizeof(double));
+        long long hskip = 0;
+        CIR_HIP(hipMemcpyAsync(hout.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipMemcpyAsync(hacc.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipMemcpyAsync(&hskip, skip.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+        CIR_HIP(hipStreamSynchronize(st));
+        c->nframes_used = nu;
+        return finalize(c, hout, hacc, hskip);
+    }
+};
+
+int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
+    const cir_params& p = c->p;
+    BatchRun r;
+    CIR_TRY(r.init(c, nframes, times));
+    const int nu = r.nu, B = r.B;
+    hipStream_t st = c->stream;
     const bool use_ckpt = p.ckpt_every > 0 || p.resume != 0 || p.max_frames > 0;
     uint64_t key = 0;
-    if (use_ckpt) CIR_TRY(job_key(c, S, nframes, xyz_d, times, &key));
+    if (use_ckpt) CIR_TRY(job_key(c, r.S, nframes, xyz_d, times, &key));
     int start = 0;
     if (p.resume) {
-        std::vector<unsigned char> hy(Y0.bytes);
-        std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom)), hg(R > 0 ? size_t(nslot) * 81 : 0);
+        std::vector<unsigned char> hy(r.Y0.bytes);
+        std::vector<double> ho(r.out.bytes / sizeof(double)), ha(r.acc.bytes / sizeof(double));
+        std::vector<double> hg(r.R > 0 ? size_t(r.nslot) * 81 : 0);
         long long hs = 0;
         int64_t next = 0;
         CIR_TRY(ckpt_read(ckpt_path(p), key, &next, hy, ho, ha, &hs, hg));
-        if (R > 0) CIR_HIP(hipMemcpyAsync(rg0.p, hg.data(), rg0.bytes, hipMemcpyHostToDevice, st));
         if (next < 0 || next > nu) return fail(CIR_ERR_IO, "resume: checkpoint position out of range");
-        CIR_HIP(hipMemcpyAsync(Y0.p, hy.data(), hy.size(), hipMemcpyHostToDevice, st));
-        CIR_HIP(hipMemcpyAsync(out.p, ho.data(), out.bytes, hipMemcpyHostToDevice, st));
-        CIR_HIP(hipMemcpyAsync(acc.p, ha.data(), acc.bytes, hipMemcpyHostToDevice, st));
-        CIR_HIP(hipMemcpyAsync(skip.p, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+        if (r.R > 0) CIR_HIP(hipMemcpyAsync(r.rg0.p, hg.data(), r.rg0.bytes, hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(r.Y0.p, hy.data(), hy.size(), hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(r.out.p, ho.data(), r.out.bytes, hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(r.acc.p, ha.data(), r.acc.bytes, hipMemcpyHostToDevice, st));
+        CIR_HIP(hipMemcpyAsync(r.skip.p, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
         CIR_HIP(hipStreamSynchronize(st));
         start = int(next);
     }
     long long done_here = 0, since_ckpt = 0;
     for (int c0 = start; c0 < nu; c0 += B) {
         const int nb = std::min(B, nu - c0);
-        CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxc.as<float>()));
-        {
-            ProfScope ps(pf, CIR_PH_SITES, st);
-            CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc,
-                                    p.cutoff, xyzc.as<float>(), boxc.as<float>(), c->boxk, sites[set].p, st));
-        }
-        {
-            ProfScope ps(pf, CIR_PH_HAMILTONIAN, st);
-            CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxc.as<float>(), c->boxk,
-                                    sites[set].p, H.p, st));
-        }
-        CIR_TRY(eigh_batched(c->prop_prec, nb, N, ld, H.p, V[set].p, w[set].as<double>(), work.p, st, pf));
-        {
-            ProfScope ps(pf, CIR_PH_IPR, st);
-            CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V[set].p,
-                             w[set].as<double>(), wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom,
-                             skip.as<long long>(), st));
-        }
-        {
-            ProfScope ps(pf, CIR_PH_SW, st);
-            CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V[set].p,
-                                     w[set].as<double>(), sites[set].p, worig.as<double>() + c0, accp,
-                                     m2.as<double>(), st));
-        }
-        for (int u = c0; u < c0 + nb; ++u) {
-            const int bl = u - c0;
-            const char* Vu = V[set].as<char>() + size_t(bl) * N * ld * esz;
-            const double* wu = w[set].as<double>() + size_t(bl) * N;
-            const char* su = sites[set].as<char>() + size_t(bl) * CIR_NSITE * N * esz;
-            bool has_prop = false;
-            for (int q = 0; q < nslot; ++q) has_prop |= S.action[size_t(u) * nslot + q] == 2;
-            if (has_prop) {
-                ProfScope ps(pf, CIR_PH_PROPAGATE, sp);
-                CIR_TRY(propagate(rp, c->prop_prec, N, ld, 9 * nslot, ldy, double(p.dt), Vu, wu, Ycur, Ynext, G.p,
-                                  sp));
-            }
-            {
-                ProfScope ps(pf, CIR_PH_CORRELATE, sp);
-                CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, sp));
-                if (R > 0 && rany[u])
-                    CIR_TRY(reunitarise(rp, N, nslot, ldy, rmode.as<int>() + size_t(u) * nslot, rg0.as<double>(),
-                                        Ynext, rwork.as<double>(), sp));
-                CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot,
-                                  tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), sp));
-            }
-            std::swap(Ycur, Ynext);
-        }
+        CIR_TRY(gather_frames(c, xyz_d, box_d, r.S.uniq.data() + c0, nb, r.xyzc.as<float>(), r.boxc.as<float>()));
+        CIR_TRY(r.chunk(c0, nb));
         done_here += nb;
         since_ckpt += nb;
         const bool more = c0 + nb < nu;
         const bool halt = more && p.max_frames > 0 && done_here >= p.max_frames;
         if (more && (halt || (p.ckpt_every > 0 && since_ckpt >= p.ckpt_every))) {
-            std::vector<unsigned char> hy(Y0.bytes);
-            std::vector<double> ho(size_t(ns) * nt * 6), ha(4 * size_t(nom)), hg(R > 0 ? size_t(nslot) * 81 : 0);
+            std::vector<unsigned char> hy;
+            std::vector<double> ho, ha, hg;
             long long hs = 0;
-            if (R > 0) CIR_HIP(hipMemcpyAsync(hg.data(), rg0.p, rg0.bytes, hipMemcpyDeviceToHost, st));
-            CIR_HIP(hipMemcpyAsync(hy.data(), Ycur, hy.size(), hipMemcpyDeviceToHost, st));
-            CIR_HIP(hipMemcpyAsync(ho.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
-            CIR_HIP(hipMemcpyAsync(ha.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
-            CIR_HIP(hipMemcpyAsync(&hs, skip.p, sizeof(hs), hipMemcpyDeviceToHost, st));
-            CIR_HIP(hipStreamSynchronize(st));
+            CIR_TRY(r.save(hy, ho, ha, hs, hg));
             CIR_TRY(ckpt_write(ckpt_path(p), key, c0 + nb, hy, ho, ha, hs, hg));
             since_ckpt = 0;
         }
@@ -511,15 +566,26 @@ int run_batched(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d,
             return CIR_INCOMPLETE;
         }
     }
-    std::vector<double> hout(size_t(ns) * nt * 6), hacc(4 * size_t(nom));
-    long long hskip = 0;
-    CIR_HIP(hipMemcpyAsync(hout.data(), out.p, out.bytes, hipMemcpyDeviceToHost, st));
-    CIR_HIP(hipMemcpyAsync(hacc.data(), acc.p, acc.bytes, hipMemcpyDeviceToHost, st));
-    CIR_HIP(hipMemcpyAsync(&hskip, skip.p, sizeof(long long), hipMemcpyDeviceToHost, st));
-    CIR_HIP(hipStreamSynchronize(st));
-    c->nframes_used = nu;
-    return finalize(c, hout, hacc, hskip);
+    return r.finish();
 }
+
+// A batched run fed from the host: the schedule comes from the frame times
+// given up front (calcIR.cu:360-395 fixes the frames from the times alone),
+// the frames then arrive in trajectory order, only those the schedule uses are
+// copied into the chunk buffers, and a chunk runs as soon as its last distinct
+// frame has arrived.  The trajectory never has to be resident, as the
+// reference's on-demand xdr_seek/read_xtc reads (calcIR.cu:380-386) allow.
+struct StreamRun {
+    BatchRun run;
+    int nframes = 0;
+    int next = 0;  // trajectory index of the next frame to arrive
+    int c0 = 0;    // first distinct frame of the chunk being filled
+    hipEvent_t e0 = nullptr;
+    ~StreamRun() {
+        if (e0) (void)hipEventDestroy(e0);
+    }
+};
+void stream_release(StreamRun* s) { delete s; }
 
 int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
     const cir_params& p = c->p;
@@ -670,6 +736,15 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     return CIR_OK;
 }
 
+namespace {
+void reset_phases(cir_ctx* c) {
+    for (int i = 0; i < CIR_NPHASE; ++i) {
+        c->phase_sec[i] = 0.0;
+        c->phase_cnt[i] = 0;
+    }
+}
+}  // namespace
+
 extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
     if (!c || !xyz_d || !box_d || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_run_device: bad arguments");
     {
@@ -680,10 +755,7 @@ extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const
         CIR_TRY(check_boxes(c, nframes, hb.data()));
     }
     c->have = false;
-    for (int i = 0; i < CIR_NPHASE; ++i) {
-        c->phase_sec[i] = 0.0;
-        c->phase_cnt[i] = 0;
-    }
+    reset_phases(c);
     hipEvent_t e0, e1;
     CIR_HIP(hipEventCreate(&e0));
     CIR_HIP(hipEventCreate(&e1));
@@ -696,6 +768,93 @@ extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    c->prof.collect(c->phase_sec, c->phase_cnt, CIR_NPHASE);
+    return st;
+}
+
+extern "C" int cir_stream_begin(cir_ctx* c, int nframes, const float* times) {
+    if (!c || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_stream_begin: bad arguments");
+    c->streamed.reset();
+    if (c->p.mode == CIR_MODE_PLAIN) return fail(CIR_ERR_ARG, "cir_stream_begin: streaming needs mode batched");
+    if (c->p.ckpt_every > 0 || c->p.resume != 0 || c->p.max_frames > 0)
+        return fail(CIR_ERR_ARG,
+                    "cir_stream_begin: checkpoint_every, resume and max_frames need the whole trajectory (cir_run)");
+    c->have = false;
+    reset_phases(c);
+    std::unique_ptr<StreamRun, StreamDeleter> s(new StreamRun());
+    CIR_HIP(hipEventCreate(&s->e0));
+    CIR_HIP(hipEventRecord(s->e0, c->stream));
+    CIR_TRY(s->run.init(c, nframes, times));
+    s->nframes = nframes;
+    c->streamed = std::move(s);
+    return CIR_OK;
+}
+
+namespace {
+int stream_push(cir_ctx* c, int first, int count, const float* xyz, const float* box) {
+    if (count < 0 || (count > 0 && (!xyz || !box))) return fail(CIR_ERR_ARG, "cir_stream_push: bad arguments");
+    StreamRun* s = c->streamed.get();
+    if (!s) return fail(CIR_ERR_ARG, "cir_stream_push: no stream is open (cir_stream_begin)");
+    if (first != s->next)
+        return fail(CIR_ERR_ARG, "cir_stream_push: frames arrive in trajectory order; expected frame " +
+                                     std::to_string(s->next) + ", got " + std::to_string(first));
+    if (count > s->nframes - first) return fail(CIR_ERR_ARG, "cir_stream_push: frames past the end of the trajectory");
+    const size_t fx = size_t(c->natoms) * 3;
+    const int bs = box_floats(c);
+    for (size_t i = 0; i < fx * count; ++i)
+        if (!std::isfinite(xyz[i])) return fail(CIR_ERR_NONFINITE, "non-finite coordinate in the trajectory");
+    CIR_TRY(check_boxes(c, count, box));
+    BatchRun& r = s->run;
+    int st = CIR_OK;
+    for (int k = 0; k < count && st == CIR_OK; ++k) {
+        const int u = r.S.uof[size_t(first) + k];
+        if (u < 0) continue;
+        // distinct frames arrive in ascending order and a chunk runs when its
+        // last one arrives, so this one is at 0 <= slot < B
+        const int slot = u - s->c0;
+        CIR_HIP(hipMemcpyAsync(r.xyzc.as<float>() + size_t(slot) * fx, xyz + size_t(k) * fx, fx * sizeof(float),
+                               hipMemcpyHostToDevice, c->stream));
+        CIR_HIP(hipMemcpyAsync(r.boxc.as<float>() + size_t(slot) * bs, box + size_t(k) * bs, bs * sizeof(float),
+                               hipMemcpyHostToDevice, c->stream));
+        const int nb = std::min(r.B, r.nu - s->c0);
+        if (slot == nb - 1) {
+            st = r.chunk(s->c0, nb);
+            s->c0 += nb;
+        }
+    }
+    s->next = first + count;
+    // the caller may reuse its arrays once this returns
+    if (st == CIR_OK) {
+        const hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+    }
+    return st;
+}
+}  // namespace
+
+extern "C" int cir_stream_push(cir_ctx* c, int first, int count, const float* xyz, const float* box) {
+    if (!c) return fail(CIR_ERR_ARG, "cir_stream_push: null context");
+    const int st = stream_push(c, first, count, xyz, box);
+    if (st != CIR_OK) c->streamed.reset();  // a failed push closes the stream
+    return st;
+}
+
+extern "C" int cir_stream_end(cir_ctx* c) {
+    if (!c) return fail(CIR_ERR_ARG, "cir_stream_end: null context");
+    std::unique_ptr<StreamRun, StreamDeleter> s = std::move(c->streamed);
+    if (!s) return fail(CIR_ERR_ARG, "cir_stream_end: no stream is open (cir_stream_begin)");
+    if (s->c0 < s->run.nu)
+        return fail(CIR_ERR_FRAME, "cir_stream_end: trajectory frame " + std::to_string(s->run.S.uniq[s->c0]) +
+                                     ", which the samples use, was never pushed");
+    const int st = s->run.finish();
+    hipEvent_t e1;
+    if (hipEventCreate(&e1) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, s->e0, e1) == hipSuccess)
+            c->seconds = ms * 1e-3;
+        (void)hipEventDestroy(e1);
+    }
     c->prof.collect(c->phase_sec, c->phase_cnt, CIR_NPHASE);
     return st;
 }

@@ -468,10 +468,89 @@ bool read_header(XdrIn& x, int* natoms, int* step, float* time, float box[9]) {
     return x.ok;
 }
 
+// Skips one frame's coordinate block: its header gives the byte count of the
+// compressed data, so nothing is decoded.
+bool skip_coords(XdrIn& x, int natoms) {
+    const int lsize = x.i32();
+    if (!x.ok || lsize != natoms) return false;
+    if (natoms <= 9) return std::fseek(x.f, 12L * natoms, SEEK_CUR) == 0;
+    for (int k = 0; k < 8; ++k) (void)x.i32();  // precision, minint[3], maxint[3], smallidx
+    const int nbytes = x.i32();
+    if (!x.ok || nbytes < 0) return false;
+    return std::fseek(x.f, (long(nbytes) + 3) & ~3L, SEEK_CUR) == 0;
+}
+
 }  // namespace
 }  // namespace cir
 
 using namespace cir;
+
+struct cir_xtc {
+    FILE* f = nullptr;
+    int natoms = 0;
+    int frame = 0;  // index of the next frame
+    ~cir_xtc() {
+        if (f) std::fclose(f);
+    }
+};
+
+extern "C" int cir_xtc_open(const char* path, int natoms, cir_xtc** out) {
+    if (!path || !out || natoms < 0) return fail(CIR_ERR_ARG, "cir_xtc_open: bad argument");
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
+    cir_xtc* r = new cir_xtc();
+    r->f = f;
+    r->natoms = natoms;
+    *out = r;
+    return CIR_OK;
+}
+
+extern "C" int cir_xtc_next(cir_xtc* r, int count, float* xyz, float* box9, float* times) {
+    if (!r || count < 0 || (count > 0 && !xyz)) return fail(CIR_ERR_ARG, "cir_xtc_next: bad argument");
+    XdrIn x{r->f};
+    for (int k = 0; k < count; ++k) {
+        int nat, step;
+        float t, bx[9];
+        if (!read_header(x, &nat, &step, &t, bx)) return fail(CIR_ERR_FRAME, "xtc: frame " + std::to_string(r->frame) + " missing");
+        if (nat != r->natoms) return fail(CIR_ERR_IO, "xtc: atom count differs from the expected " + std::to_string(r->natoms));
+        if (!read_coords(x, nat, xyz + size_t(k) * 3 * nat))
+            return fail(CIR_ERR_IO, "xtc: corrupt coordinates in frame " + std::to_string(r->frame));
+        if (box9)
+            for (int q = 0; q < 9; ++q) box9[9 * size_t(k) + q] = bx[q];
+        if (times) times[k] = t;
+        ++r->frame;
+    }
+    return CIR_OK;
+}
+
+extern "C" void cir_xtc_close(cir_xtc* r) { delete r; }
+
+extern "C" int cir_xtc_times(const char* path, int nframes, float* times, float* box9) {
+    if (!path || nframes < 0) return fail(CIR_ERR_ARG, "cir_xtc_times: bad argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
+    XdrIn x{f};
+    int na = -1;
+    for (int k = 0; k < nframes; ++k) {
+        int nat, step;
+        float t, bx[9];
+        if (!read_header(x, &nat, &step, &t, bx)) {
+            std::fclose(f);
+            return fail(CIR_ERR_FRAME, "xtc: frame " + std::to_string(k) + " missing");
+        }
+        if ((na >= 0 && nat != na) || !skip_coords(x, nat)) {
+            std::fclose(f);
+            return fail(CIR_ERR_IO, "xtc: corrupt frame " + std::to_string(k));
+        }
+        na = nat;
+        if (times) times[k] = t;
+        if (box9)
+            for (int q = 0; q < 9; ++q) box9[9 * size_t(k) + q] = bx[q];
+    }
+    std::fclose(f);
+    return CIR_OK;
+}
 
 extern "C" int cir_xtc_scan(const char* path, int* natoms, int* nframes) {
     if (!path || !natoms || !nframes) return fail(CIR_ERR_ARG, "cir_xtc_scan: null argument");

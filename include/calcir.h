@@ -23,6 +23,7 @@
  *   defaults                           calcIR.cu:36-59   -> cir_params_default
  *   model/species resolution           calcIR.cu:324-339 -> cir_resolve_model/_species
  *   sample x frame loop + finalize     calcIR.cu:360-919 -> cir_create/cir_run/cir_get_results
+ *                                      (or cir_stream_begin/_push/_end with frames fed in order)
  *   get_eproj_GPU                      calcIR.cu:1020-1168, calcIR.h:51-53 -> cir_eproj_sites
  *   get_kappa_GPU                      calcIR.cu:1176-1406, calcIR.h:56-60 -> cir_eproj_sites + cir_hamiltonian
  *   magma_ssyevd_gpu                   calcIR.cu:420-448 -> cir_eigh_batched
@@ -34,7 +35,8 @@
  *   cufftExecC2R x 3                   calcIR.cu:775-827 -> cir_spectrum_c2r
  *   output writers                     calcIR.cu:843-919 -> cir_write_outputs
  *   minImage/mag3/dot3                 calcIR.cu:1457-1476, calcIR.h:79-88 -> cir_minimage
- *   xdrfile read_xtc* / xdr_seek       calcIR.cu:212-219, 341-395 -> cir_xtc_scan/_read (+ _write)
+ *   xdrfile read_xtc* / xdr_seek       calcIR.cu:212-219, 341-395 -> cir_xtc_scan/_read/_times,
+ *                                      cir_xtc_open/_next/_close (+ _write)
  */
 #ifndef CALCIR_H
 #define CALCIR_H
@@ -204,6 +206,18 @@ int   cir_xtc_read_full(const char* path, int first, int count, int natoms, floa
                         float* times, int* steps);
 int   cir_xtc_write_full(const char* path, int natoms, int nframes, const float* xyz, const float* box9,
                          const float* times, float precision, int append);
+/* Times and full boxes of the first nframes frames without decoding any
+ * coordinates (each compressed block is skipped by its byte count): times
+ * nframes ps, box9 nframes*9 (either may be NULL). */
+int   cir_xtc_times(const char* path, int nframes, float* times, float* box9);
+/* Sequential reading, the role of xdrfile's open + read_xtc loop
+ * (calcIR.cu:341-395): cir_xtc_open starts at frame 0; cir_xtc_next decodes
+ * the next count frames (xyz count*natoms*3 nm, box9 count*9, times count;
+ * box9 and times may be NULL) and fails with CIR_ERR_FRAME past the end. */
+typedef struct cir_xtc cir_xtc;
+int   cir_xtc_open(const char* path, int natoms, cir_xtc** out);
+int   cir_xtc_next(cir_xtc* reader, int count, float* xyz, float* box9, float* times);
+void  cir_xtc_close(cir_xtc* reader);
 /* The drop-in CLI: argv as calcIR.exe (calcIR.cu:9-18). */
 int   cir_main(int argc, char** argv);
 
@@ -216,6 +230,20 @@ int   cir_run(cir_ctx* ctx, int nframes, const float* xyz, const float* box, con
 /* Same with the trajectory already on the device. */
 int   cir_run_device(cir_ctx* ctx, int nframes, const float* xyz_d, const float* box_d,
                      const float* times);
+/* Streaming form of cir_run for trajectories that are never resident, as the
+ * reference reads each frame on demand (xdr_seek + read_xtc,
+ * calcIR.cu:341-395).  cir_stream_begin takes the times of all nframes
+ * frames, which fix the frames every (sample, point) uses (calcIR.cu:360-395);
+ * cir_stream_push hands over frames first..first+count-1 (host arrays, xyz
+ * count*natoms*3 nm, box count*3, or count*9 with the triclinic key) in
+ * trajectory order, and runs each chunk of used frames once its last frame
+ * has arrived; cir_stream_end finishes the run (results as after cir_run)
+ * and fails with CIR_ERR_FRAME if a used frame never arrived.  Batched mode
+ * only, without checkpoint_every / resume / max_frames.  A failing push
+ * closes the stream. */
+int   cir_stream_begin(cir_ctx* ctx, int nframes, const float* times);
+int   cir_stream_push(cir_ctx* ctx, int first, int count, const float* xyz, const float* box);
+int   cir_stream_end(cir_ctx* ctx);
 /* Independent jobs at once on one GPU (for example boxes of uneven size):
  * job j is cir_run(ctxs[j], nframes[j], xyz[j], box[j], times[j]); each job
  * runs on its own stream (a context bound to the null stream gets one for the

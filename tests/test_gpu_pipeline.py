@@ -336,3 +336,112 @@ def test_reunitarisation_survives_checkpoint_resume(tmp_path):
     g = C.run(C.default_params(cptf=ck, resume=1, **kw), xyz, box, times)
     for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
         assert rel(g[k], ref[k]) < 1e-12, (k, rel(g[k], ref[k]))
+
+
+def _blocks(xyz, box, sizes):
+    first = 0
+    for k in sizes:
+        yield xyz[first:first + k], box[first:first + k]
+        first += k
+    if first < xyz.shape[0]:
+        yield xyz[first:], box[first:]
+
+
+@pytest.mark.parametrize("prec,tol,chunk,sizes", [
+    ("fp64", 1e-12, 8, (1, 7, 13, 1, 2)),  # ragged blocks across chunk ends
+    ("fp64", 1e-12, 0, ()),                 # the whole trajectory in one push
+    ("bf16x3", 1e-5, 16, (1,) * 60),        # frame by frame
+])
+def test_streamed_run_equals_cir_run(prec, tol, chunk, sizes):
+    # cir_stream_begin / _push / _end give cir_run's results: the same
+    # schedule, the same chunks, frames copied as they arrive
+    xyz, box, times = frames(128, 60, 1001)
+    p = C.default_params(precision=prec, mode="batched", chunk=chunk, **KW)
+    announce("stream prec=%s chunk=%d blocks=%d" % (prec, chunk, len(sizes)))
+    ref = C.run(p, xyz, box, times)
+    g = C.run_streamed(p, times, _blocks(xyz, box, sizes), xyz.shape[1])
+    assert g["nframes_used"] == ref["nframes_used"] == 50 and g["nsteps"] == ref["nsteps"]
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        assert rel(g[k], ref[k]) < tol, (k, rel(g[k], ref[k]))
+    for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
+        assert rel_l2(g[k], ref[k]) < tol, k
+    assert rel(g["sw"], ref["sw"]) < tol
+    assert np.abs(g["pw"] - ref["pw"]).max() <= (1e-12 if prec == "fp64" else 1e-3) * ref["pw"].max()
+
+
+def test_streamed_triclinic_run_equals_cir_run():
+    from cases import triclinic_frames
+    xyz, box9, times = triclinic_frames(64, 40, 31)
+    box9 = np.ascontiguousarray(box9.reshape(len(times), 9))
+    p = C.default_params(precision="fp64", mode="batched", chunk=8, triclinic=1,
+                         **dict(KW, nsamples=2, ntcfpoints=12))
+    announce("stream triclinic fp64")
+    ref = C.run(p, xyz, box9, times)
+    g = C.run_streamed(p, times, _blocks(xyz, box9, (5, 5, 11)), xyz.shape[1])
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        assert rel(g[k], ref[k]) < 1e-12, k
+
+
+def test_stream_guards():
+    import ctypes
+    xyz, box, times = frames(32, 24, 5)
+    L = C.lib()
+    kw = dict(KW, nsamples=2, ntcfpoints=8, sampleEvery=0.05)
+    natoms = xyz.shape[1]
+
+    def ctx_for(**over):
+        p = C.default_params(precision="fp64", mode="batched", **dict(kw, **over))
+        ctx = ctypes.c_void_p()
+        C.check(L.cir_create(ctypes.byref(p), natoms, None, ctypes.byref(ctx)))
+        return ctx
+
+    def push(ctx, first, count):
+        x = np.ascontiguousarray(xyz[first:first + count])
+        b = np.ascontiguousarray(box[first:first + count])
+        return L.cir_stream_push(ctx, first, count, x.ctypes.data, b.ctypes.data)
+
+    announce("stream guards")
+    ctx = ctx_for()
+    try:
+        # push without begin; frames out of order close the stream
+        assert push(ctx, 0, 2) == 1
+        C.check(L.cir_stream_begin(ctx, len(times), times.ctypes.data))
+        C.check(push(ctx, 0, 4))
+        assert push(ctx, 5, 2) == 1
+        assert "expected frame 4" in L.cir_last_error().decode()
+        assert push(ctx, 4, 2) == 1  # the failed stream is closed
+        # a used frame never pushed: CIR_ERR_FRAME at the end
+        C.check(L.cir_stream_begin(ctx, len(times), times.ctypes.data))
+        C.check(push(ctx, 0, 10))
+        assert L.cir_stream_end(ctx) == 6
+        # frames past the end, non-finite coordinates
+        C.check(L.cir_stream_begin(ctx, len(times), times.ctypes.data))
+        C.check(push(ctx, 0, 20))
+        x = np.ascontiguousarray(np.concatenate([xyz[20:], xyz[:1]]))
+        b = np.ascontiguousarray(np.concatenate([box[20:], box[:1]]))
+        assert L.cir_stream_push(ctx, 20, 5, x.ctypes.data, b.ctypes.data) == 1
+        C.check(L.cir_stream_begin(ctx, len(times), times.ctypes.data))
+        bad = np.array(xyz[:1])
+        bad[0, 3, 1] = np.nan
+        assert L.cir_stream_push(ctx, 0, 1, bad.ctypes.data, np.ascontiguousarray(box[:1]).ctypes.data) == 10
+        # a complete stream after the failures gives cir_run's result
+        C.check(L.cir_stream_begin(ctx, len(times), times.ctypes.data))
+        C.check(push(ctx, 0, len(times)))
+        C.check(L.cir_stream_end(ctx))
+        r, bufs, _ = C._collect(L, ctx, C.default_params(precision="fp64", mode="batched", **kw), False)
+        g = C._results_dict(r, bufs, None)
+    finally:
+        L.cir_destroy(ctx)
+    ref = C.run(C.default_params(precision="fp64", mode="batched", **kw), xyz, box, times)
+    assert rel(g["tcf_ir"], ref["tcf_ir"]) < 1e-12
+    # plain mode and the checkpoint keys are refused at begin
+    for over in (dict(mode="plain"), dict(checkpoint_every=4), dict(max_frames=4)):
+        ctx = ctx_for(**over) if "mode" not in over else None
+        if ctx is None:
+            p = C.default_params(precision="fp64", **dict(kw, **over))
+            ctx = ctypes.c_void_p()
+            C.check(L.cir_create(ctypes.byref(p), natoms, None, ctypes.byref(ctx)))
+        try:
+            assert L.cir_stream_begin(ctx, len(times), times.ctypes.data) == 1, over
+        finally:
+            L.cir_destroy(ctx)

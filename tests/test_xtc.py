@@ -117,3 +117,33 @@ def test_subset_append_and_errors(tmp_path):
     with pytest.raises(C.CalcirError) as e:
         C.xtc_scan(tmp_path / "missing.xtc")
     assert e.value.status == 2
+
+
+@pytest.mark.parametrize("nwater", [2, 64])  # uncompressed (natoms <= 9) and compressed frames
+def test_header_pass_and_sequential_blocks(tmp_path, nwater):
+    # cir_xtc_times (coordinate blocks skipped by their byte counts) and the
+    # sequential reader (cir_xtc_open / _next) give what cir_xtc_read gives
+    xyz, box, times = C.synth_trajectory(nwater, 23, 17)
+    tri = np.zeros((23, 3, 3), np.float32)
+    tri[:, 0, 0], tri[:, 1, 1], tri[:, 2, 2] = box[:, 0], box[:, 1], box[:, 2]
+    tri[:, 1, 0] = 0.1 * box[:, 0]  # an off-diagonal element to carry through
+    f = tmp_path / "b.xtc"
+    C.xtc_write(f, xyz, tri, times)
+    xr, br, tr, _ = C.xtc_read(f, full_box=True)
+    th, bh = C.xtc_times(f)
+    assert np.array_equal(th, tr) and np.array_equal(bh, br)
+    got = list(C.xtc_blocks(f, 7))
+    assert [g[0].shape[0] for g in got] == [7, 7, 7, 2]
+    assert np.array_equal(np.concatenate([g[0] for g in got]), xr)
+    assert np.array_equal(np.concatenate([g[1] for g in got]), br)
+    assert np.array_equal(np.concatenate([g[2] for g in got]), tr)
+    # reading past the end is a missing frame
+    L = C.lib()
+    import ctypes
+    rd = ctypes.c_void_p()
+    C.check(L.cir_xtc_open(os.fsencode(f), xyz.shape[1], ctypes.byref(rd)))
+    buf = np.zeros((24,) + xyz.shape[1:], np.float32)
+    try:
+        assert L.cir_xtc_next(rd, 24, buf.ctypes.data, None, None) == 6
+    finally:
+        L.cir_xtc_close(rd)
