@@ -60,7 +60,7 @@ class Params(ctypes.Structure):
         ("cutoff", ctypes.c_float), ("precision", ctypes.c_int), ("mode", ctypes.c_int),
         ("chunk", ctypes.c_int), ("ckpt_every", ctypes.c_int), ("resume", ctypes.c_int),
         ("max_frames", ctypes.c_int), ("adj_store", ctypes.c_int), ("triclinic", ctypes.c_int),
-        ("reunit_every", ctypes.c_int),
+        ("reunit_every", ctypes.c_int), ("origin_stride", ctypes.c_int),
     ]
 
 
@@ -188,6 +188,8 @@ def default_params(**kw):
             k = "adj_store"
         if k == "reunitarise_every":
             k = "reunit_every"
+        if k == "origin_stride_frames":
+            k = "origin_stride"
         if k == "precision" and isinstance(v, str):
             v = {"fp32": PREC_FP32, "fp64": PREC_FP64, "bf16x3": PREC_BF16X3}[v]
         if k == "mode" and isinstance(v, str):

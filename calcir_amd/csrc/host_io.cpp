@@ -59,6 +59,7 @@ extern "C" void cir_params_default(cir_params* p) {
     p->adj_store = 0;
     p->triclinic = 0;
     p->reunit_every = 0;
+    p->origin_stride = 0;
 }
 
 extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
@@ -103,6 +104,7 @@ extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
         else if (!std::strcmp(para, "adjoint_store")) std::sscanf(value, "%d", &p->adj_store);
         else if (!std::strcmp(para, "triclinic")) std::sscanf(value, "%d", &p->triclinic);
         else if (!std::strcmp(para, "reunitarise_every")) std::sscanf(value, "%d", &p->reunit_every);
+        else if (!std::strcmp(para, "origin_stride_frames")) std::sscanf(value, "%d", &p->origin_stride);
         else if (!std::strcmp(para, "precision")) {
             if (!std::strcmp(value, "fp64")) p->precision = CIR_PREC_FP64;
             else if (!std::strcmp(value, "bf16x3")) p->precision = CIR_PREC_BF16X3;
@@ -145,6 +147,7 @@ extern "C" void cir_params_echo(const cir_params* p) {
     if (p->ckpt_every > 0 || p->resume != 0 || p->max_frames > 0)
         std::printf("\tCheckpoint %s_ckpt.bin: every %d frames, resume %d, max_frames %d\n", p->cptf, p->ckpt_every,
                     p->resume, p->max_frames);
+    if (p->origin_stride > 0) std::printf("\tTime origins every %d frames\n", p->origin_stride);
 }
 
 extern "C" int cir_resolve_model(const char* model, int* imodel) {
@@ -178,8 +181,10 @@ extern "C" float cir_minimage(float dx, float boxl) {
 }
 
 extern "C" int cir_frame_index(const cir_params* p, int sample, int point, float frame_dt, float* desired_time) {
-    // calcIR.cu:378-379, float arithmetic as in the reference
-    const float t = float(sample) * p->sampleEvery + p->beginTime + p->dt * float(point);
+    // calcIR.cu:378-379, float arithmetic as in the reference; origin_stride
+    // gives sampleEvery in frames
+    const float every = (p->origin_stride > 0) ? float(p->origin_stride) * frame_dt : p->sampleEvery;
+    const float t = float(sample) * every + p->beginTime + p->dt * float(point);
     if (desired_time) *desired_time = t;
     return int(std::round(t / frame_dt));
 }

@@ -279,6 +279,7 @@ int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d
     f.val(p.cutoff);
     f.val(p.triclinic);
     f.val(p.reunit_every);
+    f.val(p.origin_stride);
     f.val(c->mc);
     f.val(S.nslot);
     f.val(int64_t(S.uniq.size()));
@@ -711,6 +712,7 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     if (p->nchrom_mol > 2) return fail(CIR_ERR_ARG, "nchrom_mol must be 1 or 2 (OH stretches of a water)");
     if (p->triclinic != 0 && p->triclinic != 1) return fail(CIR_ERR_ARG, "triclinic must be 0 or 1");
     if (p->reunit_every < 0) return fail(CIR_ERR_ARG, "reunitarise_every must be >= 0");
+    if (p->origin_stride < 0) return fail(CIR_ERR_ARG, "origin_stride_frames must be >= 0");
     if (p->ntcfpoints <= 0 || p->nsamples <= 0 || p->omegaStep <= 0 || p->nzeros < 0 || !(p->t1 > 0) || !(p->dt > 0))
         return fail(CIR_ERR_ARG, "ntcfpoints, nsamples, omegaStep, t1 and dt must be positive, nzeros >= 0");
     if (cir_nomega(p) <= 0) return fail(CIR_ERR_ARG, "omegaStop must not be below omegaStart");

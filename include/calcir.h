@@ -129,6 +129,9 @@ typedef struct cir_params {
                                   its nine propagated columns, Y^H Y, to its value at the
                                   origin (F unitary keeps it there; rounding lets it drift);
                                   0 = never                                             */
+    int   origin_stride;       /* key "origin_stride_frames": > 0 starts sample s this many
+                                  trajectory frames after sample s-1 (sampleEvery =
+                                  origin_stride * frame spacing); 0 = sampleEvery as given */
 } cir_params;
 
 /* Results of a run, all arrays caller-allocated (host).  NULL members are

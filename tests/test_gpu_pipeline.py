@@ -445,3 +445,16 @@ def test_stream_guards():
             assert L.cir_stream_begin(ctx, len(times), times.ctypes.data) == 1, over
         finally:
             L.cir_destroy(ctx)
+
+
+def test_origin_stride_frames_equals_sample_every():
+    # origins every 10 frames of 10 fs select the same frames as sampleEvery
+    # 0.1 ps, so the run is the same
+    xyz, box, times = frames(64, 40, 12)
+    kw = dict(KW, nsamples=2, ntcfpoints=12)
+    announce("origin_stride_frames 10 vs sampleEvery 0.1")
+    a = C.run(C.default_params(precision="fp64", **kw), xyz, box, times)
+    b = C.run(C.default_params(precision="fp64", origin_stride_frames=10, **dict(kw, sampleEvery=7.0)), xyz, box,
+              times)
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+        assert rel(b[k], a[k]) < 1e-12, k

@@ -79,7 +79,8 @@ def test_parser_keys_comments_and_quirks(tmp_path):
         "max_frames      200\n"
         "adjoint_store   2\n"
         "triclinic       1\n"
-        "reunitarise_every 25\n")
+        "reunitarise_every 25\n"
+        "origin_stride_frames 20\n")
     p = C.read_params(str(f))
     assert p.xtcf == b"nvt.xtc" and p.outf == b"test" and p.model == b"tip4p" and p.species == b"HOD/D2O"
     assert p.ntcfpoints == 400 and p.nsamples == 2 and p.imap == 1
@@ -89,6 +90,12 @@ def test_parser_keys_comments_and_quirks(tmp_path):
     assert (p.ckpt_every, p.resume, p.max_frames) == (40, 1, 200)
     assert (p.adj_store, p.triclinic, p.reunit_every) == (2, 1, 25)
     assert C.default_params(reunitarise_every=7).reunit_every == 7
+    assert p.origin_stride == 20 and C.default_params().origin_stride == 0
+    # origins every 20 frames of 0.01 ps: sample 3, point 5 is frame 65, as
+    # with sampleEvery = 0.2 ps
+    q = C.default_params(origin_stride_frames=20, sampleEvery=99.0, dt=0.01)
+    assert C.frame_index(q, 3, 5, 0.01)[0] == 65
+    assert C.frame_index(C.default_params(sampleEvery=0.2, dt=0.01), 3, 5, 0.01)[0] == 65
     assert (C.default_params().adj_store, C.default_params().triclinic) == (0, 0)
     assert (C.default_params().ckpt_every, C.default_params().resume, C.default_params().max_frames) == (0, 0, 0)
     assert C.default_params(checkpoint_every=5).ckpt_every == 5
