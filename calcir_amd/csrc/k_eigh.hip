@@ -1349,6 +1349,30 @@ larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __res
     }
 }
 
+// Y^T of a back-transform pass into a k-contiguous copy: Yt(m, k) =
+// Y(m, k) = A[(j0 + k) * ld + j0 + 1 + m] at Yt[m * kBtPass + k], m < mrows,
+// k < kb, through a padded 64 x 64 LDS tile (reads and writes both along
+// memory).  The update Z -= Y W then stages both operands k-contiguous.
+template <typename T>
+__global__ void __launch_bounds__(256)
+yt_kernel(int n, int ld, int j0, int kb, int mrows, const T* __restrict__ Aall, T* __restrict__ Ytall,
+          int64_t ystride) {
+    __shared__ T tile[64][65];
+    const T* A = Aall + size_t(blockIdx.z) * n * ld;
+    T* Yt = Ytall + size_t(blockIdx.z) * ystride;
+    const int m0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int kk = ty; kk < 64; kk += 4) {
+        const int m = m0 + tx, k = k0 + kk;
+        tile[kk][tx] = (m < mrows && k < kb) ? A[size_t(j0 + k) * ld + (j0 + 1 + m)] : T(0);
+    }
+    __syncthreads();
+    for (int mm = ty; mm < 64; mm += 4) {
+        const int m = m0 + mm, k = k0 + tx;
+        if (m < mrows && k < kb) Yt[size_t(m) * kBtPass + k] = tile[tx][mm];
+    }
+}
+
 inline size_t larft_smem_bytes(size_t esz) { return (size_t(kBtNB) * (kBtNB + 1) + kBtNB) * esz; }
 
 struct EighWork {
@@ -1503,13 +1527,21 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
         }
         // W1 = Y^T Z (kb x n)
         CIR_TRY(mm(kb, n, mrows, Y, ld, 1, mat, Zs, ld, 1, mat, W1, n, 1, pn, 1.0, 0.0));
-        // W2 = T W1 by block rows (T21 = 0 is never formed)
-        CIR_TRY(mm(kb1, n, kb, Tm, P, 1, pp, W1, n, 1, pn, W2, n, 1, pn, 1.0, 0.0));
+        // W2 = T W1, stored transposed (W2(k, c) at W2[c * P + k]) so that the
+        // update reads it k-contiguous: formed as W2^T = W1^T T^T by block
+        // columns of T^T (T21 = 0 is never formed), which keeps the stores
+        // unit-stride
+        CIR_TRY(mm(n, kb1, kb, W1, 1, n, pn, Tm, 1, P, pp, W2, P, 1, pn, 1.0, 0.0));
         if (kb2 > 0)
-            CIR_TRY(mm(kb2, n, kb2, Tm + int64_t(kb1) * P + kb1, P, 1, pp, W1 + int64_t(kb1) * n, n, 1, pn,
-                       W2 + int64_t(kb1) * n, n, 1, pn, 1.0, 0.0));
-        // Z -= Y W2
-        CIR_TRY(mm(mrows, n, kb, Y, 1, ld, mat, W2, n, 1, pn, Zs, ld, 1, mat, -1.0, 1.0));
+            CIR_TRY(mm(n, kb2, kb2, W1 + int64_t(kb1) * n, 1, n, pn, Tm + int64_t(kb1) * P + kb1, 1, P, pp,
+                       W2 + kb1, P, 1, pn, 1.0, 0.0));
+        // Z -= Y W2, with Y copied k-contiguous into W1 (free once W2 is formed)
+        {
+            dim3 grid(unsigned(ceil_div(mrows, 64)), unsigned(ceil_div(kb, 64)), unsigned(batch));
+            hipLaunchKernelGGL(yt_kernel<T>, grid, dim3(256), 0, s, n, ld, j0, kb, mrows, A, W1, pn);
+            CIR_CHECK_LAUNCH();
+        }
+        CIR_TRY(mm(mrows, n, kb, W1, P, 1, pn, W2, 1, P, pn, Zs, ld, 1, mat, -1.0, 1.0));
     }
     return CIR_OK;
 }

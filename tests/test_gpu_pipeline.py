@@ -211,8 +211,10 @@ def test_checkpoint_halt_and_resume_reproduce_the_uninterrupted_run(tmp_path):
         assert g["nframes_used"] == ref["nframes_used"] == 50 and g["nsteps"] == ref["nsteps"]
         for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
             assert rel(g[k], ref[k]) < tol, (prec, k, rel(g[k], ref[k]))
+        # FP32 runs differ by the order of their float atomics; the lineshapes
+        # (sums with cancellation) move by ~1e-5 between runs
         for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
-            assert rel_l2(g[k], ref[k]) < tol, (prec, k)
+            assert rel_l2(g[k], ref[k]) < (tol if prec == "fp64" else 1e-4), (prec, k)
         assert rel(g["sw"], ref["sw"]) < tol
         # Pw sums weights with device atomics (order-dependent in the last bits)
         assert np.abs(g["pw"] - ref["pw"]).max() <= (1e-12 if prec == "fp64" else 1e-3) * ref["pw"].max()
@@ -363,8 +365,11 @@ def test_streamed_run_equals_cir_run(prec, tol, chunk, sizes):
     assert g["nframes_used"] == ref["nframes_used"] == 50 and g["nsteps"] == ref["nsteps"]
     for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
         assert rel(g[k], ref[k]) < tol, (k, rel(g[k], ref[k]))
+    # two runs of an FP32-class job differ by the order of their float atomics;
+    # the lineshapes (sums with cancellation) move by ~1e-5 between runs
+    tol_spec = tol if prec == "fp64" else 1e-4
     for k in ("ftcf_ir", "ftcf_vv", "ftcf_vh"):
-        assert rel_l2(g[k], ref[k]) < tol, k
+        assert rel_l2(g[k], ref[k]) < tol_spec, (k, rel_l2(g[k], ref[k]))
     assert rel(g["sw"], ref["sw"]) < tol
     assert np.abs(g["pw"] - ref["pw"]).max() <= (1e-12 if prec == "fp64" else 1e-3) * ref["pw"].max()
 
