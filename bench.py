@@ -295,6 +295,7 @@ def main():
                      "avg_launch_ms": 1e3 * per_launch, "matrices_per_launch": nmat / max(tri_n, 1),
                      "bytes_per_matrix": tridiag_bytes(n)},
         "phases_s_per_run": {k: v[0] / args.steps for k, v in phases.items()},
+        "trajectory_frames_per_s": nmat / elapsed,
     }
     if not args.no_extras:
         print("[bench] fp32 batched", file=sys.stderr, flush=True)

@@ -3,7 +3,7 @@
 the CPU oracle's FP64 pipeline at overridden coefficients, against the GPU
 FP64 forward at a larger size, and FP32 against FP64.
 
-The loss is the c4 one: the L2 distance between the IR lineshape rows and a
+The loss is the L2 distance between the IR lineshape rows and a
 target lineshape made at seeded +-2% perturbed coefficients.  Gradients are
 compared as relative sensitivities coef_c * dL/dcoef_c, which share one
 scale across coefficients of very different sizes."""
@@ -90,7 +90,7 @@ def test_adjoint_matches_oracle_finite_differences(species, imap):
 
 def test_forward_ls_equals_run_and_gpu_fd_at_n512():
     # N = 512: finite differences of the GPU FP64 forward itself on the map
-    # coefficients the c4 loss names (omega, mu', x10)
+    # coefficients of omega, mu' and x10
     xyz, box, times = frames(256, 30, 99)
     kw = dict(KW, nsamples=2, ntcfpoints=10)
     p = C.default_params(precision="fp64", species="H2O", **kw)
@@ -190,7 +190,7 @@ def test_recomputed_eigensystems_give_the_stored_gradient():
     assert np.abs(s1 - s2).max() <= 1e-10 * np.abs(s1).max()
 
 
-def test_c4_differentiation_config():
+def test_lineshape_gradient_1024oh_workload():
     # 512 TIP4P/2005 waters (1024 OH), 400 frames, origins every 10 frames
     # (26 samples of 150 points); loss: L2 between the IR lineshape and one
     # made at seeded +-2% perturbed coefficients.  Adjoint (FP64) against
@@ -204,10 +204,10 @@ def test_c4_differentiation_config():
     pert = coef * (1 + 0.02 * np.random.default_rng(1004).choice([-1.0, 1.0], coef.size))
     p64 = C.default_params(precision="fp64", **kw)
     idx, _, scale = lineshape_map(p64)
-    announce("c4: target at perturbed coefficients")
+    announce("1024 OH: target at perturbed coefficients")
     sess = LineshapeSession(p64, xyz, box, times)
     target = sess.forward(pert)[idx] * scale
-    announce("c4: fp64 forward/backward")
+    announce("1024 OH: fp64 forward/backward")
     ft = sess.forward(coef)
     L, dL = _loss_grad(p64, ft, target)
     g64 = sess.backward(dL)
@@ -218,13 +218,13 @@ def test_c4_differentiation_config():
         for sgn in (1.0, -1.0):
             cf = coef.copy()
             cf[c] += sgn * d
-            announce("c4: fd %s %+d" % (name, sgn))
+            announce("1024 OH: fd %s %+d" % (name, sgn))
             r = sess.forward(cf)[idx] * scale - target
             vals.append(float(r @ r))
         fd = (vals[0] - vals[1]) / (2 * d)
         assert abs(g64[c] - fd) <= 1e-4 * abs(fd), (name, g64[c], fd)
     sess.close()
-    announce("c4: fp32 forward/backward")
+    announce("1024 OH: fp32 forward/backward")
     p32 = C.default_params(precision="fp32", **kw)
     sess = LineshapeSession(p32, xyz, box, times)
     ft32 = sess.forward(coef)

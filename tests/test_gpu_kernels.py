@@ -132,7 +132,7 @@ def test_eigh_small_cases(dtype):
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_eigh_hamiltonians_vs_oracle(dtype):
     xyz, box, _ = frames(128, 4, 1001)
-    announce("eigh of 4 c1 Hamiltonians dtype=%s" % dtype)
+    announce("eigh of 4 256-OH Hamiltonians dtype=%s" % dtype)
     x_d, b_d = torch.from_numpy(xyz.copy()).to(DEV), torch.from_numpy(box.copy()).to(DEV)
     sg = S.eproj_sites(x_d, b_d, dtype=dtype)
     H = S.hamiltonian(sg, b_d, avef=3400.0)
