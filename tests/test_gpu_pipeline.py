@@ -179,7 +179,9 @@ def test_cli_reads_an_xtc_trajectory(tmp_path):
         a = np.loadtxt(tmp_path / ("cli_%s.dat" % suffix))
         b = np.loadtxt(tmp_path / ("mem_%s.dat" % suffix))
         assert a.shape == b.shape, suffix
-        np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6 * np.abs(b).max(), err_msg=suffix)
+        # both runs are FP32 with float atomics (sum order varies run to run)
+        # and the files hold 6 significant digits
+        np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-5 * np.abs(b).max(), err_msg=suffix)
 
 
 def test_checkpoint_halt_and_resume_reproduce_the_uninterrupted_run(tmp_path):
@@ -188,7 +190,7 @@ def test_checkpoint_halt_and_resume_reproduce_the_uninterrupted_run(tmp_path):
     # uninterrupted result; periodic checkpoints leave the result unchanged.
     xyz, box, times = frames(128, 60, 1001)
     ck = str(tmp_path / "job")
-    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+    for prec, tol in (("fp64", 1e-12), ("fp32", 5e-5)):
         announce("checkpoint prec=%s uninterrupted" % prec)
         ref = C.run(C.default_params(precision=prec, mode="batched", chunk=8, **KW), xyz, box, times)
         announce("checkpoint prec=%s every 8 frames" % prec)
@@ -299,7 +301,7 @@ def test_run_many_uneven_boxes_equal_single_runs():
     for (p, xyz, box, times), g, (nw, prec) in zip(jobs, outs, sizes):
         announce("run_many single check nwater=%d %s" % (nw, prec))
         ref = C.run(p, xyz, box, times)
-        tol = 1e-12 if prec == "fp64" else 1e-5
+        tol = 1e-12 if prec == "fp64" else 5e-5  # FP32-class: float atomics reorder sums
         for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
             assert rel(g[k], ref[k]) < tol, (nw, prec, k, rel(g[k], ref[k]))
         assert g["nsteps"] == ref["nsteps"] == 16
@@ -352,7 +354,7 @@ def _blocks(xyz, box, sizes):
 @pytest.mark.parametrize("prec,tol,chunk,sizes", [
     ("fp64", 1e-12, 8, (1, 7, 13, 1, 2)),  # ragged blocks across chunk ends
     ("fp64", 1e-12, 0, ()),                 # the whole trajectory in one push
-    ("bf16x3", 1e-5, 16, (1,) * 60),        # frame by frame
+    ("bf16x3", 5e-5, 16, (1,) * 60),        # frame by frame
 ])
 def test_streamed_run_equals_cir_run(prec, tol, chunk, sizes):
     # cir_stream_begin / _push / _end give cir_run's results: the same
