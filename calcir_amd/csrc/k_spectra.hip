@@ -496,6 +496,19 @@ __global__ void __launch_bounds__(256) zero_rows_kernel(int n, int w2, int ldc, 
     C[size_t(e / w2) * ldc + size_t(e % w2)] = 0.f;
 }
 
+// Both split-K outputs zeroed in one launch: rows 0..n-1 of C0 (pitch ld0)
+// and of C1 (pitch ld1), w2 columns each.
+__global__ void __launch_bounds__(256) zero_rows2_kernel(int n, int w2, int ld0, float* __restrict__ C0, int ld1,
+                                                         float* __restrict__ C1) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x, ne = int64_t(n) * w2;
+    if (e >= 2 * ne) return;
+    const int64_t q = (e < ne) ? e : e - ne;
+    if (e < ne)
+        C0[size_t(q / w2) * ld0 + size_t(q % w2)] = 0.f;
+    else
+        C1[size_t(q / w2) * ld1 + size_t(q % w2)] = 0.f;
+}
+
 // K split of the propagation products, picked per (n, ncols, product form)
 // by timing the candidates once per process (the first call of a shape whose
 // output does not alias its input); the default guess aims at about 4
@@ -574,7 +587,17 @@ static int prop_pair(bool split3, int n, int ld, int ncols, int ldy, double dt, 
         CIR_CHECK_LAUNCH();
         return CIR_OK;
     };
-    CIR_TRY(zero(G, w2));
+    // distinct input and output: both accumulators are cleared up front in
+    // one launch; in place, the output is cleared after the first product
+    // has read it
+    const bool inplace = Yin == Yout;
+    if (ks > 1 && !inplace) {
+        hipLaunchKernelGGL(zero_rows2_kernel, dim3(unsigned(ceil_div(2 * int64_t(n) * w2, 256))), dim3(256), 0, s, n,
+                           w2, w2, G, ldy, Yout);
+        CIR_CHECK_LAUNCH();
+    } else {
+        CIR_TRY(zero(G, w2));
+    }
     if (split3)
         hipLaunchKernelGGL((prop_mfma_kernel<true, true, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
                            Yin, ldy, G, w2, dt, w, kseg, vecA);
@@ -582,7 +605,7 @@ static int prop_pair(bool split3, int n, int ld, int ncols, int ldy, double dt, 
         hipLaunchKernelGGL((prop_mfma_kernel<true, true, false>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
                            Yin, ldy, G, w2, dt, w, kseg, vecA);
     CIR_CHECK_LAUNCH();
-    CIR_TRY(zero(Yout, ldy));
+    if (inplace) CIR_TRY(zero(Yout, ldy));
     if (split3)
         hipLaunchKernelGGL((prop_mfma_kernel<false, false, true>), grid, dim3(kPrNT), 0, s, n, ncols, nmt_pad, ld, V,
                            G, w2, Yout, ldy, dt, w, kseg, vecA);
