@@ -1029,14 +1029,31 @@ int tridiag_single(int n, int ld, T* A, T* d, T* e, T* tau, T* P, T* buf, hipStr
     return CIR_OK;
 }
 
-// Number of eigenvalues of T below x (Sturm sequence count).
+// 1/q from the hardware reciprocal (about 24 bits) refined by Newton-Raphson
+// steps r <- r + r (1 - q r): one step gives about 48 bits (measured max
+// relative error 2.2e-15), two give full double precision.  The Sturm count
+// below needs only the signs of its recurrence: a relative error of a few
+// ulps in e^2/q is a perturbation of the off-diagonal of that size, far below
+// the bisection tolerance of an FP32-derived T (1024 eps ||T||); an FP64 T
+// (tolerance 4 eps ||T||) takes the second step.
+template <bool FULL>
+__device__ __forceinline__ double nr_rcp(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(r, fma(-q, r, 1.0), r);
+    if constexpr (FULL) r = fma(r, fma(-q, r, 1.0), r);
+    return r;
+}
+
+// Number of eigenvalues of T below x (Sturm sequence count).  |q| >= pivmin
+// keeps every reciprocal finite.
+template <bool FULL>
 __device__ __forceinline__ int sturm_count(int n, const double* sd, const double* se2, double x,
                                            double pivmin) {
     double q = sd[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     int c = q < 0.0;
     for (int k = 1; k < n; ++k) {
-        q = (sd[k] - x) - se2[k - 1] / q;
+        q = (sd[k] - x) - se2[k - 1] * nr_rcp<FULL>(q);
         if (fabs(q) < pivmin) q = -pivmin;
         c += q < 0.0;
     }
@@ -1092,7 +1109,7 @@ bisect_kernel(int n, const T* __restrict__ dall, const T* __restrict__ eall, dou
     // every eigenvalue a starting bracket kEigNT + 1 times narrower
     int* scnt = reinterpret_cast<int*>(sr + 12);
     const double step = (gu - gl) / double(kEigNT + 1);
-    scnt[tid] = sturm_count(n, sd, se2, gl + step * double(tid + 1), pivmin);
+    scnt[tid] = sturm_count<sizeof(T) == 8>(n, sd, se2, gl + step * double(tid + 1), pivmin);
     __syncthreads();
     const int j = blockIdx.x * kEigNT + tid;
     if (j >= n) return;
@@ -1108,7 +1125,7 @@ bisect_kernel(int n, const T* __restrict__ dall, const T* __restrict__ eall, dou
     const double abstol = (sizeof(T) == 4 ? 1024.0 : 4.0) * DBL_EPSILON * tnorm;
     for (int it = 0; it < 128 && (c - a) > fmax(2.0 * DBL_EPSILON * (fabs(a) + fabs(c)), abstol); ++it) {
         const double mid = 0.5 * (a + c);
-        if (sturm_count(n, sd, se2, mid, pivmin) > j)
+        if (sturm_count<sizeof(T) == 8>(n, sd, se2, mid, pivmin) > j)
             c = mid;
         else
             a = mid;
@@ -1198,8 +1215,11 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
             double u0, l, yo;
             unsigned char fl;
             if (fabs(a) >= fabs(sub)) {
-                if (a == 0.0) a = pivmin;
-                l = sub / a;
+                // a pivot below pivmin (0 included) is taken as +-pivmin, as
+                // the back substitution's reciprocal takes it; the reciprocal
+                // then stays finite and |l| <= 1
+                if (fabs(a) < pivmin) a = (a < 0.0) ? -pivmin : pivmin;
+                l = sub * nr_rcp<sizeof(T) == 8>(a);
                 u0 = a;
                 yo = yi;
                 a = dn - l * bs;
@@ -1207,7 +1227,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                 yi = yn - l * yi;
                 fl = 0;
             } else {
-                l = a / sub;
+                l = a * nr_rcp<sizeof(T) == 8>(sub);
                 u0 = sub;
                 yo = yn;
                 a = bs - l * dn;
@@ -1216,7 +1236,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                 fl = 1;
             }
             if (fabs(u0) < pivmin) u0 = (u0 < 0.0) ? -pivmin : pivmin;
-            R[size_t(i) * n] = 1.0 / u0;  // reciprocal pivot: the back substitution multiplies
+            R[size_t(i) * n] = nr_rcp<sizeof(T) == 8>(u0);  // reciprocal pivot: the back substitution multiplies
             Lm[size_t(i) * n] = l;
             F[size_t(i) * n] = fl;
             X[size_t(i) * n] = yo;
