@@ -108,6 +108,8 @@ SIGNATURES = {
     "cir_xtc_open": (_i, [ctypes.c_char_p, _i, ctypes.POINTER(_vp)]),
     "cir_xtc_next": (_i, [_vp, _i, _vp, _vp, _vp]),
     "cir_xtc_close": (None, [_vp]),
+    "cir_xtc_offsets": (_i, [ctypes.c_char_p, _i, _vp]),
+    "cir_xtc_seek": (_i, [_vp, _i]),
     "cir_xtc_write_full": (_i, [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _f, _i]),
     "cir_main": (_i, [_i, ctypes.POINTER(ctypes.c_char_p)]),
     "cir_device_count": (_i, [ctypes.POINTER(_i)]),
@@ -281,16 +283,26 @@ def xtc_times(path):
     return times, box
 
 
-def xtc_blocks(path, block):
-    """Consecutive frames of an XTC file read sequentially (cir_xtc_open /
-    cir_xtc_next), yielded as (xyz [k, natoms, 3], box matrices [k, 3, 3],
-    times [k]) with k <= block."""
+def xtc_offsets(path):
+    """Byte offset of every frame of an XTC file (cir_xtc_offsets)."""
+    _, nframes = xtc_scan(path)
+    off = np.zeros(nframes, np.int64)
+    check(lib().cir_xtc_offsets(os.fsencode(path), nframes, off.ctypes.data))
+    return off
+
+
+def xtc_blocks(path, block, first=0):
+    """Consecutive frames of an XTC file from frame `first` (cir_xtc_open,
+    cir_xtc_seek, cir_xtc_next), yielded as (xyz [k, natoms, 3], box
+    matrices [k, 3, 3], times [k]) with k <= block."""
     natoms, nframes = xtc_scan(path)
     L = lib()
     rd = ctypes.c_void_p()
     check(L.cir_xtc_open(os.fsencode(path), natoms, ctypes.byref(rd)))
     try:
-        for f0 in range(0, nframes, block):
+        if first:
+            check(L.cir_xtc_seek(rd, int(first)))
+        for f0 in range(int(first), nframes, block):
             k = min(block, nframes - f0)
             xyz = np.zeros((k, natoms, 3), np.float32)
             box = np.zeros((k, 3, 3), np.float32)

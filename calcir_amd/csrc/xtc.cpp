@@ -488,11 +488,57 @@ using namespace cir;
 struct cir_xtc {
     FILE* f = nullptr;
     int natoms = 0;
-    int frame = 0;  // index of the next frame
+    int frame = 0;                // index of the next frame
+    std::vector<int64_t> offset;  // byte offset of every frame, built on the first seek
     ~cir_xtc() {
         if (f) std::fclose(f);
     }
 };
+
+namespace {
+// Byte offsets of the frames of an open file, from the frame headers and the
+// byte counts of the coordinate blocks (nothing is decoded); leaves the file
+// position undefined.
+int frame_offsets(FILE* f, std::vector<int64_t>& off) {
+    off.clear();
+    if (std::fseek(f, 0, SEEK_SET) != 0) return fail(CIR_ERR_IO, "xtc: seek failed");
+    XdrIn x{f};
+    for (;;) {
+        const int64_t at = int64_t(std::ftell(f));
+        int nat, step;
+        float t, bx[9];
+        if (!read_header(x, &nat, &step, &t, bx)) break;
+        if (!skip_coords(x, nat)) return fail(CIR_ERR_IO, "xtc: corrupt frame " + std::to_string(off.size()));
+        off.push_back(at);
+    }
+    std::clearerr(f);
+    return CIR_OK;
+}
+}  // namespace
+
+extern "C" int cir_xtc_offsets(const char* path, int nframes, int64_t* offsets) {
+    if (!path || nframes < 0 || (nframes > 0 && !offsets)) return fail(CIR_ERR_ARG, "cir_xtc_offsets: bad argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
+    std::vector<int64_t> off;
+    const int st = frame_offsets(f, off);
+    std::fclose(f);
+    if (st != CIR_OK) return st;
+    if (int64_t(off.size()) < nframes) return fail(CIR_ERR_FRAME, "xtc: frame " + std::to_string(off.size()) + " missing");
+    for (int k = 0; k < nframes; ++k) offsets[k] = off[k];
+    return CIR_OK;
+}
+
+extern "C" int cir_xtc_seek(cir_xtc* r, int frame) {
+    if (!r || frame < 0) return fail(CIR_ERR_ARG, "cir_xtc_seek: bad argument");
+    if (r->offset.empty()) CIR_TRY(frame_offsets(r->f, r->offset));
+    if (frame >= int(r->offset.size()))
+        return fail(CIR_ERR_FRAME, "xtc: frame " + std::to_string(frame) + " missing (" +
+                                       std::to_string(r->offset.size()) + " frames)");
+    if (std::fseek(r->f, long(r->offset[frame]), SEEK_SET) != 0) return fail(CIR_ERR_IO, "xtc: seek failed");
+    r->frame = frame;
+    return CIR_OK;
+}
 
 extern "C" int cir_xtc_open(const char* path, int natoms, cir_xtc** out) {
     if (!path || !out || natoms < 0) return fail(CIR_ERR_ARG, "cir_xtc_open: bad argument");
@@ -593,7 +639,6 @@ int read_frames(const char* path, int first, int count, int natoms, float* xyz, 
     FILE* f = std::fopen(path, "rb");
     if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
     XdrIn x{f};
-    std::vector<float> skip(3 * size_t(natoms));
     int got = 0;
     for (int fr = 0; got < count; ++fr) {
         int nat, step;
@@ -606,8 +651,9 @@ int read_frames(const char* path, int first, int count, int natoms, float* xyz, 
             std::fclose(f);
             return fail(CIR_ERR_IO, "xtc: atom count differs from the expected " + std::to_string(natoms));
         }
-        float* dst = (fr >= first) ? xyz + size_t(got) * 3 * natoms : skip.data();
-        if (!read_coords(x, nat, dst)) {
+        // frames before the range are skipped by their byte counts
+        const bool ok = (fr >= first) ? read_coords(x, nat, xyz + size_t(got) * 3 * natoms) : skip_coords(x, nat);
+        if (!ok) {
             std::fclose(f);
             return fail(CIR_ERR_IO, "xtc: corrupt coordinates in frame " + std::to_string(fr));
         }

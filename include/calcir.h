@@ -36,7 +36,7 @@
  *   output writers                     calcIR.cu:843-919 -> cir_write_outputs
  *   minImage/mag3/dot3                 calcIR.cu:1457-1476, calcIR.h:79-88 -> cir_minimage
  *   xdrfile read_xtc* / xdr_seek       calcIR.cu:212-219, 341-395 -> cir_xtc_scan/_read/_times,
- *                                      cir_xtc_open/_next/_close (+ _write)
+ *                                      cir_xtc_open/_next/_seek/_close, cir_xtc_offsets (+ _write)
  */
 #ifndef CALCIR_H
 #define CALCIR_H
@@ -221,6 +221,14 @@ typedef struct cir_xtc cir_xtc;
 int   cir_xtc_open(const char* path, int natoms, cir_xtc** out);
 int   cir_xtc_next(cir_xtc* reader, int count, float* xyz, float* box9, float* times);
 void  cir_xtc_close(cir_xtc* reader);
+/* Random access, the role of the xtc_seek extension's frame offsets and
+ * xdr_seek (calcIR.cu:380): cir_xtc_offsets gives the byte offset of each of
+ * the first nframes frames (from the headers and the coordinate blocks' byte
+ * counts, nothing decoded); cir_xtc_seek positions a reader at a frame (the
+ * index is built on its first call), after which cir_xtc_next reads from
+ * there; CIR_ERR_FRAME past the last frame. */
+int   cir_xtc_offsets(const char* path, int nframes, int64_t* offsets);
+int   cir_xtc_seek(cir_xtc* reader, int frame);
 /* The drop-in CLI: argv as calcIR.exe (calcIR.cu:9-18). */
 int   cir_main(int argc, char** argv);
 

@@ -147,3 +147,34 @@ def test_header_pass_and_sequential_blocks(tmp_path, nwater):
         assert L.cir_xtc_next(rd, 24, buf.ctypes.data, None, None) == 6
     finally:
         L.cir_xtc_close(rd)
+
+
+def test_frame_offsets_and_seek(tmp_path):
+    # random access as the reference's xtc_seek + xdr_seek: the offset index
+    # comes from headers and byte counts; a seek then reads what cir_xtc_read
+    # reads from that frame
+    xyz, box, times = C.synth_trajectory(64, 17, 23)
+    f = tmp_path / "s.xtc"
+    C.xtc_write(f, xyz, box, times)
+    off = C.xtc_offsets(f)
+    assert off[0] == 0 and np.all(np.diff(off) > 0) and off[-1] < os.path.getsize(f)
+    xr, br, tr, _ = C.xtc_read(f, full_box=True)
+    for first in (11, 3, 0, 16):
+        got = list(C.xtc_blocks(f, 4, first=first))
+        assert np.array_equal(np.concatenate([g[0] for g in got]), xr[first:])
+        assert np.array_equal(np.concatenate([g[2] for g in got]), tr[first:])
+    # a range read skips the earlier frames without decoding them
+    part, _, t, _ = C.xtc_read(f, first=9, count=5)
+    assert np.array_equal(part, xr[9:14]) and np.array_equal(t, tr[9:14])
+    import ctypes
+    L = C.lib()
+    rd = ctypes.c_void_p()
+    C.check(L.cir_xtc_open(os.fsencode(f), xyz.shape[1], ctypes.byref(rd)))
+    try:
+        assert L.cir_xtc_seek(rd, 17) == 6
+        C.check(L.cir_xtc_seek(rd, 5))  # backwards after forwards
+        buf = np.zeros((2,) + xyz.shape[1:], np.float32)
+        C.check(L.cir_xtc_next(rd, 2, buf.ctypes.data, None, None))
+        assert np.array_equal(buf, xr[5:7])
+    finally:
+        L.cir_xtc_close(rd)
