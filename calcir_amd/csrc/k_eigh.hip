@@ -151,6 +151,29 @@ __device__ T block_sum1(T v, T* sred) {
 // LDS stride of a transposed 64-row panel tile (padded against bank conflicts)
 constexpr int kTileLd = 65;
 
+// Positions of V(:, l) and W(:, l) in a row of the workgroup-per-matrix
+// kernel's panel.  FP32 interleaves 16-column halves, [V0..15 | W0..15 |
+// V16..31 | W16..31], so that the first 16 reflectors of a panel use only
+// the first 128 bytes of each row (the fused pass then reads half the row);
+// FP64 keeps [V | W].  pswap maps a V position to the W position of the
+// same column and back; pcol gives the column of a position.
+template <typename T>
+__device__ __forceinline__ constexpr int posV(int l) {
+    return sizeof(T) == 4 ? (l & 15) + 32 * (l >> 4) : l;
+}
+template <typename T>
+__device__ __forceinline__ constexpr int posW(int l) {
+    return sizeof(T) == 4 ? (l & 15) + 32 * (l >> 4) + 16 : l + kTriNB;
+}
+template <typename T>
+__device__ __forceinline__ constexpr int pswap(int p) {
+    return sizeof(T) == 4 ? p ^ 16 : (p + kTriNB) & (2 * kTriNB - 1);
+}
+template <typename T>
+__device__ __forceinline__ constexpr int pcol(int p) {
+    return sizeof(T) == 4 ? (p & 15) + 16 * (p >> 5) : p & (kTriNB - 1);
+}
+
 inline size_t tridiag_smem_elems(int n) {
     const size_t uni = std::max<size_t>(2 * size_t(n), size_t(2) * 2 * kTriNB * kTileLd);
     return uni + size_t(kTriNW) + 2 * kTriNB + 2 * kTriNB + size_t(kTriNW) * 2 * kTriNB + size_t(kTriNW);
@@ -163,9 +186,11 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // 32-row blocks br = br0, br0 + brstep, ... and every block column bc <= br.  The
 // block product is formed transposed (MFMA row index = column of A, MFMA column
 // index = row of A) so that lanes run along rows of the column-major A.  Step s
-// of the K = 64 sum takes k = 32 h + s from lane half h: the b operand is the
-// panel row (V(r, .), W(r, .)) and the a operand the panel row of the column
-// with its halves swapped, (W(c, .), V(c, .)).
+// of the K = 64 sum takes panel position k = 32 h + s from lane half h: the b
+// operand is the panel row of row r at k and the a operand the panel row of
+// column c at the W position of the same column (or the V position of a W
+// position): k ^ 16 in the interleaved panel (ILV), k ^ 32 in [V | W].
+template <bool ILV>
 __device__ void trailing_update_f32(int n, int ld, int jn, float* __restrict__ A, const float* __restrict__ P,
                                     int lane, int br0, int brstep) {
     constexpr int PW = 2 * kTriNB;
@@ -188,10 +213,11 @@ __device__ void trailing_update_f32(int n, int ld, int jn, float* __restrict__ A
         // a operand of block column bc: panel row of column cb + l32, halves swapped
         auto load_a = [&](float(&av)[32], int bc) {
             const int cl = jn + bc * 32 + l32;
-            const float4* src = reinterpret_cast<const float4*>(P + size_t(cl < n ? cl : 0) * PW + (1 - h) * 32);
+            const float4* src =
+                reinterpret_cast<const float4*>(P + size_t(cl < n ? cl : 0) * PW + (ILV ? h : 1 - h) * 32);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const float4 t = (cl < n) ? src[q] : float4{};
+                const float4 t = (cl < n) ? src[ILV ? (q ^ 4) : q] : float4{};  // ILV: 16-float halves swapped
                 av[4 * q] = t.x;
                 av[4 * q + 1] = t.y;
                 av[4 * q + 2] = t.z;
@@ -255,8 +281,8 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
     T* const sL = uni;                   // trailing update: row tile [PW][kTileLd]
     T* const sR = uni + PW * kTileLd;    //                  column tile [PW][kTileLd]
     T* const sred = uni + unisz;         // kTriNW
-    T* const srow = sred + kTriNW;       // PW: panel row j (V then W)
-    T* const sdot = srow + PW;           // PW: y = V^T v then x = W^T v
+    T* const srow = sred + kTriNW;       // PW: panel row j
+    T* const sdot = srow + PW;           // PW: y = V^T v and x = W^T v at the V and W positions
     T* const sdp = sdot + PW;            // kTriNW * PW: per-wave partial dots
     T* const sred2 = sdp + kTriNW * PW;  // kTriNW: v^T A v partials
 
@@ -309,16 +335,16 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
             const T* xe = reinterpret_cast<const T*>(&xr);
             T part = T(0);
 #pragma unroll
-            for (int q = 0; q < VW; ++q) part += xe[q] * sdot[((g * VW + q) + kTriNB) & (PW - 1)];
+            for (int q = 0; q < VW; ++q) part += xe[q] * sdot[pswap<T>(g * VW + q)];
             const T corr = row_sum(part);
             const T wnew = tau_prev * (sp[r] - corr) + K_prev * sv[r];
-            const int cw = kTriNB + icol - g * VW;  // lane-local index of W(:, icol), if this lane holds it
+            const int cw = posW<T>(icol) - g * VW;  // lane-local index of W(:, icol), if this lane holds it
             if (cw >= 0 && cw < VW) {
                 T* xw = reinterpret_cast<T*>(&xr);
 #pragma unroll
                 for (int q = 0; q < VW; ++q)
                     if (q == cw) xw[q] = wnew;
-                P[size_t(r) * PW + kTriNB + icol] = wnew;
+                P[size_t(r) * PW + posW<T>(icol)] = wnew;
             }
         };
         for (int i = 0; i < nbp; ++i) {
@@ -334,11 +360,14 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
             {
                 T coefc[VW];
 #pragma unroll
-                for (int q = 0; q < VW; ++q) coefc[q] = srow[((g * VW + q) + kTriNB) & (PW - 1)];
+                for (int q = 0; q < VW; ++q) coefc[q] = srow[pswap<T>(g * VW + q)];
                 T rd[VW];
 #pragma unroll
                 for (int q = 0; q < VW; ++q) rd[q] = T(0);
                 T sig = T(0);
+                // up to 16 formed columns (FP32) sit in the first half of the
+                // row; the other half is still zero and is not read
+                const bool need = sizeof(T) != 4 || i > 16 || g * VW < 32;
                 constexpr int kU = 4;
                 for (int rb = j + wave * RPI; rb < n; rb += kU * kTriNW * RPI) {
                     VT xr[kU];
@@ -346,7 +375,7 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         const int r = rb + u * kTriNW * RPI + rsub;
-                        xr[u] = (r < n) ? *reinterpret_cast<const VT*>(P + size_t(r) * PW + g * VW) : VT{};
+                        xr[u] = (r < n && need) ? *reinterpret_cast<const VT*>(P + size_t(r) * PW + g * VW) : VT{};
                         av[u] = (r < n) ? A[size_t(j) * ld + r] : T(0);
                     }
 #pragma unroll
@@ -397,14 +426,14 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 T raw = T(0);
 #pragma unroll
                 for (int w = 0; w < kTriNW; ++w) raw += sdp[w * PW + tid];
-                const int l = tid & (kTriNB - 1);
+                const int l = pcol<T>(tid);
                 sdot[tid] = (l < i) ? scal * raw + P[size_t(j + 1) * PW + tid] : T(0);
             }
             for (int r = j + 1 + tid; r < n; r += kTriNT) {
                 const T vr = (r == j + 1) ? T(1) : sp[r] * scal;
                 sv[r] = vr;
                 A[size_t(j) * ld + r] = vr;
-                P[size_t(r) * PW + i] = vr;
+                P[size_t(r) * PW + posV<T>(i)] = vr;
             }
             for (int r = tid; r <= j; r += kTriNT) A[size_t(j) * ld + r] = T(0);
             if (tid == 0) {
@@ -547,7 +576,7 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
 #pragma unroll
                 for (int w = 0; w < kTriNW; ++w) vAv += sred2[w];
                 T xy = T(0);
-                for (int l = 0; l < kTriNB; ++l) xy += sdot[kTriNB + l] * sdot[l];
+                for (int l = 0; l < kTriNB; ++l) xy += sdot[posW<T>(l)] * sdot[posV<T>(l)];
                 Kj = T(-0.5) * tj * tj * (vAv - T(2) * xy);
             }
             tau_prev = tj;
@@ -570,7 +599,7 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
         const int jn = j0 + nbp;
         __syncthreads();
         if constexpr (sizeof(T) == 4) {
-            if (jn < n) trailing_update_f32(n, ld, jn, A, P, lane, wave, kTriNW);
+            if (jn < n) trailing_update_f32<true>(n, ld, jn, A, P, lane, wave, kTriNW);
         } else if (jn < n) {
             const int mt = n - jn;
             const int ntile = (mt + 63) / 64;
@@ -901,7 +930,7 @@ __global__ void __launch_bounds__(kS1NT)
 tri1_update_f32(int n, int ld, int jn, float* __restrict__ A, const float* __restrict__ P) {
     const int lane = threadIdx.x & 63;
     const int w = int(blockIdx.x) * (kS1NT / 64) + (int(threadIdx.x) >> 6);
-    trailing_update_f32(n, ld, jn, A, P, lane, w, int(gridDim.x) * (kS1NT / 64));
+    trailing_update_f32<false>(n, ld, jn, A, P, lane, w, int(gridDim.x) * (kS1NT / 64));
 }
 
 // FP64 trailing update: one 64 x 64 lower tile per workgroup (blockIdx.x =
@@ -1219,7 +1248,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                 // the back substitution's reciprocal takes it; the reciprocal
                 // then stays finite and |l| <= 1
                 if (fabs(a) < pivmin) a = (a < 0.0) ? -pivmin : pivmin;
-                l = sub * nr_rcp<sizeof(T) == 8>(a);
+                l = (sizeof(T) == 8) ? sub / a : sub * nr_rcp<false>(a);
                 u0 = a;
                 yo = yi;
                 a = dn - l * bs;
@@ -1227,7 +1256,7 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                 yi = yn - l * yi;
                 fl = 0;
             } else {
-                l = a * nr_rcp<sizeof(T) == 8>(sub);
+                l = (sizeof(T) == 8) ? a / sub : a * nr_rcp<false>(sub);
                 u0 = sub;
                 yo = yn;
                 a = bs - l * dn;
@@ -1236,7 +1265,9 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                 fl = 1;
             }
             if (fabs(u0) < pivmin) u0 = (u0 < 0.0) ? -pivmin : pivmin;
-            R[size_t(i) * n] = nr_rcp<sizeof(T) == 8>(u0);  // reciprocal pivot: the back substitution multiplies
+            // reciprocal pivot: the back substitution multiplies; an FP32-derived
+            // T takes the refined hardware reciprocal, an FP64 T the IEEE one
+            R[size_t(i) * n] = (sizeof(T) == 8) ? 1.0 / u0 : nr_rcp<false>(u0);
             Lm[size_t(i) * n] = l;
             F[size_t(i) * n] = fl;
             X[size_t(i) * n] = yo;
