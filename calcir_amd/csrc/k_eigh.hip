@@ -135,6 +135,65 @@ __device__ __forceinline__ void strip_sums16(const float (&dotc)[16], int lane, 
     if (i < 4 && col < nc) atomicAdd(&sp[c + col], v);
 }
 
+// FP64 forms of the above: DPP and the lane swaps move 32-bit halves, so a
+// double moves as its two halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double row_sum16(double x) {
+    x += dpp_f64<0x128>(x);
+    x += dpp_f64<0x124>(x);
+    x += dpp_f64<0x122>(x);
+    x += dpp_f64<0x121>(x);
+    return x;
+}
+// (X, Y) -> (X', Y') of v_permlane32_swap / v_permlane16_swap on doubles
+template <bool P32>
+__device__ __forceinline__ void swap_f64(double& x, double& y) {
+    const auto lo = P32 ? __builtin_amdgcn_permlane32_swap(unsigned(__double2loint(x)), unsigned(__double2loint(y)),
+                                                           false, false)
+                        : __builtin_amdgcn_permlane16_swap(unsigned(__double2loint(x)), unsigned(__double2loint(y)),
+                                                           false, false);
+    const auto hi = P32 ? __builtin_amdgcn_permlane32_swap(unsigned(__double2hiint(x)), unsigned(__double2hiint(y)),
+                                                           false, false)
+                        : __builtin_amdgcn_permlane16_swap(unsigned(__double2hiint(x)), unsigned(__double2hiint(y)),
+                                                           false, false);
+    x = __hiloint2double(int(hi[0]), int(lo[0]));
+    y = __hiloint2double(int(hi[1]), int(lo[1]));
+}
+// Sum over the 32 lanes of a half wave (lanes 32 H .. 32 H + 31), the same
+// bits in every lane: the 16-lane row sums, then the two rows of the half
+// added through v_permlane16_swap of the value with itself.
+__device__ __forceinline__ double half_sum32(double x) {
+    x = row_sum16(x);
+    double y = x;
+    swap_f64<false>(x, y);
+    return x + y;
+}
+// The 8 column sums of an FP64 strip: as strip_sums16 with one swap stage
+// fewer; lane 16 R + i (i < 2) adds column 2 R + i.
+__device__ __forceinline__ void strip_sums8(const double (&dotc)[8], int lane, int c, int nc, double* sp) {
+    double s1[4], s2[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double x = dotc[k], y = dotc[k + 4];
+        swap_f64<true>(x, y);
+        s1[k] = x + y;  // column k + 4 (lane >> 5)
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double x = s1[k], y = s1[k + 2];
+        swap_f64<false>(x, y);
+        s2[k] = row_sum16(x + y);  // column k + 2 (lane >> 4)
+    }
+    const int i = lane & 15, col = i + 2 * (lane >> 4);
+    const double v = i == 0 ? s2[0] : s2[1];
+    if (i < 2 && col < nc) atomicAdd(&sp[c + col], v);
+}
+
 template <typename T>
 __device__ T block_sum1(T v, T* sred) {
     v = wave_sum(v);
@@ -323,6 +382,8 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
         auto row_sum = [&](T x) {
             if constexpr (sizeof(T) == 4 && LPR == 16) {
                 return row_sum16(x);
+            } else if constexpr (sizeof(T) == 8 && LPR == 32) {
+                return half_sum32(x);
             } else {
 #pragma unroll
                 for (int off = 1; off < LPR; off <<= 1) x += __shfl_xor(x, off, 64);
@@ -534,6 +595,8 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                         if (k < nc) vav += sv[c + k] * dotc[k];
                     if constexpr (sizeof(T) == 4 && kStrip == 16) {
                         strip_sums16(dotc, lane, c, nc, sp);
+                    } else if constexpr (sizeof(T) == 8 && kStrip == 8) {
+                        strip_sums8(dotc, lane, c, nc, sp);
                     } else {
 #pragma unroll
                         for (int k = 0; k < kStrip; ++k) {
