@@ -269,7 +269,13 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x3_kernel(GemmArgs g) {
         }
 }
 
-// FP64: 64x64 tile, 256 threads, 4x4 outputs per thread.
+// FP64 on the FP64 MFMA (16x16x4): 64x64 tile, 256 threads = 4 waves in a
+// 2x2 arrangement, each wave 2x2 fragments of 16x16.  Tiles are staged k-major
+// in LDS (As[k][m], Bs[k][n]); fragment operand: lane l supplies row/column
+// l % 16 at k = l / 16 of a 4-deep k step; result element r of lane l is
+// row l / 16 + 4 r, column l % 16 of the fragment (measured on gfx950).
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
 __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
     constexpr int BM = 64, BN = 64, BK = 16;
     __shared__ double As[BK][BM + 1];
@@ -280,9 +286,17 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
     double* C = static_cast<double*>(g.C) + b * g.c_b;
     const int M = g.M, N = g.N, K = g.K;
     const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int l16 = lane & 15, kq = lane >> 4;
     const bool a_mc = (g.a_m == 1), b_nc = (g.b_n == 1);
-    double acc[4][4] = {};
+    doublex4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
     for (int k0 = 0; k0 < K; k0 += BK) {
         for (int e = tid; e < BM * BK; e += 256) {
             int mm, kk;
@@ -309,26 +323,34 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
             Bs[kk][nn] = (gn < N && gk < K) ? B[int64_t(gk) * g.b_k + int64_t(gn) * g.b_n] : 0.0;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < BK; ++kk) {
-            double a[4], bb[4];
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
-            for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
-            for (int i = 0; i < 4; ++i)
-                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
+#pragma unroll
+        for (int k4 = 0; k4 < BK; k4 += 4) {
+            double a[2], bb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[k4 + kq][wm + 16 * i + l16];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bb[j] = Bs[k4 + kq][wn + 16 * j + l16];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) {
-            const int m = m0 + ty + 16 * i, n = n0 + tx + 16 * j;
-            if (m < M && n < N) {
-                double* c = C + int64_t(m) * g.c_m + int64_t(n) * g.c_n;
-                double v = g.alpha * acc[i][j];
-                if (g.beta != 0.0) v += g.beta * *c;
-                *c = v;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + 16 * i + kq + 4 * r, n = n0 + wn + 16 * j + l16;
+                if (m < M && n < N) {
+                    double* c = C + int64_t(m) * g.c_m + int64_t(n) * g.c_n;
+                    double v = g.alpha * acc[i][j][r];
+                    if (g.beta != 0.0) v += g.beta * *c;
+                    *c = v;
+                }
             }
-        }
 }
 
 template <int BM, int BN, int BK, bool SPLIT, bool AMC, bool BNC, bool EARLYC>
