@@ -184,6 +184,54 @@ def test_cli_reads_an_xtc_trajectory(tmp_path):
         np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-5 * np.abs(b).max(), err_msg=suffix)
 
 
+THIRTEEN = ("irrtcf", "iritcf", "vvrtcf", "vvitcf", "vhrtcf", "vhitcf", "spdn", "Pw", "Rw", "Rmw", "irls",
+            "vvls", "vhls")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_cli_files_equal_the_oracle_files(tmp_path, prec):
+    # one input file drives the drop-in executable (XTC read while the batched
+    # run goes) and the CPU oracle on the decoded frames; each of the 13 files
+    # of calcIR.cu:843-919 agrees.  FP64: to the 6 significant digits the
+    # files hold.  BF16x3: TCFs and lineshapes within 1e-3 of the largest
+    # entry; Pw within the bin moves of the eigenvalue error.
+    nwater, nframes = 64, 40
+    xyz, box, times = frames(nwater, nframes, 11)
+    traj = tmp_path / "traj.xtc"
+    C.xtc_write(traj, xyz, box, times)
+    xq, bq, tq, _ = C.xtc_read(traj)
+    inp = tmp_path / "input.inp"
+    inp.write_text("xtcf %s\noutf %s\nmodel tip4p2005\nspecies H2O\nntcfpoints 12\nnsamples 3\n"
+                   "sampleEvery 0.1\navef 3400\nomegaStart 2800\nomegaStop 4000\nomegaStep 5\nnzeros 500\n"
+                   "precision %s\nmode batched\n" % (traj, tmp_path / "cli", prec))
+    announce("calcir CLI prec=%s against the oracle's 13 files" % prec)
+    r = subprocess.run([C.CLI_PATH, str(inp)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    p = C.read_params(str(inp))
+    p.outf = str(tmp_path / "orc").encode()
+    o = O.run(O.F64, O.config_from_params(p), xq, bq, tq)
+    C.write_outputs(p, o)
+    steps = 3 * 12 * 2 * nwater  # mode-steps binned into Pw
+    for s in THIRTEEN:
+        a = np.loadtxt(tmp_path / ("cli_%s.dat" % s))
+        b = np.loadtxt(tmp_path / ("orc_%s.dat" % s))
+        assert a.shape == b.shape, s
+        assert np.array_equal(a[:, 0], b[:, 0]), s  # time / frequency column
+        y, z = a[:, 1], b[:, 1]
+        if s in ("Pw", "Rw", "Rmw") and prec != "fp64":
+            if s == "Pw":
+                moves = 0.5 * np.abs(y - z).sum() * 5 * steps
+                assert moves <= 4 * steps * 2 * 1e-3 / 5 + 2, moves
+            continue
+        assert np.array_equal(np.isnan(y), np.isnan(z)), s
+        m = ~np.isnan(z)
+        scale = np.abs(z[m]).max() if m.any() else 1.0
+        if prec == "fp64":
+            np.testing.assert_allclose(y[m], z[m], rtol=2e-5, atol=2e-6 * scale, err_msg=s)
+        else:
+            assert np.abs(y[m] - z[m]).max() <= 1e-3 * scale, s
+
+
 def test_checkpoint_halt_and_resume_reproduce_the_uninterrupted_run(tmp_path):
     # max_frames stops a run at a chunk end with a checkpoint (CIR_INCOMPLETE);
     # resume continues from it.  Two halts and a final resume give the
