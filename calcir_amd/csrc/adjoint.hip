@@ -687,7 +687,7 @@ extern "C" int cir_set_coefs(cir_ctx* c, const double* coef) {
 }
 
 extern "C" int cir_forward_ls(cir_ctx* c, int nframes, const float* xyz, const float* box, const float* times,
-                              double* ftcf_ir) {
+                              double* ftcf_ir) try {
     if (!c || !xyz || !box || !times || !ftcf_ir || nframes <= 0)
         return fail(CIR_ERR_ARG, "cir_forward_ls: bad arguments");
     if (c->p.adj_store < 0 || c->p.adj_store > 2) return fail(CIR_ERR_ARG, "adjoint_store must be 0, 1 or 2");
@@ -707,11 +707,15 @@ extern "C" int cir_forward_ls(cir_ctx* c, int nframes, const float* xyz, const f
     if (c->real_prec == CIR_PREC_FP64)
         return forward_impl<double>(c, nframes, dx.as<float>(), db.as<float>(), times, ftcf_ir);
     return forward_impl<float>(c, nframes, dx.as<float>(), db.as<float>(), times, ftcf_ir);
+} catch (...) {
+    return cir::host_exception("cir_forward_ls");
 }
 
-extern "C" int cir_backward_ls(cir_ctx* c, const double* dL_dftcf_ir, double* dL_dcoef) {
+extern "C" int cir_backward_ls(cir_ctx* c, const double* dL_dftcf_ir, double* dL_dcoef) try {
     if (!c || !dL_dftcf_ir || !dL_dcoef) return fail(CIR_ERR_ARG, "cir_backward_ls: bad arguments");
     if (!c->adj || !c->adj->have) return fail(CIR_ERR_ARG, "cir_backward_ls: call cir_forward_ls first");
     if (c->real_prec == CIR_PREC_FP64) return backward_impl<double>(c, dL_dftcf_ir, dL_dcoef);
     return backward_impl<float>(c, dL_dftcf_ir, dL_dcoef);
+} catch (...) {
+    return cir::host_exception("cir_backward_ls");
 }

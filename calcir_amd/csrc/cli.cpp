@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -70,7 +71,7 @@ int stream_xtc(cir_ctx* ctx, const cir_params& p, int natoms, int nframes, const
 
 }  // namespace
 
-extern "C" int cir_main(int argc, char** argv) {
+extern "C" int cir_main(int argc, char** argv) try {
     if (argc != 2) {
         std::printf("Usage:\n"
                     "\tInclude as the first argument the name of an input file. No other arguments are allowed.\n");
@@ -207,4 +208,7 @@ extern "C" int cir_main(int argc, char** argv) {
     std::printf(">>> %lld (sample, tcf point) steps over %lld distinct frames, %.3f s on the device.\n",
                 (long long)r.nsteps, (long long)r.nframes_used, r.seconds);
     return 0;
+} catch (const std::exception& e) {
+    std::printf("ERROR: %s\n", e.what());
+    return EXIT_FAILURE;
 }

@@ -2,6 +2,8 @@
 #include "common.hpp"
 
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 
 namespace cir {
@@ -23,6 +25,22 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return CIR_ERR_NOMEM;
     if (e == hipErrorNoDevice) return CIR_ERR_NODEVICE;
     return CIR_ERR_HIP;
+}
+
+int host_exception(const char* where) noexcept {
+    try {
+        try {
+            throw;
+        } catch (const std::bad_alloc&) {
+            return fail(CIR_ERR_NOMEM, std::string(where) + ": host allocation failed");
+        } catch (const std::exception& e) {
+            return fail(CIR_ERR_ARG, std::string(where) + ": " + e.what());
+        } catch (...) {
+            return fail(CIR_ERR_ARG, std::string(where) + ": unknown exception");
+        }
+    } catch (...) {  // the detail text itself could not be allocated
+        return CIR_ERR_NOMEM;
+    }
 }
 
 // Map coefficients, calcIR.cu:1203-1242: imap 0 Gruenbaum 2013, imap 1 Li &

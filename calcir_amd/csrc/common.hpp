@@ -37,6 +37,11 @@ bool map_coef(int imap, int ispecies, MapCoef* c);
 void set_error(const std::string& msg);
 int fail(int status, const std::string& msg);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
+// Status for a C++ exception caught at a C entry point (called from the
+// entry point's catch handler): host allocation failure -> CIR_ERR_NOMEM,
+// anything else -> CIR_ERR_ARG, with `where` in the detail text.  No C++
+// exception crosses the C ABI.
+int host_exception(const char* where) noexcept;
 
 #define CIR_HIP(call)                                                        \
     do {                                                                     \

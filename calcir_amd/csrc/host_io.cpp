@@ -62,7 +62,7 @@ extern "C" void cir_params_default(cir_params* p) {
     p->origin_stride = 0;
 }
 
-extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
+extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) try {
     if (!path || !p) return cir::fail(CIR_ERR_ARG, "cir_params_read: null argument");
     FILE* f = std::fopen(path, "r");
     if (!f) {
@@ -121,6 +121,8 @@ extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) {
     std::fclose(f);
     if (verbose) std::printf(">>> Done reading input file and setting parameters\n");
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_params_read");
 }
 
 extern "C" void cir_params_echo(const cir_params* p) {
@@ -150,16 +152,18 @@ extern "C" void cir_params_echo(const cir_params* p) {
     if (p->origin_stride > 0) std::printf("\tTime origins every %d frames\n", p->origin_stride);
 }
 
-extern "C" int cir_resolve_model(const char* model, int* imodel) {
+extern "C" int cir_resolve_model(const char* model, int* imodel) try {
     // calcIR.cu:325-330
     if (!model || !imodel) return cir::fail(CIR_ERR_ARG, "null argument");
     if (!std::strcmp(model, "tip4p2005") || !std::strcmp(model, "e3b3")) *imodel = 1;
     else if (!std::strcmp(model, "tip4p") || !std::strcmp(model, "e3b2")) *imodel = 0;
     else return cir::fail(CIR_ERR_MODEL, std::string("model ") + model + " is not recognized");
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_resolve_model");
 }
 
-extern "C" int cir_resolve_species(const char* species, int* ispecies) {
+extern "C" int cir_resolve_species(const char* species, int* ispecies) try {
     // calcIR.cu:332-339
     if (!species || !ispecies) return cir::fail(CIR_ERR_ARG, "null argument");
     if (!std::strcmp(species, "H2O")) *ispecies = 0;
@@ -168,6 +172,8 @@ extern "C" int cir_resolve_species(const char* species, int* ispecies) {
     else if (!std::strcmp(species, "D2O")) *ispecies = 3;
     else return cir::fail(CIR_ERR_SPECIES, std::string("species ") + species + " is not recognized");
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_resolve_species");
 }
 
 extern "C" int cir_nomega(const cir_params* p) {
@@ -206,7 +212,7 @@ int write_pairs(const std::string& name, const std::vector<std::pair<float, floa
 
 }  // namespace
 
-extern "C" int cir_write_outputs(const cir_params* p, const cir_results* r) {
+extern "C" int cir_write_outputs(const cir_params* p, const cir_results* r) try {
     if (!p || !r) return cir::fail(CIR_ERR_ARG, "cir_write_outputs: null argument");
     const int ntcf = p->ntcfpoints, nz = p->nzeros, nfft = ntcf + nz;
     const int nomega = cir_nomega(p);
@@ -252,4 +258,6 @@ extern "C" int cir_write_outputs(const cir_params* p, const cir_results* r) {
         CIR_TRY(write_pairs(out + lnames[s], rows, false));
     }
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_write_outputs");
 }

@@ -699,7 +699,7 @@ extern "C" int cir_device_count(int* n) {
     return CIR_OK;
 }
 
-extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx** out) {
+extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx** out) try {
     if (!p || !out) return fail(CIR_ERR_ARG, "cir_create: null argument");
     *out = nullptr;
     int imodel = 0, ispecies = 0;
@@ -736,6 +736,8 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     c->stream = as_stream(stream);
     *out = c;
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_create");
 }
 
 namespace {
@@ -747,7 +749,7 @@ void reset_phases(cir_ctx* c) {
 }
 }  // namespace
 
-extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
+extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) try {
     if (!c || !xyz_d || !box_d || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_run_device: bad arguments");
     {
         // the boxes are few bytes: check them on the host
@@ -772,9 +774,11 @@ extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const
     (void)hipEventDestroy(e1);
     c->prof.collect(c->phase_sec, c->phase_cnt, CIR_NPHASE);
     return st;
+} catch (...) {
+    return cir::host_exception("cir_run_device");
 }
 
-extern "C" int cir_stream_begin(cir_ctx* c, int nframes, const float* times) {
+extern "C" int cir_stream_begin(cir_ctx* c, int nframes, const float* times) try {
     if (!c || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_stream_begin: bad arguments");
     c->streamed.reset();
     if (c->p.mode == CIR_MODE_PLAIN) return fail(CIR_ERR_ARG, "cir_stream_begin: streaming needs mode batched");
@@ -790,6 +794,8 @@ extern "C" int cir_stream_begin(cir_ctx* c, int nframes, const float* times) {
     s->nframes = nframes;
     c->streamed = std::move(s);
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_stream_begin");
 }
 
 namespace {
@@ -834,14 +840,17 @@ int stream_push(cir_ctx* c, int first, int count, const float* xyz, const float*
 }
 }  // namespace
 
-extern "C" int cir_stream_push(cir_ctx* c, int first, int count, const float* xyz, const float* box) {
+extern "C" int cir_stream_push(cir_ctx* c, int first, int count, const float* xyz, const float* box) try {
     if (!c) return fail(CIR_ERR_ARG, "cir_stream_push: null context");
     const int st = stream_push(c, first, count, xyz, box);
     if (st != CIR_OK) c->streamed.reset();  // a failed push closes the stream
     return st;
+} catch (...) {
+    c->streamed.reset();
+    return cir::host_exception("cir_stream_push");
 }
 
-extern "C" int cir_stream_end(cir_ctx* c) {
+extern "C" int cir_stream_end(cir_ctx* c) try {
     if (!c) return fail(CIR_ERR_ARG, "cir_stream_end: null context");
     std::unique_ptr<StreamRun, StreamDeleter> s = std::move(c->streamed);
     if (!s) return fail(CIR_ERR_ARG, "cir_stream_end: no stream is open (cir_stream_begin)");
@@ -859,6 +868,8 @@ extern "C" int cir_stream_end(cir_ctx* c) {
     }
     c->prof.collect(c->phase_sec, c->phase_cnt, CIR_NPHASE);
     return st;
+} catch (...) {
+    return cir::host_exception("cir_stream_end");
 }
 
 extern "C" int cir_set_profiling(cir_ctx* c, int on) {
@@ -876,7 +887,7 @@ extern "C" int cir_phase_times(cir_ctx* c, double* seconds, long long* groups, i
     return CIR_OK;
 }
 
-extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* box, const float* times) {
+extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* box, const float* times) try {
     if (!c || !xyz || !box || !times || nframes <= 0) return fail(CIR_ERR_ARG, "cir_run: bad arguments");
     const size_t nx = size_t(nframes) * c->natoms * 3;
     for (size_t i = 0; i < nx; ++i)
@@ -889,6 +900,8 @@ extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* b
     CIR_HIP(hipMemcpyAsync(dx.p, xyz, nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     CIR_HIP(hipMemcpyAsync(db.p, box, nbx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     return cir_run_device(c, nframes, dx.as<float>(), db.as<float>(), times);
+} catch (...) {
+    return cir::host_exception("cir_run");
 }
 
 // Independent jobs (boxes of any size) at once on one GPU: each job runs on
@@ -896,7 +909,7 @@ extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* b
 // first.  Kernels of different jobs overlap on the device, which fills the
 // GPU when single small boxes cannot.
 extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, const float* const* xyz,
-                            const float* const* box, const float* const* times, int nthreads) {
+                            const float* const* box, const float* const* times, int nthreads) try {
     if (njob < 0 || (njob > 0 && (!ctxs || !nframes || !xyz || !box || !times)))
         return fail(CIR_ERR_ARG, "cir_run_many: bad arguments");
     for (int j = 0; j < njob; ++j)
@@ -941,7 +954,11 @@ extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, 
         }
     };
     std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t) pool.emplace_back(worker);
+    try {
+        for (int t = 0; t < nt; ++t) pool.emplace_back(worker);
+    } catch (...) {  // fewer threads than asked: the ones started take all jobs
+    }
+    if (pool.empty()) worker();
     for (auto& t : pool) t.join();
     for (int j = 0; j < njob; ++j) {
         ctxs[j]->stream = saved[j];
@@ -951,9 +968,11 @@ extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, 
     for (int j = 0; j < njob; ++j)
         if (status[j] != CIR_OK) return fail(status[j], "job " + std::to_string(j) + ": " + detail[j]);
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_run_many");
 }
 
-extern "C" int cir_get_results(cir_ctx* c, cir_results* r) {
+extern "C" int cir_get_results(cir_ctx* c, cir_results* r) try {
     if (!c || !r) return fail(CIR_ERR_ARG, "cir_get_results: null argument");
     if (!c->have) return fail(CIR_ERR_ARG, "cir_get_results: no completed run");
     const int nt = c->p.ntcfpoints;
@@ -973,6 +992,8 @@ extern "C" int cir_get_results(cir_ctx* c, cir_results* r) {
     r->hist_skipped = c->hist_skipped;
     r->seconds = c->seconds;
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_get_results");
 }
 
 extern "C" void cir_destroy(cir_ctx* c) { delete c; }

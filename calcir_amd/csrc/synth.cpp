@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "calcir.h"
+#include "common.hpp"
 
 namespace {
 
@@ -125,7 +126,7 @@ float cir_synth_box_nm(int nwater) {
 }
 
 int cir_synth_trajectory(int nwater, int nframes, uint64_t seed, float box_nm, float* xyz,
-                         float* times) {
+                         float* times) try {
     if (nwater <= 0 || nframes <= 0 || !(box_nm > 0.0f) || !xyz) return CIR_ERR_ARG;
     SplitMix64 rng(seed);
     const double L = box_nm;
@@ -249,6 +250,8 @@ int cir_synth_trajectory(int nwater, int nframes, uint64_t seed, float box_nm, f
         if (times) times[f] = float(f * 0.010);
     }
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_synth_trajectory");
 }
 
 }  // extern "C"

@@ -237,6 +237,11 @@ int size_of_ints3(const unsigned int sizes[3]) {
     return nbits + nb * 8;
 }
 
+// The compressed block of natoms atoms holds at most 3 x 32 bits of an atom's
+// full coordinates and 6 flag bits per atom (about 13 bytes; the format's
+// writers size their buffer at 14.4): a larger byte count is a corrupt frame.
+bool coord_bytes_ok(int nbytes, int natoms) { return nbytes >= 0 && nbytes <= 16LL * natoms + 64; }
+
 #pragma clang fp contract(off)
 float round_half_away(float x, float precision) {
     float lf = x * precision;
@@ -256,9 +261,15 @@ bool read_coords(XdrIn& x, int natoms, float* out) {
     int minint[3], maxint[3];
     for (int k = 0; k < 3; ++k) minint[k] = x.i32();
     for (int k = 0; k < 3; ++k) maxint[k] = x.i32();
+    if (!x.ok || !(precision > 0.0f)) return false;
     unsigned int sizeint[3];
     int bitsizeint[3] = {0, 0, 0}, bitsize = 0;
-    for (int k = 0; k < 3; ++k) sizeint[k] = unsigned(maxint[k] - minint[k]) + 1;
+    for (int k = 0; k < 3; ++k) {
+        // the writer's range limit; an empty or wider range would give a zero
+        // or wrapped radix
+        if (maxint[k] < minint[k] || (long long)maxint[k] - minint[k] >= 0x7fffffffLL) return false;
+        sizeint[k] = unsigned(maxint[k] - minint[k]) + 1;
+    }
     if ((sizeint[0] | sizeint[1] | sizeint[2]) > 0xffffff) {
         for (int k = 0; k < 3; ++k) bitsizeint[k] = size_of_int(sizeint[k]);
     } else {
@@ -266,7 +277,7 @@ bool read_coords(XdrIn& x, int natoms, float* out) {
     }
     int smallidx = x.i32();
     const int nbytes = x.i32();
-    if (!x.ok || smallidx < kFirstIdx || smallidx >= kLastIdx || nbytes < 0) return false;
+    if (!x.ok || smallidx < kFirstIdx || smallidx >= kLastIdx || !coord_bytes_ok(nbytes, natoms)) return false;
     std::vector<unsigned char> data(size_t(nbytes) + 8, 0);
     if (!x.bytes(data.data(), size_t(nbytes))) return false;
     BitReader br{data.data(), size_t(nbytes)};
@@ -465,6 +476,7 @@ bool read_header(XdrIn& x, int* natoms, int* step, float* time, float box[9]) {
     *step = x.i32();
     *time = x.f32();
     for (int q = 0; q < 9; ++q) box[q] = x.f32();
+    if (x.ok && *natoms < 0) x.ok = false;  // corrupt header
     return x.ok;
 }
 
@@ -476,7 +488,7 @@ bool skip_coords(XdrIn& x, int natoms) {
     if (natoms <= 9) return std::fseek(x.f, 12L * natoms, SEEK_CUR) == 0;
     for (int k = 0; k < 8; ++k) (void)x.i32();  // precision, minint[3], maxint[3], smallidx
     const int nbytes = x.i32();
-    if (!x.ok || nbytes < 0) return false;
+    if (!x.ok || !coord_bytes_ok(nbytes, natoms)) return false;
     return std::fseek(x.f, (long(nbytes) + 3) & ~3L, SEEK_CUR) == 0;
 }
 
@@ -516,7 +528,7 @@ int frame_offsets(FILE* f, std::vector<int64_t>& off) {
 }
 }  // namespace
 
-extern "C" int cir_xtc_offsets(const char* path, int nframes, int64_t* offsets) {
+extern "C" int cir_xtc_offsets(const char* path, int nframes, int64_t* offsets) try {
     if (!path || nframes < 0 || (nframes > 0 && !offsets)) return fail(CIR_ERR_ARG, "cir_xtc_offsets: bad argument");
     FILE* f = std::fopen(path, "rb");
     if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
@@ -527,9 +539,11 @@ extern "C" int cir_xtc_offsets(const char* path, int nframes, int64_t* offsets) 
     if (int64_t(off.size()) < nframes) return fail(CIR_ERR_FRAME, "xtc: frame " + std::to_string(off.size()) + " missing");
     for (int k = 0; k < nframes; ++k) offsets[k] = off[k];
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_xtc_offsets");
 }
 
-extern "C" int cir_xtc_seek(cir_xtc* r, int frame) {
+extern "C" int cir_xtc_seek(cir_xtc* r, int frame) try {
     if (!r || frame < 0) return fail(CIR_ERR_ARG, "cir_xtc_seek: bad argument");
     if (r->offset.empty()) CIR_TRY(frame_offsets(r->f, r->offset));
     if (frame >= int(r->offset.size()))
@@ -538,9 +552,11 @@ extern "C" int cir_xtc_seek(cir_xtc* r, int frame) {
     if (std::fseek(r->f, long(r->offset[frame]), SEEK_SET) != 0) return fail(CIR_ERR_IO, "xtc: seek failed");
     r->frame = frame;
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_xtc_seek");
 }
 
-extern "C" int cir_xtc_open(const char* path, int natoms, cir_xtc** out) {
+extern "C" int cir_xtc_open(const char* path, int natoms, cir_xtc** out) try {
     if (!path || !out || natoms < 0) return fail(CIR_ERR_ARG, "cir_xtc_open: bad argument");
     *out = nullptr;
     FILE* f = std::fopen(path, "rb");
@@ -550,9 +566,11 @@ extern "C" int cir_xtc_open(const char* path, int natoms, cir_xtc** out) {
     r->natoms = natoms;
     *out = r;
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_xtc_open");
 }
 
-extern "C" int cir_xtc_next(cir_xtc* r, int count, float* xyz, float* box9, float* times) {
+extern "C" int cir_xtc_next(cir_xtc* r, int count, float* xyz, float* box9, float* times) try {
     if (!r || count < 0 || (count > 0 && !xyz)) return fail(CIR_ERR_ARG, "cir_xtc_next: bad argument");
     XdrIn x{r->f};
     for (int k = 0; k < count; ++k) {
@@ -568,11 +586,13 @@ extern "C" int cir_xtc_next(cir_xtc* r, int count, float* xyz, float* box9, floa
         ++r->frame;
     }
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_xtc_next");
 }
 
 extern "C" void cir_xtc_close(cir_xtc* r) { delete r; }
 
-extern "C" int cir_xtc_times(const char* path, int nframes, float* times, float* box9) {
+extern "C" int cir_xtc_times(const char* path, int nframes, float* times, float* box9) try {
     if (!path || nframes < 0) return fail(CIR_ERR_ARG, "cir_xtc_times: bad argument");
     FILE* f = std::fopen(path, "rb");
     if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
@@ -596,22 +616,29 @@ extern "C" int cir_xtc_times(const char* path, int nframes, float* times, float*
     }
     std::fclose(f);
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_xtc_times");
 }
 
-extern "C" int cir_xtc_scan(const char* path, int* natoms, int* nframes) {
+extern "C" int cir_xtc_scan(const char* path, int* natoms, int* nframes) try {
     if (!path || !natoms || !nframes) return fail(CIR_ERR_ARG, "cir_xtc_scan: null argument");
     FILE* f = std::fopen(path, "rb");
     if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
     XdrIn x{f};
+    // every atom takes at least two bits of the file: a larger count in a
+    // header is corrupt (and would size the decode buffer from garbage)
+    long long fsize = 0;
+    if (std::fseek(f, 0, SEEK_END) == 0) fsize = std::ftell(f);
+    std::rewind(f);
     int n = 0, na = -1;
     std::vector<float> buf;
     for (;;) {
         int nat, step;
         float t, box[9];
         if (!read_header(x, &nat, &step, &t, box)) break;
-        if (na >= 0 && nat != na) {
+        if ((na >= 0 && nat != na) || nat > 4 * fsize + 9) {
             std::fclose(f);
-            return fail(CIR_ERR_IO, "xtc: atom count changes between frames");
+            return fail(CIR_ERR_IO, "xtc: atom count changes between frames or exceeds the file");
         }
         na = nat;
         buf.resize(3 * size_t(nat));
@@ -627,6 +654,8 @@ extern "C" int cir_xtc_scan(const char* path, int* natoms, int* nframes) {
     *natoms = (na < 0) ? 0 : na;
     *nframes = n;
     return CIR_OK;
+} catch (...) {
+    return cir::host_exception("cir_xtc_scan");
 }
 
 namespace {
@@ -706,21 +735,29 @@ int write_frames(const char* path, int natoms, int nframes, const float* xyz, co
 }  // namespace
 
 extern "C" int cir_xtc_read(const char* path, int first, int count, int natoms, float* xyz, float* box,
-                            float* times, int* steps) {
+                            float* times, int* steps) try {
     return read_frames(path, first, count, natoms, xyz, box, 3, times, steps);
+} catch (...) {
+    return cir::host_exception("cir_xtc_read");
 }
 
 extern "C" int cir_xtc_read_full(const char* path, int first, int count, int natoms, float* xyz, float* box9,
-                                 float* times, int* steps) {
+                                 float* times, int* steps) try {
     return read_frames(path, first, count, natoms, xyz, box9, 9, times, steps);
+} catch (...) {
+    return cir::host_exception("cir_xtc_read_full");
 }
 
 extern "C" int cir_xtc_write(const char* path, int natoms, int nframes, const float* xyz, const float* box,
-                             const float* times, float precision, int append) {
+                             const float* times, float precision, int append) try {
     return write_frames(path, natoms, nframes, xyz, box, 3, times, precision, append);
+} catch (...) {
+    return cir::host_exception("cir_xtc_write");
 }
 
 extern "C" int cir_xtc_write_full(const char* path, int natoms, int nframes, const float* xyz, const float* box9,
-                                  const float* times, float precision, int append) {
+                                  const float* times, float precision, int append) try {
     return write_frames(path, natoms, nframes, xyz, box9, 9, times, precision, append);
+} catch (...) {
+    return cir::host_exception("cir_xtc_write_full");
 }

@@ -7,7 +7,10 @@
  * Conventions
  *   - Every entry point returns an int status (cir_status); none exits the
  *     process.  cir_strerror() names a status; cir_last_error() returns the
- *     detail text of the most recent failure on the calling thread.
+ *     detail text of the most recent failure on the calling thread.  No
+ *     C++ exception crosses the ABI: a failed host allocation returns
+ *     CIR_ERR_NOMEM.  Corrupt XTC input (bad counts or ranges in a frame)
+ *     returns CIR_ERR_IO.
  *   - Pointers named *_d are device pointers (hipMalloc'd, on the current
  *     HIP device); "stream" is a hipStream_t passed as void* (NULL = the
  *     null stream).  Per-stage calls enqueue work and return without

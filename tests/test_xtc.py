@@ -178,3 +178,52 @@ def test_frame_offsets_and_seek(tmp_path):
         assert np.array_equal(buf, xr[5:7])
     finally:
         L.cir_xtc_close(rd)
+
+
+def _patched(tmp_path, name, data, offset, value):
+    b = bytearray(data)
+    b[offset:offset + 4] = struct.pack(">i", value)
+    g = tmp_path / name
+    g.write_bytes(bytes(b))
+    return g
+
+
+@pytest.mark.timeout(60)
+def test_corrupt_headers_and_coordinate_blocks_are_errors(tmp_path):
+    # A compressed frame: 52 header bytes (magic, natoms, step, time, box),
+    # then natoms, precision, minint[3] at 60, maxint[3] at 72, smallidx,
+    # and the byte count at 88 (maxint[2] < minint[2] makes the z radix 0,
+    # a divisor of the decode).  Corrupt values end as errors: no
+    # backward seek (a negative atom count), no zero radix (maxint < minint),
+    # no allocation sized by garbage (byte or atom counts beyond the file).
+    xyz, box, times = C.synth_trajectory(16, 3, 5)
+    f = tmp_path / "ok.xtc"
+    C.xtc_write(f, xyz, box, times)
+    data = f.read_bytes()
+    assert struct.unpack(">i", data[4:8])[0] == 64
+    nbytes = struct.unpack(">i", data[88:92])[0]
+    frame = 92 + (nbytes + 3) // 4 * 4
+    assert len(data) == 3 * frame
+    L = C.lib()
+    off = np.zeros(3, np.int64)
+    t = np.zeros(3, np.float32)
+    for name, at, value in (("radix.xtc", 80, struct.unpack(">i", data[68:72])[0] - 1),
+                            ("nbytes.xtc", 88, 0x7FFFFF00),
+                            ("huge.xtc", 4, 0x7FFFFFFF)):
+        g = _patched(tmp_path, name, data, at, value)
+        with pytest.raises(C.CalcirError):
+            C.xtc_scan(g)
+        if name != "huge.xtc":
+            rd = np.zeros((1, 64, 3), np.float32)
+            assert L.cir_xtc_read(os.fsencode(g), 0, 1, 64, rd.ctypes.data, None, None, None) != 0
+        if name == "nbytes.xtc":  # the header-only passes skip by the byte count
+            assert L.cir_xtc_offsets(os.fsencode(g), 3, off.ctypes.data) != 0
+            assert L.cir_xtc_times(os.fsencode(g), 3, t.ctypes.data, None) != 0
+    # a negative atom count in the second frame's header: the passes stop there
+    g = _patched(tmp_path, "neg.xtc", data, frame + 4, -5)
+    assert C.xtc_scan(g) == (64, 1)
+    assert L.cir_xtc_offsets(os.fsencode(g), 1, off.ctypes.data) == 0 and off[0] == 0
+    assert L.cir_xtc_offsets(os.fsencode(g), 2, off.ctypes.data) != 0
+    assert L.cir_xtc_times(os.fsencode(g), 2, t.ctypes.data, None) != 0
+    # the untouched file still reads
+    assert C.xtc_scan(f) == (64, 3)
