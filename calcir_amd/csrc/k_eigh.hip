@@ -233,9 +233,33 @@ __device__ __forceinline__ constexpr int pcol(int p) {
     return sizeof(T) == 4 ? (p & 15) + 16 * (p >> 5) : p & (kTriNB - 1);
 }
 
+// LDS of the workgroup-per-matrix kernel, in elements of T: the union of
+// (v, p) and, for FP64, the trailing update's two staging tiles (FP32 updates
+// on MFMA straight from the panel), rounded to 16 bytes, then the small
+// reduction arrays.  The FP32 kernel also keeps the first `rl` rows of each
+// panel in LDS (tridiag_panel_rows), ahead of all of this.
+template <typename T>
+constexpr size_t tridiag_uni_elems(int n) {
+    return sizeof(T) == 4 ? (2 * size_t(n) + 3) & ~size_t(3)
+                          : std::max<size_t>(2 * size_t(n), size_t(2) * 2 * kTriNB * kTileLd);
+}
+template <typename T>
 inline size_t tridiag_smem_elems(int n) {
-    const size_t uni = std::max<size_t>(2 * size_t(n), size_t(2) * 2 * kTriNB * kTileLd);
-    return uni + size_t(kTriNW) + 2 * kTriNB + 2 * kTriNB + size_t(kTriNW) * 2 * kTriNB + size_t(kTriNW);
+    return tridiag_uni_elems<T>(n) + size_t(kTriNW) + 2 * kTriNB + 2 * kTriNB + size_t(kTriNW) * 2 * kTriNB +
+           size_t(kTriNW);
+}
+constexpr size_t kLdsBytes = 160 * 1024;  // LDS of a gfx950 CU, all of it one workgroup's
+// Panel rows the FP32 kernel keeps in LDS: what the rest of the 160 KiB holds
+// (rows j0 .. j0 + rl - 1 of the panel at j0; the fused pass reads them every
+// reflector), at least the 33 rows a panel's row j and j + 1 reads need, else
+// none.
+template <typename T>
+inline int tridiag_panel_rows(int n) {
+    if (sizeof(T) != 4) return 0;
+    const size_t base = tridiag_smem_elems<T>(n) * sizeof(T), row = 2 * kTriNB * sizeof(T);
+    if (base >= kLdsBytes) return 0;
+    const int rl = int(std::min<size_t>(size_t(n), (kLdsBytes - base) / row));
+    return rl >= 64 ? rl : 0;
 }
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -328,13 +352,14 @@ __device__ void trailing_update_f32(int n, int ld, int jn, float* __restrict__ A
 template <typename T>
 __global__ void __launch_bounds__(kTriNT, 2)
 tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __restrict__ eall,
-               T* __restrict__ tauall, T* __restrict__ VWall) {
+               T* __restrict__ tauall, T* __restrict__ VWall, int rl) {
     using VT = typename Vec16<T>::type;
     constexpr int VW = Vec16<T>::n;
     constexpr int PW = 2 * kTriNB;  // panel row width
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* const uni = reinterpret_cast<T*>(smem_raw);
-    const size_t unisz = max(2 * size_t(n), size_t(2) * PW * kTileLd);
+    T* const sP = reinterpret_cast<T*>(smem_raw);  // rl x PW: panel rows j0 .. j0 + rl - 1
+    T* const uni = sP + size_t(rl) * PW;
+    const size_t unisz = tridiag_uni_elems<T>(n);
     T* const sv = uni;                   // Householder vector (n)
     T* const sp = uni + n;               // column / product (n)
     T* const sL = uni;                   // trailing update: row tile [PW][kTileLd]
@@ -357,10 +382,14 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
 
     for (int j0 = 0; j0 < n - 1; j0 += kTriNB) {
         const int nbp = min(kTriNB, n - 1 - j0);
+        // rows below lim of this panel live in LDS (sP), the rest in P
+        const int lim = min(n, j0 + rl);
+        auto prow = [&](int r) -> T* { return (r < lim) ? sP + size_t(r - j0) * PW : P + size_t(r) * PW; };
         // clear the panel rows this panel uses (columns not formed yet are
         // read as zeros)
-        for (size_t q = size_t(j0) * PW + size_t(tid) * VW; q < size_t(n) * PW; q += size_t(kTriNT) * VW)
+        for (size_t q = size_t(lim) * PW + size_t(tid) * VW; q < size_t(n) * PW; q += size_t(kTriNT) * VW)
             *reinterpret_cast<VT*>(P + q) = VT{};
+        for (int q = tid * VW; q < (lim - j0) * PW; q += kTriNT * VW) *reinterpret_cast<VT*>(sP + q) = VT{};
         __syncthreads();
         // Per reflector (column j = j0 + i), with tau_prev, K_prev, x/y (sdot),
         // v (sv) and p = A_panel v (sp) of reflector i - 1 still in LDS:
@@ -391,8 +420,9 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
             }
         };
         // W(r, i-1) from the lane's panel-row vector xr (entries updated in
-        // place); returns nothing, writes the finished entry to P
-        auto finish_w = [&](VT& xr, int r, int icol) {
+        // place); returns nothing, writes the finished entry to the panel row
+        // prw (in LDS or in P)
+        auto finish_w = [&](VT& xr, T* prw, int r, int icol) {
             const T* xe = reinterpret_cast<const T*>(&xr);
             T part = T(0);
 #pragma unroll
@@ -405,15 +435,16 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
 #pragma unroll
                 for (int q = 0; q < VW; ++q)
                     if (q == cw) xw[q] = wnew;
-                P[size_t(r) * PW + posW<T>(icol)] = wnew;
+                prw[posW<T>(icol)] = wnew;
             }
         };
         for (int i = 0; i < nbp; ++i) {
             const int j = j0 + i;
             // ---- A0: srow = panel row j (W(j, i-1) finished)
             if (wave == 0 && rsub == 0) {
-                VT xr = *reinterpret_cast<const VT*>(P + size_t(j) * PW + g * VW);
-                if (i > 0) finish_w(xr, j, i - 1);
+                T* const prw = prow(j);
+                VT xr = *reinterpret_cast<const VT*>(prw + g * VW);
+                if (i > 0) finish_w(xr, prw, j, i - 1);
                 *reinterpret_cast<VT*>(srow + g * VW) = xr;
             }
             __syncthreads();
@@ -430,20 +461,24 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 // row; the other half is still zero and is not read
                 const bool need = sizeof(T) != 4 || i > 16 || g * VW < 32;
                 constexpr int kU = 4;
-                for (int rb = j + wave * RPI; rb < n; rb += kU * kTriNW * RPI) {
+                // rows [r0, r1) of the panel at base (sP with row offset j0,
+                // or P): two loops, each with its address space known
+                auto rows = [&](T* base, int roff, int r0, int r1) {
+                for (int rb = r0 + wave * RPI; rb < r1; rb += kU * kTriNW * RPI) {
                     VT xr[kU];
                     T av[kU];
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         const int r = rb + u * kTriNW * RPI + rsub;
-                        xr[u] = (r < n && need) ? *reinterpret_cast<const VT*>(P + size_t(r) * PW + g * VW) : VT{};
-                        av[u] = (r < n) ? A[size_t(j) * ld + r] : T(0);
+                        xr[u] = (r < r1 && need) ? *reinterpret_cast<const VT*>(base + size_t(r - roff) * PW + g * VW)
+                                                 : VT{};
+                        av[u] = (r < r1) ? A[size_t(j) * ld + r] : T(0);
                     }
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         const int r = rb + u * kTriNW * RPI + rsub;
-                        const bool live = r < n;
-                        if (i > 0 && live) finish_w(xr[u], r, i - 1);
+                        const bool live = r < r1;
+                        if (i > 0 && live) finish_w(xr[u], base + size_t(r - roff) * PW, r, i - 1);
                         const T* xe = reinterpret_cast<const T*>(&xr[u]);
                         T part = T(0);
 #pragma unroll
@@ -457,6 +492,9 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                         if (live && g == 0) sp[r] = uu;
                     }
                 }
+                };
+                rows(sP, j0, j, lim);
+                rows(P, 0, max(j, lim), n);
                 // partials: rows of a wave instruction, then waves
 #pragma unroll
                 for (int q = 0; q < VW; ++q)
@@ -488,13 +526,13 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
 #pragma unroll
                 for (int w = 0; w < kTriNW; ++w) raw += sdp[w * PW + tid];
                 const int l = pcol<T>(tid);
-                sdot[tid] = (l < i) ? scal * raw + P[size_t(j + 1) * PW + tid] : T(0);
+                sdot[tid] = (l < i) ? scal * raw + prow(j + 1)[tid] : T(0);
             }
             for (int r = j + 1 + tid; r < n; r += kTriNT) {
                 const T vr = (r == j + 1) ? T(1) : sp[r] * scal;
                 sv[r] = vr;
                 A[size_t(j) * ld + r] = vr;
-                P[size_t(r) * PW + posV<T>(i)] = vr;
+                prow(r)[posV<T>(i)] = vr;
             }
             for (int r = tid; r <= j; r += kTriNT) A[size_t(j) * ld + r] = T(0);
             if (tid == 0) {
@@ -653,13 +691,18 @@ tridiag_kernel(int n, int ld, T* __restrict__ Aall, T* __restrict__ dall, T* __r
                 // a row group (LPR lanes) is live or dead as a whole, so the
                 // row sums never read an inactive lane
                 if (r < n) {
-                    VT xr = *reinterpret_cast<const VT*>(P + size_t(r) * PW + g * VW);
-                    finish_w(xr, r, nbp - 1);
+                    T* const prw = prow(r);
+                    VT xr = *reinterpret_cast<const VT*>(prw + g * VW);
+                    finish_w(xr, prw, r, nbp - 1);
                 }
             }
         }
         // trailing update: A(jn:, jn:) -= V W^T + W V^T on lower tiles
         const int jn = j0 + nbp;
+        __syncthreads();
+        // the update reads the panel from P: the LDS rows from jn on go there
+        for (int q = (jn - j0) * PW + tid * VW; q < (lim - j0) * PW; q += kTriNT * VW)
+            *reinterpret_cast<VT*>(P + size_t(j0) * PW + q) = *reinterpret_cast<const VT*>(sP + q);
         __syncthreads();
         if constexpr (sizeof(T) == 4) {
             if (jn < n) trailing_update_f32<true>(n, ld, jn, A, P, lane, wave, kTriNW);
@@ -1551,12 +1594,13 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
             CIR_TRY(tridiag_single<T>(n, ld, A + size_t(b) * n * ld, dd + size_t(b) * n, ee + size_t(b) * n,
                                       tau + size_t(b) * n, vw + size_t(b) * 2 * kTriNB * ld, s1, s));
     } else {
-        const size_t sm = tridiag_smem_elems(n) * sizeof(T);
-        if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "eigh: n too large for the LDS of one workgroup");
+        const int rl = tridiag_panel_rows<T>(n);
+        const size_t sm = (tridiag_smem_elems<T>(n) + size_t(rl) * 2 * kTriNB) * sizeof(T);
+        if (sm > kLdsBytes) return fail(CIR_ERR_ARG, "eigh: n too large for the LDS of one workgroup");
         CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
         ProfScope ps(prof, CIR_PH_TRIDIAG, s);
-        hipLaunchKernelGGL(tridiag_kernel<T>, dim3(batch), dim3(kTriNT), sm, s, n, ld, A, dd, ee, tau, vw);
+        hipLaunchKernelGGL(tridiag_kernel<T>, dim3(batch), dim3(kTriNT), sm, s, n, ld, A, dd, ee, tau, vw, rl);
         CIR_CHECK_LAUNCH();
     }
     // 2. eigenvalues
