@@ -1281,17 +1281,23 @@ __device__ __forceinline__ double hash_uniform(uint32_t a, uint32_t b) {
 
 // 3 FP64 arrays and a byte array of n x n, rounded up to 256 bytes so that
 // the next matrix's arrays stay aligned
-__host__ __device__ inline size_t invit_scratch_bytes(int n) { return (25 * size_t(n) * n + 255) & ~size_t(255); }
+__host__ __device__ inline size_t invit_scratch_bytes(int n) {
+    return (25 * ((size_t(n) + 63) & ~size_t(63)) * n + 255) & ~size_t(255);
+}
 
 // Scratch of one matrix (invit_scratch_bytes): FP64 arrays X (the vectors),
 // R (reciprocal pivots) and L (multipliers), then the pivot flags F (bytes),
-// element (k, c) at [k*n + c], c the eigenvalue index (lanes run along c, so
-// every access is coalesced).  The LU of T - lambda_c I with partial pivoting
+// each of n rows by n eigenvalues padded to a multiple of 64: element (k, c),
+// c the eigenvalue index, at [(c / 64) 64 n + 64 k + c % 64].  Lanes run along
+// c, so every access is coalesced, and a wave's successive rows are adjacent
+// 512-byte blocks (a row-major n x n layout put them n elements apart).  The LU of T - lambda_c I with partial pivoting
 // has U = (u0, u1, u2) per row; only 1/u0 is kept: u1 and u2 are rebuilt from
 // T, the flag and the previous row's multiplier (a pivoted row has u1 =
 // d(i+1) - lambda, u2 = e(i+1); an unpivoted row u2 = 0 and u1 the carried
 // e(i), times -l(i-1) after a pivoted row), which cuts the scratch traffic by
-// a third.  The second iteration reuses the factorisation.  Factors and
+// a third.  The first iteration's right-hand side is regenerated from its
+// seed, so X is first written by the back substitution.  The second iteration
+// reuses the factorisation.  Factors and
 // vectors are FP64 even for an FP32-derived T: eigenvectors of eigenvalues
 // closer than ~1e-7 ||T|| (but not clustered) are only told apart by factors
 // accurate beyond FP32, and near-degenerate pivots grow the vector past the
@@ -1308,7 +1314,11 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
     const T* e = eall + size_t(b) * n;
     const double* w = wall + size_t(b) * n;
     T* Z = Zall + size_t(b) * n * ld;
-    const size_t nn = size_t(n) * n;
+    // element (k, c) at col(c) + 64 k: tiles of 64 eigenvalues (one wave),
+    // rows contiguous within a tile, so a lane's walk down the rows streams
+    // 512-byte blocks per wave
+    const size_t nn = ((size_t(n) + 63) & ~size_t(63)) * n;
+    auto col = [n](int c) { return size_t(c >> 6) * 64 * n + (c & 63); };
     double* const SX = reinterpret_cast<double*>(scr + size_t(bl) * invit_scratch_bytes(n));
     double* const SR = SX + nn;
     double* const SL = SR + nn;
@@ -1336,21 +1346,21 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
         double lam = w[mb];
         if (mb > c) lam = fmax(lam, lam_prev + 10.0 * DBL_EPSILON * fmax(fabs(lam_prev), 1.0));
         lam_prev = lam;
-        double* X = SX + mb;
-        double* R = SR + mb;
-        double* Lm = SL + mb;
-        unsigned char* F = SF + mb;
-        // iteration 1, forward: factor T - lam I = P L U, apply P L^{-1} to a
-        // seeded start vector
+        double* X = SX + col(mb);
+        double* R = SR + col(mb);
+        double* Lm = SL + col(mb);
+        unsigned char* F = SF + col(mb);
+        // iteration 1, forward: factor T - lam I = P L U.  Its right-hand
+        // side is a seeded vector b taken as already transformed by P L^{-1}
+        // (the start vector is P L b, as in EISPACK's tinvit), so the first
+        // back substitution regenerates b and no vector is stored here.
         double a = sd[0] - lam;
         double bs = (n > 1) ? se[0] : 0.0;
-        double yi = hash_uniform(uint32_t(mb), 0u);
         for (int i = 0; i < n - 1; ++i) {
             const double sub = se[i];
             const double dn = sd[i + 1] - lam;
             const double en = (i + 1 < n - 1) ? se[i + 1] : 0.0;
-            const double yn = hash_uniform(uint32_t(mb), uint32_t(i + 1));
-            double u0, l, yo;
+            double u0, l;
             unsigned char fl;
             if (fabs(a) >= fabs(sub)) {
                 // a pivot below pivmin (0 included) is taken as +-pivmin, as
@@ -1359,61 +1369,56 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                 if (fabs(a) < pivmin) a = (a < 0.0) ? -pivmin : pivmin;
                 l = (sizeof(T) == 8) ? sub / a : sub * nr_rcp<false>(a);
                 u0 = a;
-                yo = yi;
                 a = dn - l * bs;
                 bs = en;
-                yi = yn - l * yi;
                 fl = 0;
             } else {
                 l = (sizeof(T) == 8) ? a / sub : a * nr_rcp<false>(sub);
                 u0 = sub;
-                yo = yn;
                 a = bs - l * dn;
                 bs = -l * en;
-                yi = yi - l * yn;
                 fl = 1;
             }
             if (fabs(u0) < pivmin) u0 = (u0 < 0.0) ? -pivmin : pivmin;
             // reciprocal pivot: the back substitution multiplies; an FP32-derived
             // T takes the refined hardware reciprocal, an FP64 T the IEEE one
-            R[size_t(i) * n] = (sizeof(T) == 8) ? 1.0 / u0 : nr_rcp<false>(u0);
-            Lm[size_t(i) * n] = l;
-            F[size_t(i) * n] = fl;
-            X[size_t(i) * n] = yo;
+            R[size_t(i) * 64] = (sizeof(T) == 8) ? 1.0 / u0 : nr_rcp<false>(u0);
+            Lm[size_t(i) * 64] = l;
+            F[size_t(i) * 64] = fl;
         }
         if (fabs(a) < pivmin) a = (a < 0.0) ? -pivmin : pivmin;
         const double alast = a;
-        double scale = 1.0, big = 0.0;
+        double scale = 1.0, big = 0.0, yi = hash_uniform(uint32_t(mb), uint32_t(n - 1));
         for (int it = 0; it < 2; ++it) {
             if (it == 1) {
                 // forward with the kept factorisation: P L^{-1} (scale x)
                 yi = X[0] * scale;
                 for (int i = 0; i < n - 1; ++i) {
-                    const double yn = X[size_t(i + 1) * n] * scale;
-                    const double l = Lm[size_t(i) * n];
+                    const double yn = X[size_t(i + 1) * 64] * scale;
+                    const double l = Lm[size_t(i) * 64];
                     double yo;
-                    if (F[size_t(i) * n] == 0) {
+                    if (F[size_t(i) * 64] == 0) {
                         yo = yi;
                         yi = yn - l * yi;
                     } else {
                         yo = yn;
                         yi = yi - l * yn;
                     }
-                    X[size_t(i) * n] = yo;
+                    X[size_t(i) * 64] = yo;
                 }
             }
             // backward: U x = y, u1/u2 rebuilt from T and the flags
             double x1 = yi / alast, x2 = 0.0;
-            X[size_t(n - 1) * n] = x1;
+            X[size_t(n - 1) * 64] = x1;
             double nrm2 = x1 * x1;
             big = x1;
-            unsigned char fi = (n > 1) ? F[size_t(n - 2) * n] : 0;
+            unsigned char fi = (n > 1) ? F[size_t(n - 2) * 64] : 0;
             for (int i = n - 2; i >= 0; --i) {
                 unsigned char fp = 0;
                 double lp = 0.0;
                 if (i > 0) {
-                    fp = F[size_t(i - 1) * n];
-                    lp = Lm[size_t(i - 1) * n];
+                    fp = F[size_t(i - 1) * 64];
+                    lp = Lm[size_t(i - 1) * 64];
                 }
                 double u1, u2;
                 if (fi) {
@@ -1423,8 +1428,9 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                     u1 = (i > 0 && fp) ? -lp * se[i] : se[i];
                     u2 = 0.0;
                 }
-                const double xi = (X[size_t(i) * n] - u1 * x1 - u2 * x2) * R[size_t(i) * n];
-                X[size_t(i) * n] = xi;
+                const double rhs = (it == 0) ? hash_uniform(uint32_t(mb), uint32_t(i)) : X[size_t(i) * 64];
+                const double xi = (rhs - u1 * x1 - u2 * x2) * R[size_t(i) * 64];
+                X[size_t(i) * 64] = xi;
                 nrm2 += xi * xi;
                 if (fabs(xi) >= fabs(big)) big = xi;  // first largest component in ascending order
                 x2 = x1;
@@ -1434,13 +1440,13 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
             // orthogonalise against the cluster's earlier vectors
             if (mb > c) {
                 for (int q = c; q < mb; ++q) {
-                    const double* Xq = SX + q;
+                    const double* Xq = SX + col(q);
                     double dot = 0.0;
-                    for (int k = 0; k < n; ++k) dot += X[size_t(k) * n] * Xq[size_t(k) * n];
+                    for (int k = 0; k < n; ++k) dot += X[size_t(k) * 64] * Xq[size_t(k) * 64];
                     nrm2 = 0.0;
                     for (int k = 0; k < n; ++k) {
-                        const double v = X[size_t(k) * n] - dot * Xq[size_t(k) * n];
-                        X[size_t(k) * n] = v;
+                        const double v = X[size_t(k) * 64] - dot * Xq[size_t(k) * 64];
+                        X[size_t(k) * 64] = v;
                         nrm2 += v * v;
                     }
                 }
@@ -1450,23 +1456,23 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
         if (single) {
             // normalise, largest component positive, straight into Z
             const double f = (big < 0.0) ? -scale : scale;
-            for (int k = 0; k < n; ++k) Z[size_t(k) * ld + mb] = T(X[size_t(k) * n] * f);
+            for (int k = 0; k < n; ++k) Z[size_t(k) * ld + mb] = T(X[size_t(k) * 64] * f);
             continue;
         }
         // cluster members stay in scratch, normalised, for the later members
         big = 0.0;
         for (int k = 0; k < n; ++k) {
-            const double v = X[size_t(k) * n] * scale;
-            X[size_t(k) * n] = v;
+            const double v = X[size_t(k) * 64] * scale;
+            X[size_t(k) * 64] = v;
             if (fabs(v) > fabs(big)) big = v;
         }
         if (big < 0.0)
-            for (int k = 0; k < n; ++k) X[size_t(k) * n] = -X[size_t(k) * n];
+            for (int k = 0; k < n; ++k) X[size_t(k) * 64] = -X[size_t(k) * 64];
     }
     if (single) return;
     for (int mb = c; mb <= cend; ++mb) {
-        const double* X = SX + mb;
-        for (int k = 0; k < n; ++k) Z[size_t(k) * ld + mb] = T(X[size_t(k) * n]);
+        const double* X = SX + col(mb);
+        for (int k = 0; k < n; ++k) Z[size_t(k) * ld + mb] = T(X[size_t(k) * 64]);
     }
 }
 
