@@ -68,10 +68,23 @@ struct Prof {
         hipEvent_t a, b;
     };
     std::vector<Rec> recs;
+    Prof() = default;
+    Prof(const Prof&) = delete;
+    Prof& operator=(const Prof&) = delete;
+    ~Prof() {  // records a failed run left uncollected
+        for (Rec& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
     int begin(int phase, hipStream_t s) {
         if (!on) return -1;
         Rec r{phase, nullptr, nullptr};
-        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+        if (hipEventCreate(&r.a) != hipSuccess) return -1;
+        if (hipEventCreate(&r.b) != hipSuccess) {
+            (void)hipEventDestroy(r.a);
+            return -1;
+        }
         (void)hipEventRecord(r.a, s);
         recs.push_back(r);
         return int(recs.size()) - 1;
@@ -92,6 +105,23 @@ struct Prof {
             (void)hipEventDestroy(r.b);
         }
         recs.clear();
+    }
+};
+
+// Two HIP events that bracket a timed region, destroyed with the scope on
+// every return path.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    hipError_t create() {
+        hipError_t e = hipEventCreate(&a);
+        return e == hipSuccess ? hipEventCreate(&b) : e;
+    }
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
     }
 };
 

@@ -542,9 +542,9 @@ static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double 
             tune = Yin != Yout;
     }
     if (tune) {
-        hipEvent_t e0, e1;
-        CIR_HIP(hipEventCreate(&e0));
-        CIR_HIP(hipEventCreate(&e1));
+        EventPair ev;
+        CIR_HIP(ev.create());
+        hipEvent_t e0 = ev.a, e1 = ev.b;
         float best_ms = 0.f;
         int best_ks = guess;
         for (int cand = 1; cand <= std::min(kPropMaxSplit, std::max(1, n / 256)); cand *= 2) {
@@ -560,8 +560,6 @@ static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double 
                 best_ks = cand;
             }
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         ks = best_ks;
         std::lock_guard<std::mutex> lk(prop_tune().mu);
         prop_tune().best.emplace(key, ks);

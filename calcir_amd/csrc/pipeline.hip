@@ -760,18 +760,15 @@ extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const
     }
     c->have = false;
     reset_phases(c);
-    hipEvent_t e0, e1;
-    CIR_HIP(hipEventCreate(&e0));
-    CIR_HIP(hipEventCreate(&e1));
-    CIR_HIP(hipEventRecord(e0, c->stream));
+    EventPair ev;
+    CIR_HIP(ev.create());
+    CIR_HIP(hipEventRecord(ev.a, c->stream));
     int st = (c->p.mode == CIR_MODE_PLAIN) ? run_plain(c, nframes, xyz_d, box_d, times)
                                            : run_batched(c, nframes, xyz_d, box_d, times);
-    if (hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
+    if (hipEventRecord(ev.b, c->stream) == hipSuccess && hipEventSynchronize(ev.b) == hipSuccess) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) c->seconds = ms * 1e-3;
+        if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) c->seconds = ms * 1e-3;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     c->prof.collect(c->phase_sec, c->phase_cnt, CIR_NPHASE);
     return st;
 } catch (...) {
