@@ -254,8 +254,8 @@ inline size_t tridiag_smem_elems(int n) {
 constexpr size_t kLdsBytes = 160 * 1024;  // LDS of a gfx950 CU, all of it one workgroup's
 // Panel rows the FP32 kernel keeps in LDS: what the rest of the 160 KiB holds
 // (rows j0 .. j0 + rl - 1 of the panel at j0; the fused pass reads them every
-// reflector), at least the 33 rows a panel's row j and j + 1 reads need, else
-// none.
+// reflector).  Fewer than 64 (a panel's rows j and j + 1 alone need 33) and
+// the head is not worth its copy-out: then none.
 template <typename T>
 inline int tridiag_panel_rows(int n) {
     if (sizeof(T) != 4) return 0;
