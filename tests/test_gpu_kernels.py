@@ -136,6 +136,42 @@ def test_eigh_small_cases(dtype):
         assert orth < orth_tol, (name, orth)
 
 
+@pytest.mark.parametrize("prec", [C.PREC_FP32, C.PREC_BF16X3], ids=["fp32", "bf16x3"])
+def test_eigh_batched_panel_split_between_lds_and_global(prec):
+    # 17 matrices of n = 700 (one more than the whole-GPU path takes) run one
+    # workgroup per matrix; the FP32 kernel keeps the first ~598 rows of each
+    # panel in LDS and the rest in the global panel, so every panel before
+    # the last ~600 rows has both kinds of rows.  Hamiltonian-like spectra
+    # (diagonal spread 600, couplings ~30) against LAPACK.
+    rng = np.random.default_rng(700)
+    batch, n = 17, 700
+    H = np.zeros((batch, n, n))
+    for b in range(batch):
+        a = rng.standard_normal((n, n)) * 30.0
+        H[b] = (a + a.T) / 2 + np.diag(rng.uniform(-300.0, 300.0, n))
+    Hc = H.astype(np.float32).astype(np.float64)
+    announce("eigh batched n=%d batch=%d prec=%d" % (n, batch, prec))
+    Ht = torch.from_numpy(np.ascontiguousarray(np.transpose(H, (0, 2, 1)))).to(torch.float32).to(DEV).contiguous()
+    w, V = S.eigh(Ht, n, prec=prec)
+    w = w.cpu().numpy()
+    V = V.cpu().double().numpy()
+    eps = 6e-8
+    # BF16x3 forms the back-transform from split-BF16 products (2^-16
+    # relative per product term): residual and orthogonality held to 3e-4,
+    # as in the 2048-OH back-transform test
+    vtol = 50 * eps * np.sqrt(n) if prec == C.PREC_FP32 else 3e-4
+    for b in range(batch):
+        wn = np.linalg.eigvalsh(Hc[b])
+        scale = np.abs(wn).max()
+        assert np.all(np.diff(w[b]) >= 0), b
+        assert np.abs(w[b] - wn).max() <= 50 * eps * scale * np.sqrt(n), b
+        Vb = V[b][:, :n]
+        res = np.abs(Hc[b] @ Vb - Vb * w[b]).max() / scale
+        orth = np.abs(Vb.T @ Vb - np.eye(n)).max()
+        assert res < vtol, (b, res)
+        assert orth < vtol, (b, orth)
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_eigh_hamiltonians_vs_oracle(dtype):
     xyz, box, _ = frames(128, 4, 1001)
