@@ -227,3 +227,22 @@ def test_corrupt_headers_and_coordinate_blocks_are_errors(tmp_path):
     assert L.cir_xtc_times(os.fsencode(g), 2, t.ctypes.data, None) != 0
     # the untouched file still reads
     assert C.xtc_scan(f) == (64, 3)
+
+
+def test_cli_rejects_a_corrupt_trajectory_without_crashing(tmp_path):
+    # the drop-in executable reads the trajectory before it touches the GPU:
+    # a coordinate block whose z range is empty ends the run with
+    # EXIT_FAILURE and a message, not a signal
+    import subprocess
+    xyz, box, times = C.synth_trajectory(16, 3, 5)
+    f = tmp_path / "ok.xtc"
+    C.xtc_write(f, xyz, box, times)
+    data = f.read_bytes()
+    g = _patched(tmp_path, "bad.xtc", data, 80, struct.unpack(">i", data[68:72])[0] - 1)
+    inp = tmp_path / "input.inp"
+    inp.write_text("xtcf %s\noutf %s\nmodel tip4p2005\nspecies H2O\nntcfpoints 2\nnsamples 1\n" % (g, tmp_path / "o"))
+    r = subprocess.run([C.CLI_PATH, str(inp)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=60)
+    assert r.returncode == 1, (r.returncode, r.stdout[-500:])
+    assert "could not be opened" in r.stdout
+    assert not list(tmp_path.glob("o_*.dat"))
