@@ -61,6 +61,7 @@ class Params(ctypes.Structure):
         ("chunk", ctypes.c_int), ("ckpt_every", ctypes.c_int), ("resume", ctypes.c_int),
         ("max_frames", ctypes.c_int), ("adj_store", ctypes.c_int), ("triclinic", ctypes.c_int),
         ("reunit_every", ctypes.c_int), ("origin_stride", ctypes.c_int),
+        ("oh_per_molecule", ctypes.c_int), ("msite_bisector", ctypes.c_int), ("time_prec", ctypes.c_float),
     ]
 
 
@@ -110,6 +111,7 @@ SIGNATURES = {
     "cir_xtc_close": (None, [_vp]),
     "cir_xtc_offsets": (_i, [ctypes.c_char_p, _i, _vp]),
     "cir_xtc_seek": (_i, [_vp, _i]),
+    "cir_xtc_precision": (_i, [ctypes.c_char_p, _i, ctypes.POINTER(_f)]),
     "cir_xtc_write_full": (_i, [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _f, _i]),
     "cir_main": (_i, [_i, ctypes.POINTER(ctypes.c_char_p)]),
     "cir_device_count": (_i, [ctypes.POINTER(_i)]),
@@ -192,6 +194,8 @@ def default_params(**kw):
             k = "reunit_every"
         if k == "origin_stride_frames":
             k = "origin_stride"
+        if k == "time_precision":
+            k = "time_prec"
         if k == "precision" and isinstance(v, str):
             v = {"fp32": PREC_FP32, "fp64": PREC_FP64, "bf16x3": PREC_BF16X3}[v]
         if k == "mode" and isinstance(v, str):
@@ -283,6 +287,14 @@ def xtc_times(path):
     return times, box
 
 
+def xtc_precision(path, frame=1):
+    """Coordinate precision stored in a frame of an XTC file (-1 for frames of
+    at most 9 atoms, which carry none)."""
+    v = ctypes.c_float()
+    check(lib().cir_xtc_precision(os.fsencode(path), int(frame), ctypes.byref(v)))
+    return v.value
+
+
 def xtc_offsets(path):
     """Byte offset of every frame of an XTC file (cir_xtc_offsets)."""
     _, nframes = xtc_scan(path)
@@ -349,6 +361,29 @@ def map_coefs(imap, ispecies):
     return c
 
 
+def check_trajectory(p, xyz, box, times):
+    """Host arrays of a trajectory as float32 contiguous numpy arrays, checked
+    against what the C ABI reads through their pointers: xyz [nframes, natoms,
+    3] with natoms a multiple of natom_mol; box [nframes, 3] (the diagonal),
+    or with the triclinic key [nframes, 9] / [nframes, 3, 3]; times
+    [nframes].  Raises ValueError otherwise."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    box = np.ascontiguousarray(box, np.float32)
+    times = np.ascontiguousarray(times, np.float32)
+    if xyz.ndim != 3 or xyz.shape[2] != 3 or xyz.shape[0] < 1 or xyz.shape[1] < 1:
+        raise ValueError("xyz must be [nframes, natoms, 3], got %s" % (xyz.shape,))
+    nframes, natoms = xyz.shape[0], xyz.shape[1]
+    if p.natom_mol <= 0 or natoms % p.natom_mol:
+        raise ValueError("natoms %d is not a multiple of natom_mol %d" % (natoms, p.natom_mol))
+    width = 9 if p.triclinic else 3
+    if box.shape[0] != nframes or box.reshape(nframes, -1).shape[1] != width:
+        raise ValueError("box must hold %d floats per frame for %d frames (triclinic %d), got %s"
+                         % (width, nframes, p.triclinic, box.shape))
+    if times.ndim != 1 or times.size != nframes:
+        raise ValueError("times must hold one value per frame (%d), got %s" % (nframes, times.shape))
+    return xyz, box, times
+
+
 def run(p, xyz, box, times, stream=None, profile=False, coefs=None):
     """The whole calcIR job (every sample, calcIR.cu:360-919) on one GPU.
 
@@ -356,9 +391,7 @@ def run(p, xyz, box, times, stream=None, profile=False, coefs=None):
     times [nframes] ps.  Returns a dict with complex TCFs (decayed and
     1/nsamples scaled), the C2R outputs, Sw, Pw, Rw, Rmw and counters.
     coefs (13 doubles, COEF_NAMES order) replaces the map table row."""
-    xyz = np.ascontiguousarray(xyz, np.float32)
-    box = np.ascontiguousarray(box, np.float32)
-    times = np.ascontiguousarray(times, np.float32)
+    xyz, box, times = check_trajectory(p, xyz, box, times)
     nframes, natoms = xyz.shape[0], xyz.shape[1]
     L = lib()
     ctx = ctypes.c_void_p()
@@ -410,6 +443,11 @@ def run_streamed(p, times, blocks, natoms, profile=False, coefs=None):
     use are copied to the device; the trajectory is never resident.  Returns
     what run() returns."""
     times = np.ascontiguousarray(times, np.float32)
+    if times.ndim != 1 or times.size < 2:
+        raise ValueError("times must hold one value per trajectory frame")
+    if int(natoms) <= 0 or p.natom_mol <= 0 or int(natoms) % p.natom_mol:
+        raise ValueError("natoms must be a positive multiple of natom_mol")
+    width = 9 if p.triclinic else 3
     L = lib()
     ctx = ctypes.c_void_p()
     check(L.cir_create(ctypes.byref(p), int(natoms), None, ctypes.byref(ctx)))
@@ -427,6 +465,11 @@ def run_streamed(p, times, blocks, natoms, profile=False, coefs=None):
             xyz = np.ascontiguousarray(xyz, np.float32)
             box = np.ascontiguousarray(box, np.float32)
             k = xyz.shape[0]
+            if xyz.shape != (k, int(natoms), 3) or box.shape[0] != k or box.reshape(k, -1).shape[1] != width:
+                raise ValueError("block shapes %s / %s do not match natoms %d and %d box floats per frame"
+                                 % (xyz.shape, box.shape, int(natoms), width))
+            if first + k > times.size:
+                raise ValueError("more frames pushed than times given")
             check(L.cir_stream_push(ctx, first, k, xyz.ctypes.data, box.ctypes.data))
             first += k
         check(L.cir_stream_end(ctx))
@@ -445,9 +488,7 @@ def run_many(jobs, nthreads=8):
     keep = []
     try:
         for j, (p, xyz, box, times) in enumerate(jobs):
-            xyz = np.ascontiguousarray(xyz, np.float32)
-            box = np.ascontiguousarray(box, np.float32)
-            times = np.ascontiguousarray(times, np.float32)
+            xyz, box, times = check_trajectory(p, xyz, box, times)
             keep.append((p, xyz, box, times))
             c = ctypes.c_void_p()
             check(L.cir_create(ctypes.byref(p), xyz.shape[1], None, ctypes.byref(c)))

@@ -18,7 +18,7 @@ import functools
 
 import numpy as np
 
-from . import NCOEF, check, lib
+from . import NCOEF, check, check_trajectory, lib
 
 __all__ = ["LineshapeSession", "IRLineshape", "lineshape_map", "lineshape_rows"]
 
@@ -57,10 +57,12 @@ class LineshapeSession:
 
     def __init__(self, p, xyz, box, times, stream=None):
         self.p = p
-        self.xyz = np.ascontiguousarray(xyz, np.float32)
-        self.box = np.ascontiguousarray(box, np.float32)
-        self.times = np.ascontiguousarray(times, np.float32)
+        self.xyz, self.box, self.times = check_trajectory(p, xyz, box, times)
         self.nfft = p.ntcfpoints + p.nzeros
+        # forward passes run so far and the coefficients of the last one:
+        # cir_backward_ls differentiates the context's last forward pass
+        self.generation = 0
+        self.forward_coefs = None
         self._ctx = ctypes.c_void_p()
         check(lib().cir_create(ctypes.byref(p), self.xyz.shape[1], stream, ctypes.byref(self._ctx)))
 
@@ -89,11 +91,15 @@ class LineshapeSession:
                 raise ValueError("coefs must hold %d values" % NCOEF)
             check(lib().cir_set_coefs(self._ctx, cf.ctypes.data))
         ft = np.zeros(self.nfft)
+        self.forward_coefs = None
         check(lib().cir_forward_ls(self._ctx, self.xyz.shape[0], self.xyz.ctypes.data, self.box.ctypes.data,
                                    self.times.ctypes.data, ft.ctypes.data))
+        self.generation += 1
+        self.forward_coefs = self.coefs()
         return ft
 
     def backward(self, dL_dft):
+        """dL/dcoef at the coefficients of the last forward pass."""
         g = np.ascontiguousarray(dL_dft, np.float64)
         if g.shape != (self.nfft,):
             raise ValueError("dL_dft must hold %d values" % self.nfft)
@@ -113,17 +119,29 @@ def _function():
 
         @staticmethod
         def forward(ctx, coefs, session):
-            ft = session.forward(coefs.detach().cpu().numpy())
+            cf = coefs.detach().cpu().numpy().astype(np.float64)
+            ft = session.forward(cf)
             idx, _, scale = lineshape_map(session.p)
             ctx.session = session
+            ctx.coefs = cf
+            ctx.generation = session.generation
+            ctx.device = coefs.device
             ctx.idx, ctx.scale = idx, scale
-            return torch.as_tensor(ft[idx] * scale, dtype=torch.float64)
+            return torch.as_tensor(ft[idx] * scale, dtype=torch.float64, device=coefs.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            g = np.zeros(ctx.session.nfft)
+            sess = ctx.session
+            # the session differentiates its last forward pass: if another
+            # forward ran since this one, run this one's again first
+            if sess.generation != ctx.generation or sess.forward_coefs is None or \
+                    not np.array_equal(sess.forward_coefs, ctx.coefs):
+                sess.forward(ctx.coefs)
+                ctx.generation = sess.generation
+            g = np.zeros(sess.nfft)
             np.add.at(g, ctx.idx, grad_out.detach().cpu().numpy().astype(np.float64) * ctx.scale)
-            return torch.as_tensor(ctx.session.backward(g), dtype=torch.float64), None
+            grad = torch.as_tensor(sess.backward(g), dtype=torch.float64)
+            return grad.to(ctx.device), None
 
     return IRLineshape
 

@@ -452,7 +452,7 @@ int forward_impl(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d
         const int bs = box_floats(c);
         CIR_TRY(gather_frames(c, xyz_d, box_d, S.uniq.data() + c0, nb, xyzc.as<float>(), boxes + bs * size_t(c0)));
         void* sc = Sb + size_t(c0) * CIR_NSITE * N * esz;
-        CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc, p.cutoff,
+        CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, site_rules(p), c->mc, p.cutoff,
                             xyzc.as<float>(), boxes + bs * size_t(c0), c->boxk, sc, st));
         const int v0 = A->store ? c0 : 0;  // slot of frame c0 in V, w
         CIR_TRY(hamiltonian(rp, nb, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, boxes + bs * size_t(c0),

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -134,6 +135,13 @@ extern "C" int cir_main(int argc, char** argv) try {
         std::vector<float> box9(size_t(nframes) * 9);
         st = cir_xtc_times(p.xtcf, nframes, times.data(), box9.data());
         if (st != CIR_OK) return fail_msg(st);
+        // calcIR.cu:342-345 rounds the frame spacing at the file's precision,
+        // as read with frame 1 (files of at most 9 atoms store none: keep the
+        // default)
+        float xprec = -1.0f;
+        st = cir_xtc_precision(p.xtcf, 1, &xprec);
+        if (st != CIR_OK) return fail_msg(st);
+        if (xprec > 0.0f) p.time_prec = xprec;
         // rectangular boxes go through as their diagonal (calcIR.cu:404-405);
         // any off-diagonal element switches to the triclinic minimum image
         bool tri = p.triclinic != 0;
@@ -158,7 +166,9 @@ extern "C" int cir_main(int argc, char** argv) try {
     std::printf(">>> Will read the trajectory from: %s.\n", p.xtcf);
     std::printf(">>> Found %d atoms and %d molecules.\n", natoms, nmol);
     std::printf(">>> Found %d chromophores.\n", nchrom);
-    std::printf(">>> Frame time offset is: %f (ps)\n", times[1] - times[0]);
+    // calcIR.cu:345-346: the rounded spacing the schedule uses
+    std::printf(">>> Frame time offset is: %f (ps)\n",
+                float(std::round((times[1] - times[0]) * p.time_prec) / (1. * p.time_prec)));
     std::printf("\n>>> Now calculating the absorption spectrum\n");
     std::printf("----------------------------------------------------------\n");
     std::fflush(stdout);

@@ -8,7 +8,7 @@
 //   CIR_PREC_FP32   FP32 MFMA (v_mfma_f32_32x32x2f32), FP32 accumulation
 //   CIR_PREC_BF16X3 split-BF16 x 3 MFMA (v_mfma_f32_32x32x16_bf16):
 //                   a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation
-//   CIR_PREC_FP64   FP64 VALU register-tiled product
+//   CIR_PREC_FP64   FP64 MFMA (v_mfma_f64_16x16x4f64), 64x64 tiles
 #pragma once
 
 #include "common.hpp"

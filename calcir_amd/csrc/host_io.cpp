@@ -60,6 +60,9 @@ extern "C" void cir_params_default(cir_params* p) {
     p->triclinic = 0;
     p->reunit_every = 0;
     p->origin_stride = 0;
+    p->oh_per_molecule = 0;
+    p->msite_bisector = 0;
+    p->time_prec = 1000.0f;
 }
 
 extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) try {
@@ -105,6 +108,9 @@ extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) try
         else if (!std::strcmp(para, "triclinic")) std::sscanf(value, "%d", &p->triclinic);
         else if (!std::strcmp(para, "reunitarise_every")) std::sscanf(value, "%d", &p->reunit_every);
         else if (!std::strcmp(para, "origin_stride_frames")) std::sscanf(value, "%d", &p->origin_stride);
+        else if (!std::strcmp(para, "oh_per_molecule")) std::sscanf(value, "%d", &p->oh_per_molecule);
+        else if (!std::strcmp(para, "msite_bisector")) std::sscanf(value, "%d", &p->msite_bisector);
+        else if (!std::strcmp(para, "time_precision")) std::sscanf(value, "%f", &p->time_prec);
         else if (!std::strcmp(para, "precision")) {
             if (!std::strcmp(value, "fp64")) p->precision = CIR_PREC_FP64;
             else if (!std::strcmp(value, "bf16x3")) p->precision = CIR_PREC_BF16X3;
@@ -150,6 +156,8 @@ extern "C" void cir_params_echo(const cir_params* p) {
         std::printf("\tCheckpoint %s_ckpt.bin: every %d frames, resume %d, max_frames %d\n", p->cptf, p->ckpt_every,
                     p->resume, p->max_frames);
     if (p->origin_stride > 0) std::printf("\tTime origins every %d frames\n", p->origin_stride);
+    if (p->oh_per_molecule) std::printf("\tChromophore k of a molecule is its O-H%s stretch\n", "(1+k)");
+    if (p->msite_bisector) std::printf("\tThree-site water: -1.04 site on the HOH bisector\n");
 }
 
 extern "C" int cir_resolve_model(const char* model, int* imodel) try {

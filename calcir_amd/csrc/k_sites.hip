@@ -36,11 +36,11 @@ __device__ __forceinline__ bool within_cutoff(float ox, float oy, float oz, floa
 
 template <typename T>
 __global__ void __launch_bounds__(kFieldBlock)
-eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float cutoff,
+eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, int rules, float cutoff,
                    const float* __restrict__ xyz, const float* __restrict__ box, int boxk, MapT<T> mc,
                    T* __restrict__ sites) {
     __shared__ float sO[3][kFieldTile];
-    __shared__ T sA[9][kFieldTile];  // H1, H2, M' (x,y,z each)
+    __shared__ T sA[9][kFieldTile];  // H1, H2, negative site (x,y,z each)
 
     const int f = blockIdx.y;
     const int nmol = natoms / natom_mol;
@@ -48,14 +48,20 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
     const float* x = xyz + size_t(f) * natoms * 3;
     const PBox<float> Bf = load_box<float>(box, boxk, f);
     const PBox<T> Bt = load_box<T>(box, boxk, f);
-    const bool have_m = natom_mol >= 4;
+    // calcIR.cu:1117-1160: the charged sites of molecule m are its atoms
+    // 1..natom_mol-1, +0.52 on atoms 1 and 2 and -1.04 on atom 3 (later atoms
+    // carry nothing), so three-site water has no negative site.  The
+    // msite_bisector rule builds one for it instead.
+    const bool built_m = (rules & kRuleMsiteBisector) && natom_mol == 3;
+    const bool have_m = natom_mol >= 4 || built_m;
 
     const int chrom = blockIdx.x * kFieldBlock + threadIdx.x;
     const bool active = chrom < N;
     const int n = active ? chrom / nchrom_mol : 0;
-    // calcIR.cu:1061-1070 picks H1/H2 by chrom%2; with one OH per molecule
-    // (nchrom_mol 1) every chromophore is H1.  Identical for nchrom_mol 2.
-    const int hatom = 1 + chrom % nchrom_mol;
+    // calcIR.cu:1061-1070 picks H1 or H2 by chrom % 2 whatever nchrom_mol is
+    // (with nchrom_mol 1, even molecules use H1 and odd ones H2); the
+    // oh_per_molecule rule takes H(1 + chrom % nchrom_mol).
+    const int hatom = 1 + ((rules & kRuleOhPerMolecule) ? chrom % nchrom_mol : chrom % 2);
     float hxf = 0, hyf = 0, hzf = 0;
     if (active) {
         hxf = x[3 * (n * natom_mol + hatom) + 0];
@@ -80,21 +86,9 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
                 sA[3 + c][t] = T(mx[6 + c]);
             }
             const T o[3] = {T(ox), T(oy), T(oz)};
-            T m[3];
-            if (have_m) {
-                m[0] = T(mx[9]);
-                m[1] = T(mx[10]);
-                m[2] = T(mx[11]);
-                if (imodel != 0) {
-                    // calcIR.cu:1124-1140: M moved to 0.015 nm along O->M
-                    T om[3] = {m[0] - o[0], m[1] - o[1], m[2] - o[2]};
-                    minimage3<T>(om[0], om[1], om[2], Bt);
-                    const T rom = sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
-                    for (int c = 0; c < 3; ++c) m[c] = o[c] + T(kOMTip4p) * om[c] / rom;
-                }
-            } else {
-                // three-site water: the negative site is built on the HOH
-                // bisector, 0.015 nm from O (the TIP4P geometry)
+            T m[3] = {T(0), T(0), T(0)};
+            if (built_m) {
+                // the TIP4P negative site on the HOH bisector, 0.015 nm from O
                 T h1[3] = {T(mx[3]) - o[0], T(mx[4]) - o[1], T(mx[5]) - o[2]};
                 T h2[3] = {T(mx[6]) - o[0], T(mx[7]) - o[1], T(mx[8]) - o[2]};
                 minimage3<T>(h1[0], h1[1], h1[2], Bt);
@@ -105,6 +99,17 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
                 for (int c = 0; c < 3; ++c) bi[c] = h1[c] / r1 + h2[c] / r2;
                 const T rb = sqrt(bi[0] * bi[0] + bi[1] * bi[1] + bi[2] * bi[2]);
                 for (int c = 0; c < 3; ++c) m[c] = o[c] + T(kOMTip4p) * bi[c] / rb;
+            } else if (have_m) {
+                m[0] = T(mx[9]);
+                m[1] = T(mx[10]);
+                m[2] = T(mx[11]);
+                if (imodel != 0) {
+                    // calcIR.cu:1124-1140: M moved to 0.015 nm along O->M
+                    T om[3] = {m[0] - o[0], m[1] - o[1], m[2] - o[2]};
+                    minimage3<T>(om[0], om[1], om[2], Bt);
+                    const T rom = sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+                    for (int c = 0; c < 3; ++c) m[c] = o[c] + T(kOMTip4p) * om[c] / rom;
+                }
             }
             for (int c = 0; c < 3; ++c) sA[6 + c][t] = m[c];
         }
@@ -114,7 +119,7 @@ eproj_sites_kernel(int natoms, int natom_mol, int nchrom_mol, int imodel, float 
             const int m = m0 + t;
             if (m == n) continue;  // calcIR.cu:1099
             if (!within_cutoff(sO[0][t], sO[1][t], sO[2][t], hxf, hyf, hzf, Bf, cutoff)) continue;
-            for (int s = 0; s < 3; ++s) {
+            for (int s = 0; s < (have_m ? 3 : 2); ++s) {
                 T dx = hx - sA[3 * s + 0][t], dy = hy - sA[3 * s + 1][t], dz = hz - sA[3 * s + 2][t];
                 minimage3<T>(dx, dy, dz, Bt);
                 dx = dx * bohr;
@@ -235,13 +240,13 @@ hamiltonian_kernel(int N, int ld, int nchrom_mol, int hod, T avef, T wi0, T wi1,
 }
 
 template <typename T>
-int launch_eproj_sites(int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel,
+int launch_eproj_sites(int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, int rules,
                        float cutoff, const float* xyz, const float* box, int boxk, const MapCoef& mc,
                        void* sites, hipStream_t s) {
     const int N = (natoms / natom_mol) * nchrom_mol;
     dim3 grid(unsigned(ceil_div(N, kFieldBlock)), unsigned(nframes));
     hipLaunchKernelGGL(eproj_sites_kernel<T>, grid, dim3(kFieldBlock), 0, s, natoms, natom_mol,
-                       nchrom_mol, imodel, cutoff, xyz, box, boxk, map_t<T>(mc), static_cast<T*>(sites));
+                       nchrom_mol, imodel, rules, cutoff, xyz, box, boxk, map_t<T>(mc), static_cast<T*>(sites));
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
@@ -258,17 +263,23 @@ int launch_hamiltonian(int nframes, int N, int ld, int nchrom_mol, int ispecies,
     return CIR_OK;
 }
 
-int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, const MapCoef& mc,
-                float cutoff, const float* xyz_d, const float* box_d, int boxk, void* sites_d, hipStream_t s) {
-    if (nframes <= 0 || natom_mol < 3 || nchrom_mol < 1 || nchrom_mol > 2 || natoms <= 0 || natoms % natom_mol ||
-        !xyz_d || !box_d || !sites_d || (boxk != 0 && boxk != 1))
+int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, int rules,
+                const MapCoef& mc, float cutoff, const float* xyz_d, const float* box_d, int boxk, void* sites_d,
+                hipStream_t s) {
+    // atoms 1 and 2 of a molecule are its hydrogens (calcIR.cu:1061-1070)
+    if (nframes <= 0 || natom_mol < 3 || nchrom_mol < 1 || natoms <= 0 || natoms % natom_mol || !xyz_d ||
+        !box_d || !sites_d || (boxk != 0 && boxk != 1) || (rules & ~(kRuleOhPerMolecule | kRuleMsiteBisector)))
         return fail(CIR_ERR_ARG, "cir_eproj_sites: bad sizes or null pointer");
+    if ((rules & kRuleOhPerMolecule) && nchrom_mol > 2)
+        return fail(CIR_ERR_ARG, "cir_eproj_sites: one or two OH stretches per molecule with oh_per_molecule");
     if (int64_t(nframes) > 65535) return fail(CIR_ERR_ARG, "cir_eproj_sites: nframes > 65535");
+    if (int64_t(natoms / natom_mol) * nchrom_mol > (int64_t(1) << 30))
+        return fail(CIR_ERR_ARG, "cir_eproj_sites: too many chromophores");
     if (prec == CIR_PREC_FP64)
-        return launch_eproj_sites<double>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, boxk,
-                                          mc, sites_d, s);
-    return launch_eproj_sites<float>(nframes, natoms, natom_mol, nchrom_mol, imodel, cutoff, xyz_d, box_d, boxk,
-                                     mc, sites_d, s);
+        return launch_eproj_sites<double>(nframes, natoms, natom_mol, nchrom_mol, imodel, rules, cutoff, xyz_d,
+                                          box_d, boxk, mc, sites_d, s);
+    return launch_eproj_sites<float>(nframes, natoms, natom_mol, nchrom_mol, imodel, rules, cutoff, xyz_d, box_d,
+                                     boxk, mc, sites_d, s);
 }
 
 int hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies, const MapCoef& mc, float avef,
@@ -293,8 +304,8 @@ extern "C" int cir_eproj_sites(int prec, int nframes, int natoms, int natom_mol,
                                void* stream) {
     MapCoef mc;
     if (!map_coef(imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map/species not in the map table");
-    return eproj_sites(prec, nframes, natoms, natom_mol, nchrom_mol, imodel, mc, cutoff, xyz_d, box_d, 0, sites_d,
-                       as_stream(stream));
+    return eproj_sites(prec, nframes, natoms, natom_mol, nchrom_mol, imodel, 0, mc, cutoff, xyz_d, box_d, 0,
+                       sites_d, as_stream(stream));
 }
 
 extern "C" int cir_hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies,

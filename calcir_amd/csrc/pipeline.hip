@@ -37,9 +37,13 @@ namespace cir {
 int build_schedule(const cir_params& p, int nframes, const float* times, Schedule& S) {
     const int ns = p.nsamples, nt = p.ntcfpoints;
     if (nframes < 2) return fail(CIR_ERR_FRAME, "at least two frames are needed to find the frame spacing");
-    // calcIR.cu:342-345 (XTC precision 1000)
-    const float frame_dt = float(std::round((times[1] - times[0]) * 1000.0f) / (1. * 1000.0f));
-    if (!(frame_dt > 0.0f)) return fail(CIR_ERR_FRAME, "trajectory times are not increasing");
+    // calcIR.cu:342-345: the spacing of the first two frames rounded at the
+    // trajectory's precision (float product, rounded, divided in double)
+    const float prec = p.time_prec;
+    if (!(prec > 0.0f)) return fail(CIR_ERR_ARG, "time_precision must be positive");
+    const float frame_dt = float(std::round((times[1] - times[0]) * prec) / (1. * prec));
+    if (!(frame_dt > 0.0f))
+        return fail(CIR_ERR_FRAME, "trajectory times are not increasing at the time precision");
     S.fidx.assign(size_t(ns) * nt, 0);
     for (int s = 0; s < ns; ++s)
         for (int k = 0; k < nt; ++k) {
@@ -280,6 +284,9 @@ int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d
     f.val(p.triclinic);
     f.val(p.reunit_every);
     f.val(p.origin_stride);
+    f.val(p.oh_per_molecule);
+    f.val(p.msite_bisector);
+    f.val(p.time_prec);
     f.val(c->mc);
     f.val(S.nslot);
     f.val(int64_t(S.uniq.size()));
@@ -442,7 +449,7 @@ struct BatchRun {
         double* accp = acc.as<double>();
         {
             ProfScope ps(pf, CIR_PH_SITES, st);
-            CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc, p.cutoff,
+            CIR_TRY(eproj_sites(rp, nb, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, site_rules(p), c->mc, p.cutoff,
                                 xyzc.as<float>(), boxc.as<float>(), c->boxk, sites.p, st));
         }
         {
@@ -627,7 +634,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
             Prof* pf = &c->prof;
             {
                 ProfScope ps(pf, CIR_PH_SITES, st);
-                CIR_TRY(eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, c->mc,
+                CIR_TRY(eproj_sites(rp, 1, c->natoms, p.natom_mol, p.nchrom_mol, c->imodel, site_rules(p), c->mc,
                                         p.cutoff, xyz1.as<float>(), box1.as<float>(), c->boxk, sites1.p, st));
             }
             {
@@ -709,10 +716,15 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     if (!map_coef(p->imap, ispecies, &mc)) return fail(CIR_ERR_MAP, "map must be 0 or 1");
     if (p->natom_mol < 3 || p->nchrom_mol < 1 || natoms <= 0 || natoms % p->natom_mol)
         return fail(CIR_ERR_ARG, "natoms must be a positive multiple of natom_mol (>= 3)");
-    if (p->nchrom_mol > 2) return fail(CIR_ERR_ARG, "nchrom_mol must be 1 or 2 (OH stretches of a water)");
+    if (p->oh_per_molecule != 0 && p->oh_per_molecule != 1) return fail(CIR_ERR_ARG, "oh_per_molecule must be 0 or 1");
+    if (p->msite_bisector != 0 && p->msite_bisector != 1) return fail(CIR_ERR_ARG, "msite_bisector must be 0 or 1");
+    if (p->oh_per_molecule && p->nchrom_mol > 2)
+        return fail(CIR_ERR_ARG, "oh_per_molecule needs nchrom_mol 1 or 2 (OH stretches of a water)");
     if (p->triclinic != 0 && p->triclinic != 1) return fail(CIR_ERR_ARG, "triclinic must be 0 or 1");
     if (p->reunit_every < 0) return fail(CIR_ERR_ARG, "reunitarise_every must be >= 0");
     if (p->origin_stride < 0) return fail(CIR_ERR_ARG, "origin_stride_frames must be >= 0");
+    if (!(p->time_prec > 0.0f) || !std::isfinite(p->time_prec))
+        return fail(CIR_ERR_ARG, "time_precision must be positive");
     if (p->ntcfpoints <= 0 || p->nsamples <= 0 || p->omegaStep <= 0 || p->nzeros < 0 || !(p->t1 > 0) || !(p->dt > 0))
         return fail(CIR_ERR_ARG, "ntcfpoints, nsamples, omegaStep, t1 and dt must be positive, nzeros >= 0");
     if (cir_nomega(p) <= 0) return fail(CIR_ERR_ARG, "omegaStop must not be below omegaStart");

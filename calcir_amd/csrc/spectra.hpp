@@ -10,9 +10,15 @@ namespace cir {
 // coefficient set (k_sites.hip); the cir_eproj_sites / cir_hamiltonian entry
 // points resolve the set from the map table.
 // boxk 0: box_d holds 3 floats (rectangular diagonal) per frame; 1: 9 floats
-// (triclinic a, b, c rows).
-int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, const MapCoef& mc,
-                float cutoff, const float* xyz_d, const float* box_d, int boxk, void* sites_d, hipStream_t s);
+// (triclinic a, b, c rows).  rules: 0 for the reference's H choice and charged
+// sites, or kRuleOhPerMolecule / kRuleMsiteBisector (site_rules()).
+enum { kRuleOhPerMolecule = 1, kRuleMsiteBisector = 2 };
+inline int site_rules(const cir_params& p) {
+    return (p.oh_per_molecule ? kRuleOhPerMolecule : 0) | (p.msite_bisector ? kRuleMsiteBisector : 0);
+}
+int eproj_sites(int prec, int nframes, int natoms, int natom_mol, int nchrom_mol, int imodel, int rules,
+                const MapCoef& mc, float cutoff, const float* xyz_d, const float* box_d, int boxk, void* sites_d,
+                hipStream_t s);
 int hamiltonian(int prec, int nframes, int n, int ld, int nchrom_mol, int ispecies, const MapCoef& mc, float avef,
                 const float* box_d, int boxk, const void* sites_d, void* H_d, hipStream_t s);
 

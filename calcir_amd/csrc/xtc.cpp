@@ -620,6 +620,49 @@ extern "C" int cir_xtc_times(const char* path, int nframes, float* times, float*
     return cir::host_exception("cir_xtc_times");
 }
 
+extern "C" int cir_xtc_precision(const char* path, int frame, float* precision) try {
+    if (!path || frame < 0 || !precision) return fail(CIR_ERR_ARG, "cir_xtc_precision: bad argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CIR_ERR_IO, std::string("could not open ") + path);
+    XdrIn x{f};
+    int st = CIR_OK;
+    for (int k = 0;; ++k) {
+        int nat, step;
+        float t, bx[9];
+        if (!read_header(x, &nat, &step, &t, bx)) {
+            st = fail(CIR_ERR_FRAME, "xtc: frame " + std::to_string(k) + " missing");
+            break;
+        }
+        if (k < frame) {
+            if (!skip_coords(x, nat)) {
+                st = fail(CIR_ERR_IO, "xtc: corrupt frame " + std::to_string(k));
+                break;
+            }
+            continue;
+        }
+        const int lsize = x.i32();
+        if (!x.ok || lsize != nat) {
+            st = fail(CIR_ERR_IO, "xtc: corrupt frame " + std::to_string(k));
+            break;
+        }
+        if (nat <= 9) {
+            *precision = -1.0f;  // stored uncompressed, as xdrfile reports it
+        } else {
+            const float pr = x.f32();
+            if (!x.ok || !(pr > 0.0f)) {
+                st = fail(CIR_ERR_IO, "xtc: corrupt precision in frame " + std::to_string(k));
+                break;
+            }
+            *precision = pr;
+        }
+        break;
+    }
+    std::fclose(f);
+    return st;
+} catch (...) {
+    return cir::host_exception("cir_xtc_precision");
+}
+
 extern "C" int cir_xtc_scan(const char* path, int* natoms, int* nframes) try {
     if (!path || !natoms || !nframes) return fail(CIR_ERR_ARG, "cir_xtc_scan: null argument");
     FILE* f = std::fopen(path, "rb");

@@ -135,6 +135,25 @@ typedef struct cir_params {
     int   origin_stride;       /* key "origin_stride_frames": > 0 starts sample s this many
                                   trajectory frames after sample s-1 (sampleEvery =
                                   origin_stride * frame spacing); 0 = sampleEvery as given */
+    int   oh_per_molecule;     /* key "oh_per_molecule": 1 = chromophore k of a molecule is
+                                  the O-H(1+k) stretch (nchrom_mol 1 or 2), so with
+                                  nchrom_mol 1 every molecule contributes its H1 stretch;
+                                  0 (default) = the reference's rule, H1 or H2 by chrom % 2
+                                  whatever nchrom_mol is (calcIR.cu:1061-1070, 1266-1275,
+                                  1360-1370): with nchrom_mol 1, even molecules use H1 and
+                                  odd molecules H2                                      */
+    int   msite_bisector;      /* key "msite_bisector": 1 = with natom_mol 3, a -1.04 charge
+                                  on the HOH bisector 0.015 nm from O (the TIP4P M site)
+                                  enters the field; 0 (default) = the reference's charged
+                                  sites, atoms 1..natom_mol-1 with +0.52 on atoms 1, 2 and
+                                  -1.04 on atom 3 (calcIR.cu:1117-1160), so three-site
+                                  water carries only its two H charges                  */
+    float time_prec;           /* key "time_precision": the frame spacing is the first two
+                                  frames' time difference rounded at this precision,
+                                  round(dt * prec) / prec (calcIR.cu:342-345, where prec is
+                                  the XTC file's own coordinate precision).  Default 1000
+                                  (GROMACS' default); the CLI sets it from the file
+                                  (cir_xtc_precision)                                   */
 } cir_params;
 
 /* Results of a run, all arrays caller-allocated (host).  NULL members are
@@ -231,6 +250,10 @@ void  cir_xtc_close(cir_xtc* reader);
  * index is built on its first call), after which cir_xtc_next reads from
  * there; CIR_ERR_FRAME past the last frame. */
 int   cir_xtc_offsets(const char* path, int nframes, int64_t* offsets);
+/* The coordinate precision stored in frame `frame` (the prec that xdrfile's
+ * read_xtc returns, calcIR.cu:342-345); -1 for frames of at most 9 atoms,
+ * which the format stores uncompressed without a precision. */
+int   cir_xtc_precision(const char* path, int frame, float* precision);
 int   cir_xtc_seek(cir_xtc* reader, int frame);
 /* The drop-in CLI: argv as calcIR.exe (calcIR.cu:9-18). */
 int   cir_main(int argc, char** argv);

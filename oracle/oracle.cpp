@@ -153,10 +153,22 @@ inline bool within_cutoff_f32(const float* mo, const float* nh, const Box<float>
     return !(r > cutoff);
 }
 
-// The H atom of a chromophore: calcIR.cu:1061-1070 takes H1 or H2 by
-// chrom%2; with one OH per molecule (nchrom_mol 1) it is always H1.  The two
+// The H atom of a chromophore: calcIR.cu:1061-1070 (and 1266-1275,
+// 1360-1370) takes H1 or H2 by chrom % 2 whatever nchrom_mol is, so with
+// nchrom_mol 1 molecule n uses H1 for even n and H2 for odd n.  The
+// oh_per_molecule variant takes H(1 + chrom % nchrom_mol) instead; the two
 // agree for nchrom_mol 2.
-inline int h_atom(const orc_config* cfg, int chrom) { return 1 + chrom % cfg->nchrom_mol; }
+inline int h_atom(const orc_config* cfg, int chrom) {
+    return 1 + (cfg->oh_per_molecule ? chrom % cfg->nchrom_mol : chrom % 2);
+}
+
+// Layouts the reference's indexing supports: atoms 1 and 2 of every
+// molecule are its hydrogens (calcIR.cu:1061-1070), so natom_mol >= 3; the
+// oh_per_molecule variant names at most two OH stretches per molecule.
+bool layout_ok(const orc_config* cfg, int natoms) {
+    if (cfg->natom_mol < 3 || cfg->nchrom_mol <= 0 || natoms <= 0 || natoms % cfg->natom_mol) return false;
+    return !(cfg->oh_per_molecule && cfg->nchrom_mol > 2);
+}
 
 int threads_of(const orc_config* cfg) {
 #ifdef _OPENMP
@@ -195,22 +207,26 @@ void eproj_t(const orc_config* cfg, const float* x, int natoms, const float* box
             const float* mof = &x[3 * (m * nam)];
             if (!within_cutoff_f32(mof, nhf, boxs, cfg->cutoff)) continue;
             T mo[3] = {T(mof[0]), T(mof[1]), T(mof[2])};
-            // sites 1, 2 (H) and 3 (M); a three-site water gets its M built
-            // on the HOH bisector 0.015 nm from O (the TIP4P geometry)
-            for (int i = 1; i <= 3; ++i) {
+            // calcIR.cu:1117-1160: atoms 1..natom_mol-1 of molecule m; atoms 1
+            // and 2 carry +0.52, atom 3 carries -1.04, later atoms nothing.
+            // The msite_bisector variant gives three-site water a -1.04 site
+            // on the HOH bisector 0.015 nm from O (the TIP4P geometry).
+            const bool built_m = cfg->msite_bisector && nam == 3;
+            const int last = built_m ? 3 : std::min(nam - 1, 3);
+            for (int i = 1; i <= last; ++i) {
                 T mx[3] = {T(0), T(0), T(0)};
                 if (i < nam) {
                     const float* maf = &x[3 * (m * nam + i)];
                     for (int c = 0; c < 3; ++c) mx[c] = T(maf[c]);
                 }
-                if (i == 3 && nam >= 4 && cfg->imodel != 0) {
+                if (i == 3 && !built_m && cfg->imodel != 0) {
                     // calcIR.cu:1124-1140: M moved to 0.015 nm along O->M
                     T om[3];
                     for (int c = 0; c < 3; ++c) om[c] = mx[c] - mo[c];
                     mi3<T>(om, box);
                     T rom = std::sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
                     for (int c = 0; c < 3; ++c) mx[c] = mo[c] + T(kOMTip4p) * om[c] / rom;
-                } else if (i == 3 && nam == 3) {
+                } else if (i == 3 && built_m) {
                     T h1[3], h2[3], bi[3];
                     for (int c = 0; c < 3; ++c) {
                         h1[c] = T(x[3 * (m * nam + 1) + c]) - mo[c];
@@ -610,7 +626,7 @@ int run_t(const orc_config* cfg, int natoms, int nframes, const float* xyz, cons
     using C = std::complex<T>;
     MapCoef mc;
     if (!cfg_coef(cfg, &mc)) return 2;
-    if (cfg->natom_mol <= 0 || cfg->nchrom_mol <= 0 || natoms % cfg->natom_mol) return 1;
+    if (!layout_ok(cfg, natoms)) return 1;
     if (nframes < 2 || cfg->ntcfpoints <= 0 || cfg->nsamples <= 0) return 1;
     const int nthreads = threads_of(cfg);
     const int nmol = natoms / cfg->natom_mol;
@@ -620,7 +636,9 @@ int run_t(const orc_config* cfg, int natoms, int nframes, const float* xyz, cons
     const int nfft = ntcf + cfg->nzeros;
 
     // calcIR.cu:342-345 with the XTC precision of 1000
-    const float frame_dt = float(std::round((times[1] - times[0]) * 1000.0f) / 1000.0);
+    // calcIR.cu:342-345: the first spacing rounded at the trajectory's precision
+    const float tprec = cfg->time_prec > 0.0f ? cfg->time_prec : 1000.0f;
+    const float frame_dt = float(std::round((times[1] - times[0]) * tprec) / (1. * tprec));
 
     std::vector<C> tcf(ntcf), tcfvv(ntcf), tcfvh(ntcf);
     std::vector<double> sw(nomega, 0.0), pw(nomega, 0.0), rw(nomega, 0.0), rmw(nomega, 0.0);
@@ -836,7 +854,7 @@ int orc_frame_index(const orc_config* cfg, int sample, int point, float frame_dt
 
 int orc_eproj(int prec, const orc_config* cfg, const float* x, int natoms, const float box[3],
               double* eproj) {
-    if (!cfg || !x || !eproj || cfg->natom_mol <= 0 || natoms % cfg->natom_mol) return 1;
+    if (!cfg || !x || !eproj || !layout_ok(cfg, natoms)) return 1;
     if (prec == ORC_F32)
         eproj_t<float>(cfg, x, natoms, box, eproj);
     else
@@ -848,6 +866,7 @@ int orc_sites(int prec, const orc_config* cfg, const float* x, int natoms, const
               const double* eproj, double* site) {
     MapCoef mc;
     if (!cfg || !cfg_coef(cfg, &mc)) return 2;
+    if (!x || !eproj || !site || !layout_ok(cfg, natoms)) return 1;
     if (prec == ORC_F32) {
         std::vector<Site<float>> s;
         sites_t<float>(cfg, mc, x, natoms, box, eproj, s);
@@ -864,6 +883,7 @@ int orc_hamiltonian(int prec, const orc_config* cfg, const float* x, int natoms,
                     const double* eproj, double* H) {
     MapCoef mc;
     if (!cfg || !cfg_coef(cfg, &mc)) return 2;
+    if (!x || !eproj || !H || !layout_ok(cfg, natoms)) return 1;
     const int N = (natoms / cfg->natom_mol) * cfg->nchrom_mol;
     if (prec == ORC_F32) {
         std::vector<Site<float>> s;

@@ -91,6 +91,20 @@ typedef struct orc_config {
     /* box arrays hold 9 floats per frame (rows a, b, c of a GROMACS box, a
      * along x, b in the xy plane) instead of the 3-float diagonal */
     int    triclinic;
+    /* oh_per_molecule != 0: chromophore k of a molecule is the O-H(1+k)
+     * stretch (nchrom_mol <= 2).  0 (the reference, calcIR.cu:1061-1070,
+     * 1266-1275, 1360-1370): H1 or H2 by chrom % 2, so with nchrom_mol 1
+     * molecules alternate H1, H2 by parity. */
+    int    oh_per_molecule;
+    /* msite_bisector != 0 with natom_mol 3: a -1.04 site is built on the HOH
+     * bisector 0.015 nm from O.  0 (the reference, calcIR.cu:1117-1160): the
+     * charged sites are atoms 1..natom_mol-1, H (+0.52) for atoms 1, 2 and M
+     * (-1.04) for atom 3, so three-site water carries no negative site. */
+    int    msite_bisector;
+    /* precision at which the first two frames' spacing is rounded,
+     * round(dt * prec) / prec (calcIR.cu:342-345; prec is the XTC file's);
+     * 0 means 1000 */
+    float  time_prec;
 } orc_config;
 
 /* Outputs of orc_run.  All arrays are caller-allocated. */

@@ -31,6 +31,7 @@ class Config(ctypes.Structure):
         ("omegaStart", ctypes.c_float), ("omegaStop", ctypes.c_float), ("omegaStep", ctypes.c_int),
         ("propagation", ctypes.c_int), ("nthreads", ctypes.c_int),
         ("use_coef", ctypes.c_int), ("coef", ctypes.c_double * 13), ("triclinic", ctypes.c_int),
+        ("oh_per_molecule", ctypes.c_int), ("msite_bisector", ctypes.c_int), ("time_prec", ctypes.c_float),
     ]
 
 
@@ -86,11 +87,15 @@ def lib():
 def config(imodel=1, ispecies=0, imap=0, cutoff=0.7831, avef=3415.2, dt=0.010, t1=0.260,
            ntcfpoints=150, nsamples=1, nzeros=25600, sampleEvery=10.0, beginTime=0.0,
            omegaStart=2000.0, omegaStop=5000.0, omegaStep=5, propagation=PROP_VECTOR, nthreads=0,
-           natom_mol=4, nchrom_mol=2, coef=None, triclinic=0):
+           natom_mol=4, nchrom_mol=2, coef=None, triclinic=0, oh_per_molecule=0, msite_bisector=0,
+           time_prec=1000.0):
     c = Config(natom_mol, nchrom_mol, imodel, ispecies, imap, cutoff, avef, dt, t1, ntcfpoints,
                nsamples, nzeros, sampleEvery, beginTime, omegaStart, omegaStop, omegaStep,
                propagation, nthreads)
     c.triclinic = int(triclinic)
+    c.oh_per_molecule = int(oh_per_molecule)
+    c.msite_bisector = int(msite_bisector)
+    c.time_prec = float(time_prec)
     if coef is not None:
         set_coef(c, coef)
     return c
@@ -222,4 +227,5 @@ def config_from_params(p, coef=None):
                   t1=p.t1, ntcfpoints=p.ntcfpoints, nsamples=p.nsamples, nzeros=p.nzeros,
                   sampleEvery=p.sampleEvery, beginTime=p.beginTime, omegaStart=p.omegaStart,
                   omegaStop=p.omegaStop, omegaStep=p.omegaStep, natom_mol=p.natom_mol,
-                  nchrom_mol=p.nchrom_mol, coef=coef, triclinic=p.triclinic)
+                  nchrom_mol=p.nchrom_mol, coef=coef, triclinic=p.triclinic,
+                  oh_per_molecule=p.oh_per_molecule, msite_bisector=p.msite_bisector, time_prec=p.time_prec)

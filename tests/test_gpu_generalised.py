@@ -1,8 +1,10 @@
-"""GPU parity of the generalisations beyond the reference against the
-oracle (which is pinned by brute-force images, lattice invariance and
-reduction to the reference case): triclinic boxes, three-site water, one OH
-chromophore per molecule; the CLI on a triclinic XTC file and the lineshape
-adjoint in a triclinic box."""
+"""GPU parity against the oracle for the molecule layouts the reference
+accepts (three-site water, one OH chromophore per molecule, with the
+reference's chrom % 2 and atom-index semantics) and for the generalisations
+beyond it (the oh_per_molecule and msite_bisector rules, triclinic boxes);
+the CLI on XTC files of these layouts and the lineshape adjoint in a
+triclinic box.  The oracle is pinned by closed forms, brute-force sums,
+lattice invariance and reduction to the two-OH, four-site case."""
 
 import subprocess
 
@@ -60,17 +62,58 @@ def test_zero_shear_triclinic_equals_rectangular():
         C.run(C.default_params(precision="fp64", triclinic=1, **KW), xyz, bad, times)
 
 
-@pytest.mark.parametrize("natom_mol,nchrom_mol", [(3, 2), (4, 1), (3, 1)])
-def test_three_site_and_one_oh_pipelines_match_oracle(natom_mol, nchrom_mol):
-    xyz, box, times = frames(128, 40, 1001)
+def _layout(xyz, natom_mol):
     if natom_mol == 3:
-        xyz = np.ascontiguousarray(xyz.reshape(xyz.shape[0], -1, 4, 3)[:, :, :3, :].reshape(xyz.shape[0], -1, 3))
-    p = C.default_params(precision="fp64", natom_mol=natom_mol, nchrom_mol=nchrom_mol, **KW)
-    announce("pipeline natom_mol=%d nchrom_mol=%d" % (natom_mol, nchrom_mol))
+        return np.ascontiguousarray(xyz.reshape(xyz.shape[0], -1, 4, 3)[:, :, :3, :].reshape(xyz.shape[0], -1, 3))
+    return xyz
+
+
+@pytest.mark.parametrize("natom_mol,nchrom_mol,rules", [
+    (3, 2, {}), (4, 1, {}), (3, 1, {}),  # the reference's own semantics
+    (3, 2, dict(msite_bisector=1)), (4, 1, dict(oh_per_molecule=1)),
+    (3, 1, dict(msite_bisector=1, oh_per_molecule=1))])
+def test_three_site_and_one_oh_pipelines_match_oracle(natom_mol, nchrom_mol, rules):
+    xyz, box, times = frames(128, 40, 1001)
+    xyz = _layout(xyz, natom_mol)
+    p = C.default_params(precision="fp64", natom_mol=natom_mol, nchrom_mol=nchrom_mol, **rules, **KW)
+    announce("pipeline natom_mol=%d nchrom_mol=%d rules=%s" % (natom_mol, nchrom_mol, rules))
     g = C.run(p, xyz, box, times)
     o = O.run(O.F64, O.config_from_params(p), xyz, box, times)
     _cmp(g, o, 1e-10)
     assert np.allclose(g["pw"], o["pw"], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("natom_mol,nchrom_mol,extra", [(3, 2, ""), (4, 1, ""), (3, 1, "msite_bisector 1\n"),
+                                                        (4, 1, "oh_per_molecule 1\n")])
+def test_cli_layouts_match_the_oracle(tmp_path, natom_mol, nchrom_mol, extra):
+    # the drop-in executable on an XTC file of three-site or four-site water,
+    # with the reference's natom_mol / nchrom_mol semantics (or a rule key),
+    # against the oracle's 13 files written from the same decoded frames
+    xyz, box, times = frames(64, 30, 17)
+    xyz = _layout(xyz, natom_mol)
+    traj = tmp_path / "w.xtc"
+    C.xtc_write(traj, xyz, box, times)
+    xq, bq, tq, _ = C.xtc_read(traj)
+    inp = tmp_path / "input.inp"
+    inp.write_text("xtcf %s\noutf %s\nmodel tip4p2005\nspecies H2O\nntcfpoints 10\nnsamples 2\n"
+                   "sampleEvery 0.1\navef 3400\nomegaStart 2800\nomegaStop 4000\nnzeros 502\nprecision fp64\n"
+                   "natom_mol %d\nnchrom_mol %d\n%s" % (traj, tmp_path / "cli", natom_mol, nchrom_mol, extra))
+    announce("calcir CLI natom_mol=%d nchrom_mol=%d %s" % (natom_mol, nchrom_mol, extra.strip()))
+    r = subprocess.run([C.CLI_PATH, str(inp)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    p = C.read_params(str(inp))
+    p.outf = str(tmp_path / "orc").encode()
+    o = O.run(O.F64, O.config_from_params(p), xq, bq, tq)
+    C.write_outputs(p, o)
+    for s in ("irrtcf", "iritcf", "vvrtcf", "vhrtcf", "spdn", "Pw", "Rw", "Rmw", "irls", "vvls", "vhls"):
+        a = np.loadtxt(tmp_path / ("cli_%s.dat" % s))
+        b = np.loadtxt(tmp_path / ("orc_%s.dat" % s))
+        assert a.shape == b.shape, s
+        y, z = a[:, 1], b[:, 1]
+        assert np.array_equal(np.isnan(y), np.isnan(z)), s
+        m = ~np.isnan(z)
+        scale = np.abs(z[m]).max() if m.any() else 1.0
+        np.testing.assert_allclose(y[m], z[m], rtol=2e-5, atol=2e-6 * scale, err_msg=s)
 
 
 def test_cli_reads_a_triclinic_xtc(tmp_path):

@@ -513,3 +513,32 @@ def test_origin_stride_frames_equals_sample_every():
               times)
     for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
         assert rel(b[k], a[k]) < 1e-12, k
+
+
+def test_cli_rounds_the_frame_spacing_at_the_file_precision(tmp_path):
+    # calcIR.cu:342-345: frame_dt = round(dt * prec) / prec with the XTC file's
+    # own precision.  Frames 0.0125 ps apart in a precision-10000 file give
+    # frame_dt 0.0125 (at precision 1000 it would be 0.013 and the time check
+    # would refuse the run after a few points).
+    xyz, box, _ = frames(32, 40, 3)
+    times = (np.arange(40) * 0.0125).astype(np.float32)
+    traj = tmp_path / "fine.xtc"
+    C.xtc_write(traj, xyz, box, times, precision=10000.0)
+    inp = tmp_path / "input.inp"
+    inp.write_text("xtcf %s\noutf %s\nmodel tip4p2005\nspecies H2O\nntcfpoints 20\nnsamples 2\n"
+                   "sampleEvery 0.1\ndt 0.0125\navef 3400\nomegaStart 2000\nomegaStop 4500\nnzeros 100\n"
+                   "precision fp64\n" % (traj, tmp_path / "cli"))
+    announce("calcir CLI, frame spacing at precision 10000")
+    r = subprocess.run([C.CLI_PATH, str(inp)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "Frame time offset is: 0.012500" in r.stdout
+    xq, bq, tq, _ = C.xtc_read(traj)
+    p = C.read_params(str(inp))
+    p.time_prec = 10000.0
+    o = O.run(O.F64, O.config_from_params(p), xq, bq, tq)
+    a = np.loadtxt(tmp_path / "cli_irrtcf.dat")
+    np.testing.assert_allclose(a[:, 1], o["tcf_ir"].real, rtol=2e-5, atol=2e-6 * np.abs(o["tcf_ir"].real).max())
+    # the same frames at the default precision 1000 are refused by the time check
+    p.time_prec = 1000.0
+    with pytest.raises(C.CalcirError):
+        C.run(p, xq, bq, tq)

@@ -47,6 +47,8 @@ def test_defaults_are_the_reference_defaults():
     assert p.dt == pytest.approx(0.010) and p.t1 == pytest.approx(0.260) and p.avef == pytest.approx(3415.2)
     assert p.cutoff == pytest.approx(0.7831)
     assert C.nomega(p) == 601
+    # the reference's H choice and charged sites; GROMACS' default precision
+    assert (p.oh_per_molecule, p.msite_bisector, p.time_prec) == (0, 0, 1000.0)
 
 
 def test_parser_keys_comments_and_quirks(tmp_path):
@@ -237,3 +239,33 @@ def test_cli_usage_and_input_errors(tmp_path):
     assert r.returncode == 1 and "could not be opened" in r.stdout
     # the echo of calcIR.cu:70-87
     assert "\tSetting model to tip4p\n" in r.stdout and "\tSetting species to H2O\n" in r.stdout
+
+
+def test_rule_and_precision_keys_parse(tmp_path):
+    f = tmp_path / "input.inp"
+    f.write_text("nchrom_mol 1\nnatom_mol 3\noh_per_molecule 1\nmsite_bisector 1\ntime_precision 10000\n")
+    p = C.read_params(str(f))
+    assert (p.nchrom_mol, p.natom_mol, p.oh_per_molecule, p.msite_bisector) == (1, 3, 1, 1)
+    assert p.time_prec == 10000.0
+
+
+def test_trajectory_shapes_are_checked_before_the_library_reads_them():
+    xyz, box, times = C.synth_trajectory(4, 3, 1)
+    p = C.default_params(species="H2O")
+    C.check_trajectory(p, xyz, box, times)
+    bad = [(xyz.reshape(3, -1), box, times),                  # [nframes, natoms*3]
+           (xyz[:, :15], box, times),                         # natoms not a multiple of natom_mol
+           (xyz, box[:2], times),                             # a box per frame
+           (xyz, box, times[:2]),                             # a time per frame
+           (xyz, np.zeros((3, 9), np.float32), times)]        # 9 floats without the triclinic key
+    for args in bad:
+        with pytest.raises(ValueError):
+            C.check_trajectory(p, *args)
+        with pytest.raises(ValueError):
+            C.run(p, *args)
+    pt = C.default_params(species="H2O", triclinic=1)
+    C.check_trajectory(pt, xyz, np.zeros((3, 3, 3), np.float32), times)
+    with pytest.raises(ValueError):
+        C.check_trajectory(pt, xyz, box, times)
+    with pytest.raises(ValueError):
+        C.run_many([(p, xyz, box, times[:1])])

@@ -1,12 +1,18 @@
-"""Oracle pinning of the generalisations beyond the reference: triclinic
-boxes (the reference uses only the box diagonal, calcIR.cu:404-405), one OH
-chromophore per molecule (the reference hard-codes chrom%2,
-calcIR.cu:1061-1070) and three-site water (the reference reads atom 3 as the
-M site, calcIR.cu:1124-1158).
+"""Oracle pinning of the molecule layouts the reference accepts and of the
+generalisations beyond it.
 
-Pinned by: reduction to the reference case (zero shear, nchrom_mol 2 rows),
-brute-force nearest-image Coulomb sums and couplings, invariance under
-lattice translations of single molecules, and the TIP4P M geometry."""
+Reference semantics: H1 or H2 by chrom % 2 whatever nchrom_mol is
+(calcIR.cu:1061-1070, 1266-1275, 1360-1370), and charged sites = atoms
+1..natom_mol-1 with +0.52 on atoms 1, 2 and -1.04 on atom 3
+(calcIR.cu:1117-1160).  Generalisations: triclinic boxes (the reference uses
+only the box diagonal, calcIR.cu:404-405), the oh_per_molecule rule (H1 for
+nchrom_mol 1) and the msite_bisector rule (a built M site for three-site
+water).
+
+Pinned by: hand-placed collinear molecules against the Coulomb closed form,
+a brute-force numpy field sum, reduction to the nchrom_mol 2 / natom_mol 4
+case, brute-force nearest-image sums and couplings, lattice-translation
+invariance and the TIP4P M geometry."""
 
 import itertools
 
@@ -109,26 +115,109 @@ def test_lattice_translation_of_one_molecule_changes_nothing(vec):
     assert np.abs(H1 - H0).max() <= 1e-4 * np.abs(H0).max()
 
 
-def test_three_site_water_builds_the_tip4p_m_site():
+def test_msite_bisector_builds_the_tip4p_m_site_for_three_site_water():
     # the synthetic frames carry M on the HOH bisector; with model tip4p2005 it
-    # is moved to 0.015 nm from O (calcIR.cu:1124-1140), which is where a
-    # three-site water's built M sits
+    # is moved to 0.015 nm from O (calcIR.cu:1124-1140), which is where the
+    # msite_bisector rule builds a three-site water's negative site
     xyz, box, _ = frames(64, 2, 31)
     x4 = xyz[1]
     x3 = x4.reshape(-1, 4, 3)[:, :3, :].reshape(-1, 3).copy()
     e4 = O.eproj(O.F64, _cfg(imodel=1), x4, box[1])
-    e3 = O.eproj(O.F64, _cfg(imodel=1, natom_mol=3), x3, box[1])
+    e3 = O.eproj(O.F64, _cfg(imodel=1, natom_mol=3, msite_bisector=1), x3, box[1])
     assert np.abs(e3 - e4).max() <= 1e-6 * np.abs(e4).max()
     H4 = O.hamiltonian(O.F64, _cfg(imodel=1), x4, box[1], e4)
-    H3 = O.hamiltonian(O.F64, _cfg(imodel=1, natom_mol=3), x3, box[1], e3)
+    H3 = O.hamiltonian(O.F64, _cfg(imodel=1, natom_mol=3, msite_bisector=1), x3, box[1], e3)
     assert np.abs(H3 - H4).max() <= 1e-5 * np.abs(H4).max()
+    # the rule changes nothing for four-site water
+    assert np.array_equal(O.eproj(O.F64, _cfg(imodel=1, msite_bisector=1), x4, box[1]), e4)
 
 
-def test_one_oh_per_molecule_is_the_h1_rows():
+def test_oh_per_molecule_with_one_oh_is_the_h1_rows():
     xyz, box, _ = frames(32, 2, 41)
     e2 = O.eproj(O.F64, _cfg(), xyz[1], box[1])
-    e1 = O.eproj(O.F64, _cfg(nchrom_mol=1), xyz[1], box[1])
+    e1 = O.eproj(O.F64, _cfg(nchrom_mol=1, oh_per_molecule=1), xyz[1], box[1])
     assert np.array_equal(e1, e2[0::2])
     H2 = O.hamiltonian(O.F64, _cfg(), xyz[1], box[1], e2)
-    H1 = O.hamiltonian(O.F64, _cfg(nchrom_mol=1), xyz[1], box[1], e1)
+    H1 = O.hamiltonian(O.F64, _cfg(nchrom_mol=1, oh_per_molecule=1), xyz[1], box[1], e1)
     assert np.array_equal(H1, H2[0::2, 0::2])
+    # with two OH per molecule the rule is the reference's
+    assert np.array_equal(O.eproj(O.F64, _cfg(oh_per_molecule=1), xyz[1], box[1]), e2)
+
+
+def test_reference_one_oh_alternates_h1_h2_by_molecule_parity():
+    # calcIR.cu:1061-1070: chrom % 2 picks the H, and with nchrom_mol 1 chrom is
+    # the molecule index, so molecule n is the stretch of H(1 + n % 2): the rows
+    # 2n + n % 2 of the two-OH case, field and Hamiltonian alike
+    xyz, box, _ = frames(32, 2, 41)
+    idx = np.array([2 * n + n % 2 for n in range(32)])
+    for prec in (O.F64, O.F32):
+        e2 = O.eproj(prec, _cfg(), xyz[1], box[1])
+        e1 = O.eproj(prec, _cfg(nchrom_mol=1), xyz[1], box[1])
+        assert np.array_equal(e1, e2[idx])
+        H2 = O.hamiltonian(prec, _cfg(), xyz[1], box[1], e2)
+        H1 = O.hamiltonian(prec, _cfg(nchrom_mol=1), xyz[1], box[1], e1)
+        # every off-diagonal pair is intermolecular in both layouts
+        assert np.array_equal(H1, H2[np.ix_(idx, idx)])
+    s = O.sites(O.F64, _cfg(nchrom_mol=1), xyz[1], box[1], e1)
+    x = xyz[1].astype(np.float64).reshape(32, 4, 3)
+    for n in (0, 1, 2, 7):
+        u = x[n, 1 + n % 2] - x[n, 0]
+        u -= float(box[1, 0]) * np.round(u / float(box[1, 0]))
+        assert np.allclose(s[n, O.SITE_U:O.SITE_U + 3], u / np.linalg.norm(u), atol=1e-6)
+
+
+def test_reference_three_site_water_field_is_the_h_charges_brute_force():
+    # calcIR.cu:1117-1160 with natom_mol 3: atoms 1 and 2 (+0.52 each) are the
+    # only charged sites.  Brute-force numpy sum over the molecules whose O is
+    # within the cutoff of the H, minimum image on each difference.
+    xyz, box, _ = frames(48, 2, 5)
+    nmol = 48
+    x4 = xyz[1].astype(np.float64).reshape(nmol, 4, 3)
+    x3 = np.ascontiguousarray(xyz[1].reshape(nmol, 4, 3)[:, :3, :].reshape(-1, 3))
+    L = float(box[1, 0])
+    for model in (0, 1):
+        e = O.eproj(O.F64, _cfg(imodel=model, natom_mol=3), x3, box[1])
+        for chrom in range(0, 2 * nmol, 5):
+            n = chrom // 2
+            h = x4[n, 1 + chrom % 2]
+            u = h - x4[n, 0]
+            u -= L * np.round(u / L)
+            u /= np.linalg.norm(u)
+            ef = np.zeros(3)
+            for m in range(nmol):
+                if m == n:
+                    continue
+                d = x4[m, 0] - h
+                d -= L * np.round(d / L)
+                if np.linalg.norm(d) > 0.7831:
+                    continue
+                for i in (1, 2):
+                    dr = h - x4[m, i]
+                    dr = (dr - L * np.round(dr / L)) * BOHR
+                    ef += 0.52 * dr / np.linalg.norm(dr) ** 3
+            assert e[chrom] == pytest.approx(ef @ u, rel=1e-9, abs=1e-12)
+        # the same molecules with their M sites carry the extra -1.04 terms
+        e4 = O.eproj(O.F64, _cfg(imodel=model), xyz[1], box[1])
+        assert np.abs(e4 - e).max() > 1e-3
+
+
+def test_reference_atoms_past_the_fourth_carry_no_charge():
+    # calcIR.cu:1150-1158 adds a term only for atoms 1, 2 (H) and 3 (M); a
+    # fifth atom per molecule (a TIP5P-like layout) changes nothing
+    xyz, box, _ = frames(32, 2, 9)
+    x4 = xyz[1].reshape(32, 4, 3)
+    extra = x4[:, :1, :] + np.float32(0.05)
+    x5 = np.ascontiguousarray(np.concatenate([x4, extra], axis=1).reshape(-1, 3))
+    for model in (0, 1):
+        e4 = O.eproj(O.F64, _cfg(imodel=model), xyz[1], box[1])
+        e5 = O.eproj(O.F64, _cfg(imodel=model, natom_mol=5), x5, box[1])
+        assert np.array_equal(e4, e5)
+
+
+def test_unsupported_layouts_are_refused():
+    xyz, box, _ = frames(32, 2, 9)
+    x2 = np.ascontiguousarray(xyz[1].reshape(32, 4, 3)[:, :2, :].reshape(-1, 3))
+    with pytest.raises(RuntimeError):
+        O.eproj(O.F64, _cfg(natom_mol=2), x2, box[1])
+    with pytest.raises(RuntimeError):
+        O.eproj(O.F64, _cfg(nchrom_mol=3, oh_per_molecule=1), xyz[1], box[1])

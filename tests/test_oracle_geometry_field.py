@@ -132,3 +132,38 @@ def test_field_minimum_image_across_the_boundary():
     e = _field0(x, L=L)
     assert e[0] == pytest.approx(_closed([L - 0.10, L - 0.20, L - 0.15], [0.52, 0.52, -1.04], L=L), rel=1e-12)
     assert e[0] == pytest.approx(_closed([-0.10, -0.20, -0.15], [0.52, 0.52, -1.04]), rel=1e-5)
+
+
+def _field(x, natom_mol=4, nchrom_mol=2, model=0, L=5.0, **rules):
+    cfg = O.config(imodel=model, natom_mol=natom_mol, nchrom_mol=nchrom_mol, **rules)
+    return O.eproj(O.F64, cfg, np.ascontiguousarray(x, np.float32), np.array([L, L, L], np.float32))
+
+
+def test_field_one_oh_per_molecule_alternates_h_by_parity_closed_form():
+    # nchrom_mol 1: chromophore n is molecule n and calcIR.cu:1061-1070 takes
+    # its H by n % 2.  Molecule 0 -> H1 at x = 0 (OH along +x); molecule 1 ->
+    # H2 at x = 0.30 (OH along -x).  All sites on the x axis.
+    m0 = _line_molecule(-0.10, 0.00, -0.25, -0.11)
+    m1 = _line_molecule(0.33, 0.45, 0.30, 0.34)
+    e = _field(np.vstack([m0, m1]), nchrom_mol=1)
+    assert e[0] == pytest.approx(_closed([0.45, 0.30, 0.34], [0.52, 0.52, -1.04], h=0.0), rel=1e-12)
+    assert e[1] == pytest.approx(-_closed([0.00, -0.25, -0.11], [0.52, 0.52, -1.04], h=0.30), rel=1e-12)
+    # the oh_per_molecule rule takes molecule 1's H1 at 0.45 (OH along +x)
+    v = _field(np.vstack([m0, m1]), nchrom_mol=1, oh_per_molecule=1)
+    assert v[0] == e[0]
+    assert v[1] == pytest.approx(_closed([0.00, -0.25, -0.11], [0.52, 0.52, -1.04], h=0.45), rel=1e-12)
+
+
+def test_field_three_site_water_closed_form():
+    # natom_mol 3: calcIR.cu:1117 loops i = 1, 2 only, so the two H charges are
+    # the whole field; the msite_bisector rule adds -1.04 at O + 0.015 nm along
+    # the HOH bisector (here both OH point along +x: at 0.33 + 0.015)
+    m0 = _line_molecule(-0.10, 0.00, -0.25, 0.0)[:3]
+    m1 = _line_molecule(0.33, 0.45, 0.40, 0.0)[:3]
+    for model in (0, 1):
+        e = _field(np.vstack([m0, m1]), natom_mol=3, model=model)
+        assert e[0] == pytest.approx(_closed([0.45, 0.40], [0.52, 0.52], h=0.0), rel=1e-12)
+        b = _field(np.vstack([m0, m1]), natom_mol=3, model=model, msite_bisector=1)
+        mprime = float(np.float32(0.33)) + 0.015
+        expect = _closed([0.45, 0.40], [0.52, 0.52], h=0.0) + (-1.04) * (-mprime) * BOHR / (mprime * BOHR) ** 3
+        assert b[0] == pytest.approx(expect, rel=1e-12)

@@ -246,3 +246,18 @@ def test_cli_rejects_a_corrupt_trajectory_without_crashing(tmp_path):
     assert r.returncode == 1, (r.returncode, r.stdout[-500:])
     assert "could not be opened" in r.stdout
     assert not list(tmp_path.glob("o_*.dat"))
+
+
+@pytest.mark.parametrize("precision", [100.0, 1000.0, 10000.0])
+def test_stored_precision_is_reported(tmp_path, precision):
+    # the precision read_xtc returns (calcIR.cu:342-345 rounds the frame
+    # spacing with it); frames of <= 9 atoms are stored without one
+    xyz, box, times = C.synth_trajectory(8, 3, 2)
+    f = tmp_path / "p.xtc"
+    C.xtc_write(f, xyz, box, times, precision=precision)
+    assert C.xtc_precision(f, 0) == precision and C.xtc_precision(f, 1) == precision
+    with pytest.raises(C.CalcirError):
+        C.xtc_precision(f, 3)
+    g = tmp_path / "small.xtc"
+    C.xtc_write(g, xyz[:, :8], box, times, precision=precision)
+    assert C.xtc_precision(g, 1) == -1.0
