@@ -141,6 +141,8 @@ SIGNATURES = {
     "cir_spectral_density": (_i, [_i, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cir_propagate": (_i, [_i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cir_propagate_workspace": (ctypes.c_size_t, [_i, _i, _i]),
+    "cir_propagate_noalloc": (_i, [_i, _i, _i, _i, _i, _f, _vp, _vp, _vp, _vp, _vp]),
+    "cir_propagate_noalloc_max_bytes": (ctypes.c_size_t, []),
     "cir_correlate": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cir_spectrum_c2r": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "cir_gemm": (_i, [_i, _i, _i, _i, _i, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll,

@@ -526,7 +526,7 @@ static int prop_pair(bool split3, int n, int ld, int ncols, int ldy, double dt, 
                      const float* Yin, float* Yout, float* G, int ks, hipStream_t s);
 
 static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
-                         const float* Yin, float* Yout, float* G, hipStream_t s) {
+                         const float* Yin, float* Yout, float* G, hipStream_t s, bool may_tune) {
     const int nmt_pad = int(ceil_div(ceil_div(n, 32), 8) * 8);
     const int tiles = nmt_pad * int(ceil_div(ncols, 16));
     const int guess = std::max(1, std::min({kPropMaxSplit, int(ceil_div(1024, tiles)), std::max(1, n / 256)}));
@@ -539,7 +539,7 @@ static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double 
         if (it != prop_tune().best.end())
             ks = it->second;
         else
-            tune = Yin != Yout;
+            tune = may_tune && Yin != Yout;
     }
     if (tune) {
         EventPair ev;
@@ -665,7 +665,7 @@ static int split_gemm(int gemm_prec, int M, int N, int K, int ks, int ns, size_t
 }
 
 int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
-              const double* w, const void* Yin, void* Yout, void* G, hipStream_t s) {
+              const double* w, const void* Yin, void* Yout, void* G, hipStream_t s, bool may_tune) {
     if (n <= 0 || ncols <= 0 || ldy < 2 * ncols || ld < n) return fail(CIR_ERR_ARG, "propagate: bad sizes");
     const size_t esz = (prec == CIR_PREC_FP64) ? sizeof(double) : sizeof(float);
     const int w2 = 2 * ncols;
@@ -674,7 +674,8 @@ int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double
     const int64_t tot = int64_t(n) * ncols;
     if (prec != CIR_PREC_FP64 && ncols <= kPrMaxCols)
         return prop_products(gemm_prec == CIR_PREC_BF16X3, n, ld, ncols, ldy, dt, static_cast<const float*>(V), w,
-                             static_cast<const float*>(Yin), static_cast<float*>(Yout), static_cast<float*>(G), s);
+                             static_cast<const float*>(Yin), static_cast<float*>(Yout), static_cast<float*>(G), s,
+                             may_tune);
     if (ns == 1) {
         // G = V^T Yin (V row-major: V(i,k) at V[i*ld+k]; A(m=k, kk=i)), phase, Yout = V G'
         GemmArgs g;

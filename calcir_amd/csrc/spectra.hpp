@@ -34,10 +34,12 @@ int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, f
                      double* sw, double* m2work, hipStream_t s);
 // Yout = V exp(+i diag(w) dt/hbar) V^T Yin, Y row-major n x ldy with Re in
 // columns [0, ncols) and Im in [ncols, 2 ncols); G: propagate_work_elems()
-// reals of work.  Yout may alias Yin.
+// reals of work.  Yout may alias Yin.  may_tune: the first call of a shape may
+// time the candidate K splits (HIP events); false takes the cached or default
+// split and creates nothing.
 size_t propagate_work_elems(int n, int ncols);
 int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
-              const double* w, const void* Yin, void* Yout, void* G, hipStream_t s);
+              const double* w, const void* Yin, void* Yout, void* G, hipStream_t s, bool may_tune = true);
 int set_identity(int prec, int n, int ldf, void* F, hipStream_t s);
 int slot_fixup(int prec, int n, int nslot, int ldy, const int* action, const void* sites, const void* Yold,
                void* Ynew, hipStream_t s);

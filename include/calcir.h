@@ -376,6 +376,20 @@ int cir_spectral_density(int prec, int batch, int n, int ld, float t1, float ave
 size_t cir_propagate_workspace(int prec, int n, int ncols);
 int cir_propagate(int prec, int n, int ld, int ncols, int ldy, float dt, const void* V_d,
                   const double* w_d, const void* Yin_d, void* Yout_d, void* work_d, void* stream);
+/* cir_propagate without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * allocates nothing: its work buffer is a fixed device array of
+ * cir_propagate_noalloc_max_bytes() bytes (32 MiB) in the library, one per
+ * GPU, held until the process ends; shapes whose cir_propagate_workspace()
+ * exceeds it return CIR_ERR_ARG.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (concurrent streams or threads)
+ * race.  The K split of the products is the one a cir_propagate call of the
+ * same shape tuned earlier in the process, else a fixed default (no timing
+ * runs, no HIP events).  Results equal cir_propagate's to rounding. */
+size_t cir_propagate_noalloc_max_bytes(void);
+int cir_propagate_noalloc(int prec, int n, int ld, int ncols, int ldy, float dt, const void* V_d,
+                          const double* w_d, const void* Yin_d, void* Yout_d, void* stream);
 /* Correlations (calcIR.cu:628-745): for each of nslot time origins s with
  * action_d[s] in {1, 2} (NULL = all), the 15 bilinear forms M_b(t)^T Y_a of
  * Y = F M(0) (slot s: columns 9s..9s+8 of Y_d as in cir_propagate) with
