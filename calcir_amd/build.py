@@ -17,8 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcalcir.so")
 CLI = os.path.join(HERE, "bin", "calcir")
 
-HIP_SOURCES = ["k_sites.hip", "k_gemm.hip", "k_eigh.hip", "k_spectra.hip", "pipeline.hip", "adjoint.hip",
-               "noalloc.hip"]
+HIP_SOURCES = ["k_sites.hip", "k_gemm.hip", "k_eigh.hip", "k_band.hip", "k_spectra.hip", "pipeline.hip",
+               "adjoint.hip", "noalloc.hip"]
 CXX_SOURCES = ["common.cpp", "host_io.cpp", "synth.cpp", "xtc.cpp", "cli.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

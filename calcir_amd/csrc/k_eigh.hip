@@ -35,7 +35,9 @@
 //      reflectors.
 
 #include <cfloat>
+#include <cstdlib>
 
+#include "band.hpp"
 #include "gemm.hpp"
 
 namespace cir {
@@ -1516,12 +1518,12 @@ larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __res
 }
 
 // Y^T of a back-transform pass into a k-contiguous copy: Yt(m, k) =
-// Y(m, k) = A[(j0 + k) * ld + j0 + 1 + m] at Yt[m * kBtPass + k], m < mrows,
+// Y(m, k) = A[(j0 + k) * ld + j0 + off + m] at Yt[m * kBtPass + k], m < mrows,
 // k < kb, through a padded 64 x 64 LDS tile (reads and writes both along
 // memory).  The update Z -= Y W then stages both operands k-contiguous.
 template <typename T>
 __global__ void __launch_bounds__(256)
-yt_kernel(int n, int ld, int j0, int kb, int mrows, const T* __restrict__ Aall, T* __restrict__ Ytall,
+yt_kernel(int n, int ld, int j0, int off, int kb, int mrows, const T* __restrict__ Aall, T* __restrict__ Ytall,
           int64_t ystride) {
     __shared__ T tile[64][65];
     const T* A = Aall + size_t(blockIdx.z) * n * ld;
@@ -1530,7 +1532,7 @@ yt_kernel(int n, int ld, int j0, int kb, int mrows, const T* __restrict__ Aall, 
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for (int kk = ty; kk < 64; kk += 4) {
         const int m = m0 + tx, k = k0 + kk;
-        tile[kk][tx] = (m < mrows && k < kb) ? A[size_t(j0 + k) * ld + (j0 + 1 + m)] : T(0);
+        tile[kk][tx] = (m < mrows && k < kb) ? A[size_t(j0 + k) * ld + (j0 + off + m)] : T(0);
     }
     __syncthreads();
     for (int mm = ty; mm < 64; mm += 4) {
@@ -1541,8 +1543,26 @@ yt_kernel(int n, int ld, int j0, int kb, int mrows, const T* __restrict__ Aall, 
 
 inline size_t larft_smem_bytes(size_t esz) { return (size_t(kBtNB) * (kBtNB + 1) + kBtNB) * esz; }
 
+// The FP32 batched path can reduce to tridiagonal form in two stages
+// (k_band.hip); the few-matrix whole-GPU path and FP64 reduce in one stage.
+// CIR_EIGH_STAGES=1 or 2 in the environment forces one form where both apply
+// (tests compare them); otherwise the two-stage form is taken from
+// kTwoStageMinN on.
+constexpr int kTwoStageMinN = 1 << 30;
+inline bool use_two_stage(size_t esz, int batch, int n) {
+    if (esz != sizeof(float) || n > sb_max_n() || n < 2 * kBandNB + 2) return false;
+    if (n >= kSingleMinN && batch <= kSingleMaxBatch) return false;
+    const char* fv = std::getenv("CIR_EIGH_STAGES");
+    const int forced = fv ? std::atoi(fv) : 0;
+    if (forced == 1) return false;
+    if (forced == 2) return true;
+    return n >= kTwoStageMinN;
+}
+
 struct EighWork {
     size_t off_d, off_e, off_tau, off_vw, off_g, off_t, off_w1, off_w2, off_s1, off_scr, total;
+    size_t off_bnd, off_p1, off_p2, off_yh, off_mm, off_q2;  // two-stage buffers
+    bool two;
     int sub;  // matrices per inverse-iteration pass
 };
 
@@ -1550,6 +1570,21 @@ inline EighWork eigh_layout(size_t esz, int batch, int n, int ld) {
     EighWork L{};
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     size_t o = 0;
+    L.two = use_two_stage(esz, batch, n);
+    if (L.two) {
+        L.off_bnd = o;
+        o = al(o + esz * batch * size_t(n) * kBandW);
+        L.off_p1 = o;
+        o = al(o + esz * batch * size_t(n) * 2 * kBandNB);
+        L.off_p2 = o;
+        o = al(o + esz * batch * size_t(n) * 2 * kBandNB);
+        L.off_yh = o;
+        o = al(o + esz * batch * size_t(n) * kBandNB);
+        L.off_mm = o;
+        o = al(o + esz * batch * size_t(kBandNB) * kBandNB);
+        L.off_q2 = o;
+        o = al(o + esz * batch * size_t(q2_blocks(n)) * kQ2BlockFloats);
+    }
     L.off_d = o;
     o = al(o + esz * batch * n);
     L.off_e = o;
@@ -1596,7 +1631,18 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     // large matrices (too few workgroups to fill the GPU) run one at a time
     // across the whole GPU
     if (ld % 4 != 0) return fail(CIR_ERR_ARG, "eigh: the leading dimension must be a multiple of 4");
-    if (n >= kSingleMinN && batch <= kSingleMaxBatch) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (L.two) {
+            ProfScope ps(prof, CIR_PH_TRIDIAG, s);
+            float* bnd = reinterpret_cast<float*>(work + L.off_bnd);
+            CIR_TRY(sy2sb(batch, n, ld, A, bnd, tau, reinterpret_cast<float*>(work + L.off_p1),
+                          reinterpret_cast<float*>(work + L.off_p2), reinterpret_cast<float*>(work + L.off_yh),
+                          reinterpret_cast<float*>(work + L.off_mm), n, s));
+            CIR_TRY(sb2st(batch, n, bnd, dd, ee, reinterpret_cast<float*>(work + L.off_q2), s));
+        }
+    }
+    if (L.two) {
+    } else if (n >= kSingleMinN && batch <= kSingleMaxBatch) {
         ProfScope ps(prof, CIR_PH_TRIDIAG, s);
         T* s1 = reinterpret_cast<T*>(work + L.off_s1);
         for (int b = 0; b < batch; ++b)
@@ -1634,11 +1680,24 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
         CIR_CHECK_LAUNCH();
     }
     delete pinv;
-    // 4. back-transform V <- Q V, last block of reflectors first
+    // 4. back-transform V <- Q V: the bulge-chasing reflectors first (two
+    // stages), then the Householder panels, last block of reflectors first
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&larft_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(larft_smem_bytes(sizeof(T)))));
     ProfScope pbt(prof, CIR_PH_BACKTRANSFORM, s);
-    const int nref = n - 1;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (L.two) CIR_TRY(q2_apply(batch, n, ld, reinterpret_cast<const float*>(work + L.off_q2), V, s));
+    }
+    // reflector of column j: unit at row j + off (one stage: the
+    // subdiagonal; two stages: the band edge), columns with reflectors
+    // 0 .. nref-1
+    const int off = L.two ? kBandNB : 1;
+    int nref = n - 1;
+    if (L.two) {
+        int np = 0;
+        for (int j0 = 0; j0 + kBandNB < n - 1; j0 += kBandNB) ++np;
+        nref = np * kBandNB;
+    }
     const int npass = int(ceil_div(nref, kBtPass));
     constexpr int P = kBtPass;
     const int64_t pp = int64_t(P) * P, pn = int64_t(P) * n;
@@ -1646,10 +1705,10 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
         const int j0 = pass * P;
         const int kb = std::min(P, nref - j0);
         const int kb1 = std::min(kBtNB, kb), kb2 = kb - kb1;
-        const int mrows = n - j0 - 1;
+        const int mrows = n - j0 - off;
         if (mrows <= 0 || kb <= 0) continue;
-        const T* Y = A + size_t(j0) * ld + (j0 + 1);  // Y(r, l) = Y[l*ld + r]
-        T* Zs = V + size_t(j0 + 1) * ld;                // rows j0+1.. of V
+        const T* Y = A + size_t(j0) * ld + (j0 + off);  // Y(r, l) = Y[l*ld + r]
+        T* Zs = V + size_t(j0 + off) * ld;                // rows j0+off.. of V
         // C(m, n) = alpha op(A) B + beta C on the per-matrix arrays; the
         // strides are (m, k) of A, (k, n) of B, (m, n) of C
         auto mm = [&](int M, int N, int K, const T* a, int64_t a_m, int64_t a_k, int64_t a_b, const T* b,
@@ -1705,7 +1764,7 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
         // Z -= Y W2, with Y copied k-contiguous into W1 (free once W2 is formed)
         {
             dim3 grid(unsigned(ceil_div(mrows, 64)), unsigned(ceil_div(kb, 64)), unsigned(batch));
-            hipLaunchKernelGGL(yt_kernel<T>, grid, dim3(256), 0, s, n, ld, j0, kb, mrows, A, W1, pn);
+            hipLaunchKernelGGL(yt_kernel<T>, grid, dim3(256), 0, s, n, ld, j0, off, kb, mrows, A, W1, pn);
             CIR_CHECK_LAUNCH();
         }
         CIR_TRY(mm(mrows, n, kb, W1, P, 1, pn, W2, 1, P, pn, Zs, ld, 1, mat, -1.0, 1.0));

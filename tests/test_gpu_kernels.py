@@ -2,6 +2,7 @@
 numpy and a plain torch float64 reference."""
 
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -167,6 +168,49 @@ def test_eigh_batched_panel_split_between_lds_and_global(prec):
         assert np.abs(w[b] - wn).max() <= 50 * eps * scale * np.sqrt(n), b
         Vb = V[b][:, :n]
         res = np.abs(Hc[b] @ Vb - Vb * w[b]).max() / scale
+        orth = np.abs(Vb.T @ Vb - np.eye(n)).max()
+        assert res < vtol, (b, res)
+        assert orth < vtol, (b, orth)
+
+
+@pytest.mark.parametrize("prec", [C.PREC_FP32, C.PREC_BF16X3], ids=["fp32", "bf16x3"])
+@pytest.mark.parametrize("n", [67, 384, 449, 1000, 2048])
+def test_eigh_two_stage_against_lapack(n, prec):
+    # batches of more than 16 FP32 matrices of n >= 384 are reduced in two
+    # stages (dense -> band of half-width 32 -> tridiagonal, bulge-chasing and
+    # panel reflectors back-transformed); smaller n (down to 66) takes that
+    # path when CIR_EIGH_STAGES=2.  Hamiltonian-like spectra (diagonal spread
+    # 600, couplings ~30, a few strong-field outliers) against LAPACK:
+    # eigenvalues, residual, orthogonality.
+    rng = np.random.default_rng(n)
+    batch = 17
+    H = np.zeros((batch, n, n), np.float32)
+    for b in range(batch):
+        a = rng.standard_normal((n, n)) * 30.0
+        d = rng.uniform(-300.0, 300.0, n)
+        d[rng.choice(n, 3, replace=False)] += rng.choice([-1.0, 1.0], 3) * 900.0
+        H[b] = ((a + a.T) / 2 + np.diag(d)).astype(np.float32)
+    announce("eigh two-stage n=%d batch=%d prec=%d" % (n, batch, prec))
+    ld = (n + 3) // 4 * 4
+    Hd = np.zeros((batch, n, ld), np.float32)
+    Hd[:, :, :n] = np.transpose(H, (0, 2, 1))
+    os.environ["CIR_EIGH_STAGES"] = "2"
+    try:
+        w, V = S.eigh(torch.from_numpy(Hd).to(DEV).contiguous(), n, prec=prec)
+    finally:
+        del os.environ["CIR_EIGH_STAGES"]
+    w = w.cpu().numpy()
+    V = V.cpu().double().numpy()
+    eps = 6e-8
+    vtol = 50 * eps * np.sqrt(n) if prec == C.PREC_FP32 else 3e-4
+    for b in (0, 7, 16) if n >= 1000 else range(batch):
+        Hb = H[b].astype(np.float64)
+        wn = np.linalg.eigvalsh(Hb)
+        scale = np.abs(wn).max()
+        assert np.all(np.diff(w[b]) >= 0), b
+        assert np.abs(w[b] - wn).max() <= 50 * eps * scale * np.sqrt(n), (b, np.abs(w[b] - wn).max() / scale)
+        Vb = V[b][:, :n]
+        res = np.abs(Hb @ Vb - Vb * w[b]).max() / scale
         orth = np.abs(Vb.T @ Vb - np.eye(n)).max()
         assert res < vtol, (b, res)
         assert orth < vtol, (b, orth)
