@@ -4,9 +4,12 @@ code (round trips at several precisions, the large-range path, water runs),
 frame subsets, appends and corrupt input.
 
 No XTC file written by GROMACS is available in this environment, so the
-decoder is pinned by (a) a byte-exact hand-assembled uncompressed frame and
-(b) round trips through the writer of the same published code; agreement
-with GROMACS-written files is parity unpinned."""
+decoder is pinned by (a) a byte-exact hand-assembled uncompressed frame, (b)
+a compressed 10-atom frame assembled bit by bit from the published
+xdr3dfcoord layout (mixed-radix triples, run flags, a run of small
+differences with the first-pair exchange), independent of this project's
+writer, and (c) round trips through the writer.  Agreement with files that
+GROMACS itself wrote stays parity unpinned (none exist here)."""
 
 import os
 import struct
@@ -261,3 +264,106 @@ def test_stored_precision_is_reported(tmp_path, precision):
     g = tmp_path / "small.xtc"
     C.xtc_write(g, xyz[:, :8], box, times, precision=precision)
     assert C.xtc_precision(g, 1) == -1.0
+
+
+class _Bits:
+    """Bit stream as the xdr3dfcoord code writes it: each field's bits most
+    significant first, packed into bytes from their top bit."""
+
+    def __init__(self):
+        self.bits = []
+
+    def put(self, nbits, value):
+        for k in range(nbits - 1, -1, -1):
+            self.bits.append((value >> k) & 1)
+
+    def bytes(self):
+        b = self.bits + [0] * (-len(self.bits) % 8)
+        return bytes(int("".join(map(str, b[i:i + 8])), 2) for i in range(0, len(b), 8))
+
+
+def _pack_ints(bits, nbits, sizes, nums):
+    """The published mixed-radix packing of a triple: the number
+    (n0 * s1 + n1) * s2 + n2, its bytes least significant first, 8 bits each
+    (most significant bit first), the last byte with the remaining
+    nbits - 8 (nbytes - 1) bits."""
+    v = (nums[0] * sizes[1] + nums[1]) * sizes[2] + nums[2]
+    nbytes = max(1, (v.bit_length() + 7) // 8)
+    by = [(v >> (8 * k)) & 0xFF for k in range(nbytes)]
+    if nbits >= 8 * nbytes:
+        for x in by:
+            bits.put(8, x)
+        bits.put(nbits - 8 * nbytes, 0)
+    else:
+        for x in by[:-1]:
+            bits.put(8, x)
+        bits.put(nbits - 8 * (nbytes - 1), by[-1])
+
+
+def _bits_of_product(sizes):
+    """Bits of a packed triple: the bit length of s0 s1 s2 (the format's
+    count, one more than needed when the product is a power of two)."""
+    return (sizes[0] * sizes[1] * sizes[2]).bit_length()
+
+
+def test_hand_assembled_compressed_frame(tmp_path):
+    # A 10-atom frame at precision 1000 (the compressed path: more than 9
+    # atoms), assembled bit by bit from the published xdr3dfcoord layout
+    # rather than by this project's writer.  Atoms 0..5 are coded as plain
+    # offsets from minint (one mixed-radix triple each), the first with a
+    # run flag (1, then run + is_smaller + 1 = 1 in 5 bits), the others with
+    # flag 0; atom 6 is a plain triple with a run of two small atoms (flag 1,
+    # value 7), each small triple in smallidx bits relative to the previous
+    # atom with offset smallnum = magicints[smallidx] / 2, the first one
+    # exchanged with the plain atom (the decoder emits it first); atom 9 is a
+    # plain triple with flag 1, value 1 (run 0 again).
+    prec = 1000.0
+    plain = [(1234, -567, 890), (3000, 2500, -100), (-1500, 40, 2222), (777, 3333, 1500), (0, 0, 0),
+             (2999, -999, 1111), (500, 600, 700), (-1200, 3100, -50)]
+    smallidx = 9          # magicints[9] = 8: triples in [0, 8)^3 packed in 9 bits
+    smallnum = 4
+    small = [(1, 7, 3), (6, 0, 5)]  # the two small triples after atom 6
+    lint = [np.array(c) for c in plain]
+    minint = np.min(lint, axis=0)
+    maxint = np.max(lint, axis=0)
+    sizes = [int(x) for x in (maxint - minint + 1)]
+    bitsize = _bits_of_product(sizes)
+    bits = _Bits()
+
+    def put_plain(c, flag_value):
+        _pack_ints(bits, bitsize, sizes, [int(x) for x in (np.array(c) - minint)])
+        if flag_value is None:
+            bits.put(1, 0)
+        else:
+            bits.put(1, 1)
+            bits.put(5, flag_value)
+
+    put_plain(plain[0], 1)
+    for c in plain[1:6]:
+        put_plain(c, None)
+    put_plain(plain[6], 7)
+    for t in small:
+        _pack_ints(bits, smallidx, [8, 8, 8], list(t))
+    put_plain(plain[7], 1)
+    payload = bits.bytes()
+    # expected decoding: atoms 0..5 plain; then S1 = small0 + atom6 - smallnum
+    # emitted before atom 6; S2 = small1 + S1 - smallnum; then the last plain
+    s1 = np.array(small[0]) + np.array(plain[6]) - smallnum
+    s2 = np.array(small[1]) + s1 - smallnum
+    expect = [np.array(c) for c in plain[:6]] + [s1, np.array(plain[6]), s2, np.array(plain[7])]
+
+    frame = struct.pack(">iiif", 1995, 10, 7, 0.5) + struct.pack(">9f", 3.0, 0, 0, 0, 3.5, 0, 0, 0, 4.0)
+    frame += struct.pack(">i", 10) + struct.pack(">f", prec)
+    frame += struct.pack(">3i", *[int(x) for x in minint]) + struct.pack(">3i", *[int(x) for x in maxint])
+    frame += struct.pack(">i", smallidx) + struct.pack(">i", len(payload))
+    frame += payload + b"\0" * (-len(payload) % 4)
+    f = tmp_path / "hand.xtc"
+    f.write_bytes(frame + frame)  # two identical frames
+    assert C.xtc_scan(f) == (10, 2)
+    xyz, box, times, steps = C.xtc_read(f)
+    inv = np.float32(1.0 / prec)
+    want = (np.array(expect, np.int64).astype(np.float32) * inv).astype(np.float32)
+    assert np.array_equal(xyz[0], want), (xyz[0] / inv, np.array(expect))
+    assert np.array_equal(xyz[1], want)
+    assert list(steps) == [7, 7] and np.array_equal(box[0], np.float32([3.0, 3.5, 4.0]))
+    assert C.xtc_precision(f, 0) == prec
