@@ -37,6 +37,7 @@ WORK = dict(species="H2O", model="tip4p2005", map=0, ntcfpoints=150, sampleEvery
             omegaStart=2800.0, omegaStop=4000.0, omegaStep=5, nzeros=1898, t1=0.26, dt=0.010,
             cutoff=0.7831)
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def nsamples_for(nframes):
@@ -44,25 +45,23 @@ def nsamples_for(nframes):
     return (nframes - 150) // 20 + 1
 
 
-def tridiag_bytes(n, esz=4, nb=32):
-    """HBM bytes per matrix of the one-stage panel-blocked tridiagonalisation
-    on lower-triangle storage: each reflector's symmetric product streams the
-    lower triangle of the trailing (n-j-1)-square once; each panel's trailing
-    update reads and writes the lower triangle of the remaining block."""
-    m = np.arange(n - 1, 0, -1, dtype=np.float64)
-    sym = float(np.sum(m * (m + 1) / 2)) * esz
-    upd = 0.0
-    for j0 in range(0, n - 1, nb):
-        r = n - min(j0 + nb, n - 1)
-        upd += 2.0 * (r * (r + 1) / 2) * esz
-    return sym + upd
+def bulge_bt_flops(n, nb=32):
+    """Algorithmic flops per matrix of applying the bulge-chasing reflectors
+    to the n eigenvectors: sweep s, task k has a reflector of length
+    min(nb, n - r0), r0 = s + 1 + k nb, and H = I - tau v v^T costs a dot and
+    an update (4 len flops) per column."""
+    tot = 0
+    for s in range(n - 2):
+        for r0 in range(s + 1, n - 1, nb):
+            tot += min(nb, n - r0)
+    return 4.0 * n * tot
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tridiag_kernel_2048oh_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "q2_apply_kernel_2048oh_traffic.json")
 
 
 def traffic_per_launch(matrices_per_launch):
-    """HBM bytes per tridiag_kernel launch from the committed counter
+    """HBM bytes per q2_apply_kernel launch from the committed counter
     measurement (FETCH_SIZE + WRITE_SIZE per matrix, each scaled by the same
     counter's reading for a 1 GiB copy); None when that file is absent."""
     try:
@@ -105,6 +104,61 @@ def spectrum_error(p, res, ref):
 
 
 CPU_POINTS = 8
+
+
+def host_info():
+    """Threads the CPU legs use and what the host is: the GPU box allots a
+    share of its cores to each GPU and says so in OMP_NUM_THREADS, which the
+    CPU legs follow (os.cpu_count() where it is unset)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or affinity
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": threads, "nproc": nproc, "affinity": affinity, "model": model}
+
+
+def cpu_c1(C, nthreads):
+    """The 256-OH workload (128 waters, L = 1.564 nm, 200 frames, 6 origins x
+    150 points, cutoff 0.75 nm) in full: the FP32 CPU oracle with the
+    reference-literal propagation beside the GPU plain FP32 and batched BF16x3
+    paths on the same frames."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    xyz, box, times = C.synth_trajectory(128, 200, 1001, box_nm=1.564)
+    kw = dict(WORK, cutoff=0.75, sampleEvery=0.1, nsamples=6)
+    p = C.default_params(precision="fp32", mode="plain", **kw)
+    cfg = O.config_from_params(p)
+    cfg.propagation = O.PROP_LITERAL
+    cfg.nthreads = nthreads
+    print("[bench] 256-OH workload on the CPU oracle (%d threads)" % nthreads, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    r = O.run(O.F32, cfg, xyz, box, times)
+    tcpu = time.perf_counter() - t0
+    out = {"workload": "128 waters (256 OH), L=1.564 nm, 200 frames, 6 origins x 150 points, cutoff 0.75 nm",
+           "cpu_oracle_fp32": {"seconds": tcpu, "frames_per_s": r["nsteps"] / tcpu, "threads": nthreads,
+                               "out_of_window_modes": int(r["hist_skipped"])}}
+    for name, prec, mode in (("gpu_plain_fp32", "fp32", "plain"), ("gpu_batched_bf16x3", "bf16x3", "batched")):
+        pg = C.default_params(precision=prec, mode=mode, **kw)
+        C.run(pg, xyz, box, times)  # warm
+        t0 = time.perf_counter()
+        g = C.run(pg, xyz, box, times)
+        tg = time.perf_counter() - t0
+        ft = g["ftcf_ir"]
+        err = float(np.linalg.norm(ft - r["ftcf_ir"]) / np.linalg.norm(r["ftcf_ir"]))
+        out[name] = {"seconds": tg, "frames_per_s": g["nsteps"] / tg, "ir_c2r_rel_l2_vs_cpu_fp32": err}
+    return out
 
 
 def cpu_baseline(xyz, box, times, nthreads):
@@ -163,7 +217,7 @@ def adjoint_bench(C):
     return out
 
 
-def uneven_boxes_bench(C, nbox=16, nframes=200):
+def uneven_boxes_bench(C, nbox=64, nframes=500):
     """Independent boxes of uneven size (water counts drawn from
     {64,128,256,512,768,1024}, seeded), one after another and at once
     (cir_run_many); BF16x3 batched path, origins every 10 frames."""
@@ -174,20 +228,27 @@ def uneven_boxes_bench(C, nbox=16, nframes=200):
     for b, nw in enumerate(counts):
         xyz, box, times = C.synth_trajectory(int(nw), nframes, 3000 + b, box_nm=C.synth_box_nm(int(nw)))
         jobs.append((C.default_params(**kw), xyz, box, times))
-    print("[bench] uneven boxes", file=sys.stderr, flush=True)
+    print("[bench] uneven boxes (%d x %d frames)" % (nbox, nframes), file=sys.stderr, flush=True)
     C.run(*jobs[-1])  # warm
     t0 = time.perf_counter()
-    for j in jobs:
+    for j in jobs[:16]:
         C.run(*j)
-    tseq = time.perf_counter() - t0
+    tseq16 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res16 = C.run_many(jobs[:16], nthreads=8)
+    tmany16 = time.perf_counter() - t0
     t0 = time.perf_counter()
     res = C.run_many(jobs, nthreads=8)
     tmany = time.perf_counter() - t0
+    steps16 = sum(int(r["nsteps"]) for r in res16)
     steps = sum(int(r["nsteps"]) for r in res)
     return {"boxes": nbox, "frames_per_box": nframes, "waters": [int(c) for c in counts],
-            "sequential_frames_per_s": steps / tseq, "concurrent_frames_per_s": steps / tmany,
-            "concurrent_speedup": tseq / tmany, "sample": "%d boxes x %d frames (host arrays, "
-            "transfers included)" % (nbox, nframes)}
+            "frames_per_s": steps / tmany, "trajectory_frames_per_s": sum(int(r["nframes_used"]) for r in res) / tmany,
+            "seconds": tmany, "steps": steps,
+            "first16_sequential_frames_per_s": steps16 / tseq16, "first16_concurrent_frames_per_s": steps16 / tmany16,
+            "first16_concurrent_speedup": tseq16 / tmany16,
+            "sample": "%d boxes x %d frames, origins every 10 frames, bf16x3 (host arrays, transfers included)"
+                      % (nbox, nframes)}
 
 
 def large_box_bench(C):
@@ -223,6 +284,7 @@ def main():
     ap.add_argument("--plain-points", type=int, default=4, help="tcf points of the plain-FP32 subset")
     ap.add_argument("--no-extras", action="store_true", help="only the timed path (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-large-box", action="store_true", help="skip the 8192-OH box among the extras")
     ap.add_argument("--large-box", action="store_true",
                     help="only the 8192-OH box (1000 frames) and its FP64 subset check, as one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -277,25 +339,42 @@ def main():
 
     n = 2 * NWATER
     nmat = res["nframes_used"] * args.steps
-    tri_s, tri_n = phases["tridiag"]
-    per_launch = tri_s / max(tri_n, 1)
-    bytes_launch = tridiag_bytes(n) * nmat / max(tri_n, 1)
-    achieved = bytes_launch / per_launch / 1e9
+    # the dominant kernel: q2_apply_kernel (the bulge-chasing reflectors
+    # applied to the eigenvectors on FP32 MFMA), alone in its phase, one
+    # launch per chunk of matrices
+    bt_s, bt_n = phases["bulge_backtransform"]
+    per_launch = bt_s / max(bt_n, 1)
+    mats_launch = nmat / max(bt_n, 1)
+    fl_bt = bulge_bt_flops(n)
+    achieved = fl_bt * mats_launch / max(per_launch, 1e-12) / 1e12
+    # the whole eigensolver on its algorithmic flops: 4/3 N^3 for the
+    # reduction to tridiagonal form and 2 N^3 for the back-transform
+    eig_keys = ("tridiag", "bulge", "bisect", "invit", "bulge_backtransform", "backtransform")
+    eig_s = sum(phases[k][0] for k in eig_keys)
+    red_s = phases["tridiag"][0] + phases["bulge"][0]
+    fl_red, fl_eig = 4.0 / 3.0 * n ** 3, (4.0 / 3.0 + 2.0) * n ** 3
     out = {
         "metric": METRIC, "value": value, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "2048-OH box: 1024 TIP4P/2005 waters, L=3.128 nm, %d seeded OU frames at 10 fs, "
                                "cutoff 0.7831 nm, 150-point TCF, origins every 20 frames (%d samples)" % (nframes, ns),
-                   "path": "batched", "precision": "f32 field/H/tridiagonalisation, f64 bisection + inverse "
-                                                   "iteration, split-bf16x3 MFMA propagator",
+                   "path": "batched", "precision": "f32 field/H/reduction to tridiagonal form, f64 bisection + "
+                                                   "inverse iteration, split-bf16x3 MFMA propagator",
                    "nsteps_per_run": steps_per_run, "distinct_frames": res["nframes_used"]},
-        "roofline": {"kernel": "tridiag_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(nmat / max(tri_n, 1)),
-                     "avg_launch_ms": 1e3 * per_launch, "matrices_per_launch": nmat / max(tri_n, 1),
-                     "bytes_per_matrix": tridiag_bytes(n)},
+        "roofline": {"kernel": "q2_apply_kernel", "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic_per_launch(mats_launch), "avg_launch_ms": 1e3 * per_launch,
+                     "matrices_per_launch": mats_launch, "flops_per_matrix": fl_bt},
         "phases_s_per_run": {k: v[0] / args.steps for k, v in phases.items()},
         "trajectory_frames_per_s": nmat / elapsed,
+        "eigensolver": {"matrices": nmat, "n": n, "seconds": eig_s, "ms_per_matrix": 1e3 * eig_s / nmat,
+                        "reduction_flops_per_matrix": fl_red, "reduction_s": red_s,
+                        "reduction_tflops": fl_red * nmat / max(red_s, 1e-12) / 1e12,
+                        "reduction_frac_fp32_peak": fl_red * nmat / max(red_s, 1e-12) / 1e12 / FP32_PEAK_TFLOPS,
+                        "eigensolve_flops_per_matrix": fl_eig,
+                        "eigensolve_tflops": fl_eig * nmat / max(eig_s, 1e-12) / 1e12,
+                        "eigensolve_frac_fp32_peak": fl_eig * nmat / max(eig_s, 1e-12) / 1e12 / FP32_PEAK_TFLOPS},
     }
     if not args.no_extras:
         print("[bench] fp32 batched", file=sys.stderr, flush=True)
@@ -340,9 +419,15 @@ def main():
         del r32, r64, rp
         out["adjoint_1024oh"] = adjoint_bench(C)
         out["uneven_boxes"] = uneven_boxes_bench(C)
+        if not args.no_large_box:
+            out["large_box_8192oh"] = large_box_bench(C)
     if not args.no_cpu:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(xyz, box, times, min(16, os.cpu_count() or 1))
+        host = host_info()
+        print("[bench] cpu baseline (%d threads)" % host["threads"], file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(xyz, box, times, host["threads"])
+        out["cpu_baseline"]["host"] = host
+        if not args.no_extras:
+            out["workload_256oh_full"] = cpu_c1(C, host["threads"])
     print(json.dumps(out), flush=True)
     return 0
 

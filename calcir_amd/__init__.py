@@ -29,7 +29,7 @@ CLI_PATH = os.path.join(HERE, "bin", "calcir")
 
 PREC_FP32, PREC_FP64, PREC_BF16X3 = 0, 1, 2
 PHASES = ("sites", "hamiltonian", "tridiag", "bisect", "invit", "backtransform", "ipr", "sw", "propagate",
-          "correlate", "finalize")
+          "correlate", "finalize", "bulge", "bulge_backtransform")
 MODE_PLAIN, MODE_BATCHED = 0, 1
 NSITE = 20
 SITE_E, SITE_W, SITE_X, SITE_P, SITE_MUP, SITE_U, SITE_R, SITE_MU, SITE_A = 0, 1, 2, 3, 4, 5, 8, 11, 14

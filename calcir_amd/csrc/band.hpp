@@ -31,16 +31,19 @@ inline int64_t q2_blocks(int n) { return q2_block_offset(n, q2_groups(n)); }
 // floats per block: V (kQ2R x kQ2G, column per reflector), tau (kQ2G), -V T (kQ2R x kQ2G)
 constexpr int kQ2BlockFloats = 2 * kQ2R * kQ2G + kQ2G;
 
-// Stage 1 for a batch: A (n x n, ld, both triangles) -> band rows BND
+// Stage 1 for a batch: A (n x n, ld, lower triangle used) -> band rows BND
 // (n x kBandW each), reflectors in A and tau (n each).  Work arrays per
-// matrix: P1, P2 pm x 2NB, Yh pm x NB, Mm NB x NB (pm >= n).
-int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P1, float* P2, float* Yh, float* Mm,
-          int pm, hipStream_t s);
+// matrix: P n x 2NB, Yh n x NB, Mm NB x NB.
+int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P, float* Yh, float* Mm, hipStream_t s);
 // Stage 2: band rows -> d, e (n each; e[n-1] = 0) and the bulge-chasing
-// reflectors in compact-WY block form Q2 (q2_blocks(n) * kQ2BlockFloats).
+// reflectors by blocks Q2 (q2_blocks(n) * kQ2BlockFloats).
 int sb2st(int batch, int n, float* BND, float* d, float* e, float* Q2, hipStream_t s);
+// The blocks of Q2 into compact-WY form (T factors, operand order of
+// q2_apply), in place.
+int q2_prep(int batch, int n, float* Q2, hipStream_t s);
 // Z <- Q2 Z for the eigenvectors Z of the tridiagonal matrix (row-major,
-// Z[i * ld + k] = component i of vector k), batch stride n * ld.
+// Z[i * ld + k] = component i of vector k), batch stride n * ld; Q2 as
+// q2_prep leaves it.
 int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s);
 
 }  // namespace cir

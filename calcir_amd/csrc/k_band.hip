@@ -6,9 +6,10 @@
 //     for each panel of kBandNB columns, sb_panel_kernel factors the panel
 //     below the band, A(r0:n, j0:j0+NB) = Q R, by Householder reflectors in
 //     registers (one workgroup per matrix); the trailing matrix then gets the
-//     two-sided update A22 <- Q^T A22 Q as three batched MFMA products over
-//     every matrix at once (X = A22 Yh with Yh = Y T, W = X - Y (Yh^T X)/2,
-//     A22 -= [W Y] [Y W]^T).  A is kept symmetric in both triangles.
+//     two-sided update A22 <- Q^T A22 Q on its lower triangle, batched over
+//     every matrix: X = A22 Yh with Yh = Y T (sb_symm_kernel), W = X -
+//     Y (Yh^T X)/2 (two small products), A22 -= W Y^T + Y W^T
+//     (sb_syr2k_kernel), on 32x32x2 FP32 MFMA.
 //   Stage 2, band -> tridiagonal (sb2st_kernel): bulge chasing, one workgroup
 //     per matrix, sweeps in groups of kSbG run by the group's waves in
 //     lock-step with a lag of two tasks (the tasks of one step touch disjoint
@@ -57,6 +58,14 @@ __device__ __forceinline__ float bwave_sum(float x) {
     return __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
 }
 
+// Make this wave's LDS stores visible to its own later LDS loads (a wave's LDS
+// operations complete in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wave sums of 32 per-lane values x[k], written to out[k] (LDS).  Two lane
 // swaps halve the values per lane (after them lane l holds the partials of
 // columns k + 8 (l >> 4), k < 8), 16-lane row sums finish them, and lane
@@ -88,22 +97,21 @@ __device__ __forceinline__ void wave_colsums32(const float (&x)[32], int lane, f
 // registers.  Per column c: one block sum for the norm, one for the 32
 // products v^T P(:, c') (the trailing columns' w and, for c' < c, the Gram
 // column Y^T v of the T factor).  Outputs: the diagonal block and R into BND,
-// tau, Y into A (back-transform layout) and into P1 (columns NB..2NB) and P2
-// (columns 0..NB), Yh = Y T into Yh; pitch pm.
+// tau, Y into A (back-transform layout) and into the Y half of the row-major
+// panel P (absolute rows, pitch 2 NB), Yh = Y T row-major (panel rows,
+// pitch NB).
 template <int RPT>
 __global__ void __launch_bounds__(kPanNT) sb_panel_kernel(int n, int ld, int j0, float* __restrict__ Aall,
                                                           float* __restrict__ BNDall, float* __restrict__ tauall,
-                                                          float* __restrict__ P1all, float* __restrict__ P2all,
-                                                          float* __restrict__ Yhall, int pm) {
+                                                          float* __restrict__ Pall, float* __restrict__ Yhall) {
     constexpr int NB = kBandNB;
     __shared__ float redA[kPanNW], redC[kPanNW][NB], dotv[NB], taus[NB], salpha;
     __shared__ float gram[NB][NB + 1], Tm[NB][NB + 1];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float* A = Aall + size_t(b) * n * ld;
     float* BND = BNDall + size_t(b) * n * kBandW;
-    float* P1 = P1all + size_t(b) * pm * 2 * NB;
-    float* P2 = P2all + size_t(b) * pm * 2 * NB;
-    float* Yh = Yhall + size_t(b) * pm * NB;
+    float* P = Pall + size_t(b) * n * 2 * NB;
+    float* Yh = Yhall + size_t(b) * n * NB;
     const int r0 = j0 + NB, m = n - r0;
 
     // the panel's diagonal block is final: lower triangle into the band rows
@@ -227,32 +235,181 @@ __global__ void __launch_bounds__(kPanNT) sb_panel_kernel(int n, int ld, int j0,
 #pragma unroll
         for (int c = 0; c < NB; ++c) y[c] = (row < c) ? 0.f : (row == c ? 1.f : p[i][c]);
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            A[size_t(j0 + c) * ld + r0 + row] = y[c];
-            P1[size_t(NB + c) * pm + row] = y[c];
-            P2[size_t(c) * pm + row] = y[c];
-        }
+        for (int c = 0; c < NB; ++c) A[size_t(j0 + c) * ld + r0 + row] = y[c];
+        float4* pr = reinterpret_cast<float4*>(P + size_t(r0 + row) * 2 * NB);
+        float4* yr = reinterpret_cast<float4*>(Yh + size_t(row) * NB);
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) pr[q] = float4{y[4 * q], y[4 * q + 1], y[4 * q + 2], y[4 * q + 3]};
+        float yh[NB];
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
             float acc = 0.f;
 #pragma unroll
             for (int q = 0; q <= c; ++q) acc += y[q] * Tm[q][c];
-            Yh[size_t(c) * pm + row] = acc;
+            yh[c] = acc;
         }
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) yr[q] = float4{yh[4 * q], yh[4 * q + 1], yh[4 * q + 2], yh[4 * q + 3]};
     }
 }
 
-// W (P1 columns 0..NB) copied into P2 columns NB..2NB, so that the rank-2NB
-// update reads [W Y] and [Y W] as plain column-major operands.
-__global__ void __launch_bounds__(256) sb_copyw_kernel(int m, int pm, float* __restrict__ P1all,
-                                                       float* __restrict__ P2all) {
-    const int b = blockIdx.y;
-    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (e >= int64_t(m) * kBandNB) return;
-    const int c = int(e / m), r = int(e % m);
-    const float* P1 = P1all + size_t(b) * pm * 2 * kBandNB;
-    float* P2 = P2all + size_t(b) * pm * 2 * kBandNB;
-    P2[size_t(kBandNB + c) * pm + r] = P1[size_t(c) * pm + r];
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// X = A22 Yh on the lower triangle of A22 (m x m, column-major, pitch ld,
+// first row and column r0), X row-major with pitch 2 NB in the W half of the
+// row-major panel P (rows r0.., P[r * 2NB + NB + p]); Yh row-major, pitch NB
+// (panel rows).  Wave w of workgroup x owns block row I = 4 x + w and reads
+// every tile once from its own side: the row tiles (I, J < I) go straight
+// into the MFMA A operand (lanes along rows, coalesced), the column tiles
+// (J >= I, I) are read along their rows too and transposed through the
+// wave's LDS tile.  The next tile's loads are in flight while a tile's MFMA
+// chain runs.  32x32x2 FP32 MFMA, K = 32 per tile.
+__global__ void __launch_bounds__(256) sb_symm_kernel(int n, int ld, int r0, const float* __restrict__ Aall,
+                                                      const float* __restrict__ Yhall, float* __restrict__ Pall) {
+    constexpr int NB = kBandNB;
+    __shared__ float tl[4][32][33];
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = n - r0, nbt = (m + 31) / 32;
+    const int I = blockIdx.x * 4 + wave;
+    if (I >= nbt) return;
+    const float* A = Aall + size_t(b) * n * ld + size_t(r0) * ld + r0;
+    const float* Yh = Yhall + size_t(b) * n * NB;
+    float* P = Pall + size_t(b) * n * 2 * NB + size_t(r0) * 2 * NB;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int ri = I * 32 + l32;  // this lane's row of block row I (M index)
+    float (*T)[33] = tl[wave];
+    // raw loads of tile J: the B operand (Yh rows J*32 + 2q + h) and the A
+    // tile, as the A operand itself (row tile) or along the tile's rows for
+    // the LDS transpose (column tile, T[2q + h][l32] = A22(J*32 + l32, I*32 + 2q + h))
+    auto load = [&](int J, float (&bv)[16], float (&raw)[16]) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int kr = J * 32 + 2 * q + h;
+            bv[q] = (kr < m) ? Yh[size_t(kr) * NB + l32] : 0.f;
+        }
+        if (J < I) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int kc = J * 32 + 2 * q + h;
+                raw[q] = (ri < m) ? A[size_t(kc) * ld + ri] : 0.f;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int gr = J * 32 + l32, gc = I * 32 + 2 * q + h;
+                raw[q] = (gr < m && gc < m) ? A[size_t(gc) * ld + gr] : 0.f;
+            }
+        }
+    };
+    floatx16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+    float bv[16], raw[16];
+    load(0, bv, raw);
+    for (int J = 0; J < nbt; ++J) {
+        float bn[16], rn[16];
+        if (J + 1 < nbt) load(J + 1, bn, rn);
+        float av[16];
+        if (J < I) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) av[q] = raw[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) T[2 * q + h][l32] = raw[q];
+            wave_lds_sync();
+            // A operand A22(ri, J*32 + k) = A(J*32 + k, ri): T[l32][k]; the
+            // diagonal tile's upper part mirrored from its lower part
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int k = 2 * q + h;
+                av[q] = (J > I || k >= l32) ? T[l32][k] : T[k][l32];
+            }
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[q], bv[q], acc, 0, 0, 0);
+        if (J + 1 < nbt) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                bv[q] = bn[q];
+                raw[q] = rn[q];
+            }
+        }
+    }
+    // acc(i, p): lane l holds column p = l32, rows 8 a + 4 h + c
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int r = I * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+        if (r < m) P[size_t(r) * 2 * NB + NB + l32] = acc[q];
+    }
+}
+
+// A22 -= W Y^T + Y W^T on the lower triangle, from the row-major panel P
+// (P[r * 2NB + k]: Y(r, k) for k < NB, W(r, k - NB) after).  Wave w of
+// workgroup x owns block row br = 4 x + w and every block column bc <= br.
+// The tile is formed transposed (MFMA row = column of A, MFMA column = row
+// of A) so that lanes run along rows of the column-major A; step s of the
+// K = 2NB sum takes panel position 32 h + s from lane half h, the A operand
+// from the panel row of the tile's column at that position, the B operand
+// from the panel row of the tile's row at the swapped half (k ^ NB).  The
+// next tile's A values and operand are in flight while a tile's MFMA chain
+// runs.
+__global__ void __launch_bounds__(256) sb_syr2k_kernel(int n, int ld, int r0, float* __restrict__ Aall,
+                                                       const float* __restrict__ Pall) {
+    constexpr int NB = kBandNB, PW = 2 * kBandNB;
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = n - r0, nbt = (m + 31) / 32;
+    const int br = blockIdx.x * 4 + wave;
+    if (br >= nbt) return;
+    float* A = Aall + size_t(b) * n * ld + size_t(r0) * ld + r0;
+    const float* P = Pall + size_t(b) * n * PW + size_t(r0) * PW;
+    const int h = lane >> 5, l32 = lane & 31;
+    const int r = br * 32 + l32;  // row of A22 this lane's column of the output holds
+    auto panel_row = [&](int row, int half, float (&v)[32]) {
+        const float4* src = reinterpret_cast<const float4*>(P + size_t(row < m ? row : 0) * PW + half * NB);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 t = (row < m) ? src[q] : float4{0.f, 0.f, 0.f, 0.f};
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    };
+    auto load_old = [&](int bc, float (&o)[16]) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = bc * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+            o[q] = (c < m && r < m && r >= c) ? A[size_t(c) * ld + r] : 0.f;
+        }
+    };
+    float bv[32], av[32], old[16];
+    panel_row(r, 1 - h, bv);
+    panel_row(l32, h, av);
+    load_old(0, old);
+    for (int bc = 0; bc <= br; ++bc) {
+        float an[32], on[16];
+        if (bc < br) {
+            panel_row((bc + 1) * 32 + l32, h, an);
+            load_old(bc + 1, on);
+        }
+        floatx16 acc;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+#pragma unroll
+        for (int q = 0; q < 32; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[q], bv[q], acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = bc * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+            if (c < m && r < m && r >= c) A[size_t(c) * ld + r] = old[q] - acc[q];
+        }
+        if (bc < br) {
+#pragma unroll
+            for (int q = 0; q < 32; ++q) av[q] = an[q];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) old[q] = on[q];
+        }
+    }
 }
 
 // Zero the band rows, then the last (unfactored) diagonal block's lower
@@ -275,11 +432,9 @@ __global__ void __launch_bounds__(256) sb_band_tail_kernel(int n, int ld, int jl
     }
 }
 
-int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P1, float* P2, float* Yh, float* Mm,
-          int pm, hipStream_t s) {
+int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P, float* Yh, float* Mm, hipStream_t s) {
     constexpr int NB = kBandNB;
     if (n > sb_max_n()) return fail(CIR_ERR_ARG, "sy2sb: n too large for the register panel");
-    if (pm < n) return fail(CIR_ERR_ARG, "sy2sb: panel pitch below n");
     {
         dim3 grid(unsigned(ceil_div(int64_t(n) * kBandW, 256)), unsigned(batch));
         hipLaunchKernelGGL(sb_band_zero_kernel, grid, dim3(256), 0, s, n, BND);
@@ -291,104 +446,66 @@ int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P1,
         const int rpt = int(ceil_div(m, kPanNT));
         if (rpt <= 1)
             hipLaunchKernelGGL(sb_panel_kernel<1>, dim3(unsigned(batch)), dim3(kPanNT), 0, s, n, ld, j0, A, BND, tau,
-                               P1, P2, Yh, pm);
+                               P, Yh);
         else if (rpt <= 2)
             hipLaunchKernelGGL(sb_panel_kernel<2>, dim3(unsigned(batch)), dim3(kPanNT), 0, s, n, ld, j0, A, BND, tau,
-                               P1, P2, Yh, pm);
+                               P, Yh);
         else if (rpt <= 3)
             hipLaunchKernelGGL(sb_panel_kernel<3>, dim3(unsigned(batch)), dim3(kPanNT), 0, s, n, ld, j0, A, BND, tau,
-                               P1, P2, Yh, pm);
+                               P, Yh);
         else
             hipLaunchKernelGGL(sb_panel_kernel<4>, dim3(unsigned(batch)), dim3(kPanNT), 0, s, n, ld, j0, A, BND, tau,
-                               P1, P2, Yh, pm);
+                               P, Yh);
         CIR_CHECK_LAUNCH();
-        const int64_t mat = int64_t(n) * ld, pstride = int64_t(pm) * 2 * NB;
-        float* A22 = A + size_t(r0) * ld + r0;
+        const int nbt = int(ceil_div(m, 32));
+        const dim3 grid(unsigned(ceil_div(nbt, 4)), unsigned(batch));
+        // X = A22 Yh into the W half of P
+        hipLaunchKernelGGL(sb_symm_kernel, grid, dim3(256), 0, s, n, ld, r0, A, Yh, P);
+        CIR_CHECK_LAUNCH();
+        const int64_t ps = int64_t(n) * 2 * NB;
         GemmArgs g;
-        g.batch = batch;
-        // X = A22 Yh into P1 columns 0..NB
-        g.M = m;
-        g.N = NB;
-        g.K = m;
-        g.A = A22;
-        g.a_m = 1;
-        g.a_k = ld;
-        g.a_b = mat;
-        g.B = Yh;
-        g.b_k = 1;
-        g.b_n = pm;
-        g.b_b = int64_t(pm) * NB;
-        g.C = P1;
-        g.c_m = 1;
-        g.c_n = pm;
-        g.c_b = pstride;
-        CIR_TRY(gemm(CIR_PREC_FP32, g, s));
         // Mm = Yh^T X (NB x NB)
-        g = GemmArgs{};
         g.batch = batch;
         g.M = NB;
         g.N = NB;
         g.K = m;
         g.A = Yh;
-        g.a_m = pm;
-        g.a_k = 1;
-        g.a_b = int64_t(pm) * NB;
-        g.B = P1;
-        g.b_k = 1;
-        g.b_n = pm;
-        g.b_b = pstride;
+        g.a_m = 1;
+        g.a_k = NB;
+        g.a_b = int64_t(n) * NB;
+        g.B = P + size_t(r0) * 2 * NB + NB;
+        g.b_k = 2 * NB;
+        g.b_n = 1;
+        g.b_b = ps;
         g.C = Mm;
         g.c_m = NB;
         g.c_n = 1;
         g.c_b = int64_t(NB) * NB;
         CIR_TRY(gemm(CIR_PREC_FP32, g, s));
-        // W = X - Y Mm / 2 in place of X (Y: P2 columns 0..NB)
+        // W = X - Y Mm / 2 in place of X
         g = GemmArgs{};
         g.batch = batch;
         g.M = m;
         g.N = NB;
         g.K = NB;
-        g.A = P2;
-        g.a_m = 1;
-        g.a_k = pm;
-        g.a_b = pstride;
+        g.A = P + size_t(r0) * 2 * NB;
+        g.a_m = 2 * NB;
+        g.a_k = 1;
+        g.a_b = ps;
         g.B = Mm;
         g.b_k = NB;
         g.b_n = 1;
         g.b_b = int64_t(NB) * NB;
-        g.C = P1;
-        g.c_m = 1;
-        g.c_n = pm;
-        g.c_b = pstride;
+        g.C = P + size_t(r0) * 2 * NB + NB;
+        g.c_m = 2 * NB;
+        g.c_n = 1;
+        g.c_b = ps;
         g.alpha = -0.5;
         g.beta = 1.0;
         CIR_TRY(gemm(CIR_PREC_FP32, g, s));
-        {
-            dim3 grid(unsigned(ceil_div(int64_t(m) * NB, 256)), unsigned(batch));
-            hipLaunchKernelGGL(sb_copyw_kernel, grid, dim3(256), 0, s, m, pm, P1, P2);
-            CIR_CHECK_LAUNCH();
-        }
-        // A22 -= [W Y] [Y W]^T (both triangles)
-        g = GemmArgs{};
-        g.batch = batch;
-        g.M = m;
-        g.N = m;
-        g.K = 2 * NB;
-        g.A = P1;
-        g.a_m = 1;
-        g.a_k = pm;
-        g.a_b = pstride;
-        g.B = P2;
-        g.b_k = pm;
-        g.b_n = 1;
-        g.b_b = pstride;
-        g.C = A22;
-        g.c_m = 1;
-        g.c_n = ld;
-        g.c_b = mat;
-        g.alpha = -1.0;
-        g.beta = 1.0;
-        CIR_TRY(gemm(CIR_PREC_FP32, g, s));
+        // A22 -= W Y^T + Y W^T (lower triangle)
+        hipLaunchKernelGGL(sb_syr2k_kernel, grid, dim3(256), 0, s, n, ld, r0, A, P);
+        CIR_CHECK_LAUNCH();
     }
     // the last diagonal block (n - j0 <= NB + 1 rows) is already banded
     hipLaunchKernelGGL(sb_band_tail_kernel, dim3(unsigned(batch)), dim3(256), 0, s, n, ld, j0, A, BND);
@@ -406,23 +523,18 @@ namespace cir {
 constexpr int kSbG = 8;                  // sweeps per group = waves per workgroup
 constexpr int kSbNT = 64 * kSbG;
 constexpr int kSbWR = 512;               // window rows (power of two)
-constexpr int kSbWP = kBandW + 1;        // window row pitch (floats)
-constexpr size_t kSbSmemFloats = size_t(kSbWR) * kSbWP + size_t(kSbG) * 2 * kBandNB;
+constexpr int kSbWP = kBandW;            // window row pitch (floats)
+// window, per-wave task scratch (v, two reduced vectors, a per-lane sink),
+// and one 16-byte sink for the read-ahead pieces of threads with no row to add
+constexpr int kSbScratch = 3 * kBandNB + 64;
+constexpr size_t kSbSmemFloats = size_t(kSbWR) * kSbWP + size_t(kSbG) * kSbScratch + 4;
 static_assert(kBandNB * (kBandW / 4) <= kSbNT, "one step's new band rows must fit one 16-byte piece per thread");
 static_assert((2 * kSbG - 1) * kBandNB - kSbG + kBandNB < kSbWR, "window must hold a step's rows plus the read-ahead");
 
-// Make this wave's LDS stores visible to its own later LDS loads (a wave's LDS
-// operations complete in order; this keeps the compiler from moving them).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
-struct SbWin {
-    float* w;
-    __device__ float& at(int r, int c) const { return w[(r & (kSbWR - 1)) * kSbWP + (r - c)]; }
-};
+// Band element (r, c), r - c < kBandW, of a row in the window is at
+// sb_rowbase(r) - c.
+__device__ __forceinline__ int sb_rowbase(int r) { return (r & (kSbWR - 1)) * kSbWP + r; }
 
 // First row a sweep still needs at its task k (k < 0: not started, so its
 // first task's row), or INT_MAX once it is done.
@@ -430,39 +542,81 @@ __device__ __forceinline__ int sb_task_row(int n, int s, int k) {
     return (k < sb_tasks(n, s)) ? s + 1 + max(k, 0) * kBandNB : 0x7fffffff;
 }
 
+// Sum of a value over the two lanes i and i + 32 of a wave, in both.
+__device__ __forceinline__ float half_pair_sum(float x) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+}
+// The value of lane i (i < 32) in lanes i and i + 32.
+__device__ __forceinline__ float from_low_half(float x) {
+    return __uint_as_float(__builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false)[0]);
+}
+
 // One wave's share of a bulge-chasing step: task (s, k) of its sweep.
 // Lane l: row i = l & 31 of the task's block, column half h = l >> 5
 // (columns 16 h .. 16 h + 15).  pv / ptau: the sweep's previous reflector
-// (columns of the previous block, this lane's half), replaced by this task's.
-__device__ void sb_task(int n, int s, int k, const SbWin& W, float* vs, float* ws, float (&pv)[16], float& ptau,
-                        float* q2v, float* q2t, int lane) {
+// (columns of the previous block, this lane's half), replaced by this task's;
+// q2val: this task's reflector as a Q2 block column.  The wave's scratch sc
+// holds v, the two reduced vectors and a per-lane sink for masked-off stores.
+// Every window access is unconditional (lanes outside the block read a row
+// inside it and discard the value, or store to the sink), so the accesses of
+// a task issue back to back.  The diagonal block is read at the start (no
+// earlier part of the task writes it), and the left application to the block
+// below and the two-sided update of the diagonal block share one pass of
+// reductions.
+__device__ void sb_task(int n, int s, int k, float* w, float* sc, float (&pv)[16], float& ptau, float& q2val,
+                        int lane) {
     constexpr int NB = kBandNB;
+    float* vs = sc;
+    float* wsB = sc + NB;
+    float* wsD = sc + 2 * NB;
+    float* sink = sc + 3 * NB + lane;
     const int i = lane & 31, h = lane >> 5;
     const int r0 = s + 1 + k * NB;
     const int len = min(NB, n - r0);
-    float bk[16];
-    float x;
+    const int p0 = r0 - NB;
+    const bool rowok = i < len;
+    const int rb = sb_rowbase(r0 + min(i, len - 1));  // this lane's row (clamped into the block)
+    float bk[16], dd[16];
+    float x = 0.f;
     if (k == 0) {
-        x = (i < len) ? W.at(r0 + i, s) : 0.f;
+        const float v = w[rb - s];
+        x = rowok ? v : 0.f;
     } else {
-        // the block below the previous one, columns p0 .. p0 + NB - 1: apply
+        // the block below the previous one, columns p0 .. p0 + NB - 1
+        const float* bp = w + (rb - p0 - 16 * h - 15);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const float v = bp[15 - c];
+            bk[c] = rowok ? v : 0.f;
+        }
+    }
+    // the diagonal block D (rows/cols r0 .. r0+len-1), both triangles, from
+    // its lower triangle
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int j = 16 * h + c;
+        const int ra = min(max(i, j), len - 1), cb = min(min(i, j), ra);
+        const float v = w[sb_rowbase(r0 + ra) - (r0 + cb)];
+        dd[c] = (rowok && j < len) ? v : 0.f;
+    }
+    if (k > 0) {
         // the previous reflector from the right (this fills the bulge)
-        const int p0 = r0 - NB;
+        float u0 = 0.f, u1 = 0.f;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) bk[c] = (i < len) ? W.at(r0 + i, p0 + 16 * h + c) : 0.f;
-        float u = 0.f;
-#pragma unroll
-        for (int c = 0; c < 16; ++c) u += bk[c] * pv[c];
-        u += __shfl_xor(u, 32, 64);
-        const float f = ptau * u;
+        for (int c = 0; c < 16; c += 2) {
+            u0 += bk[c] * pv[c];
+            u1 += bk[c + 1] * pv[c + 1];
+        }
+        const float f = ptau * half_pair_sum(u0 + u1);
 #pragma unroll
         for (int c = 0; c < 16; ++c) bk[c] -= f * pv[c];
-        x = __shfl(bk[0], i, 64);  // the block's first column, from the h = 0 lanes
+        x = from_low_half(bk[0]);  // the block's first column, from the h = 0 lanes
     }
     // reflector annihilating x[1..len)
-    const float sq = (i >= 1 && i < len) ? x * x : 0.f;
-    const float sigma = bwave_sum(h == 0 ? sq : 0.f);
-    const float alpha = __shfl(x, 0, 64);
+    const float sq = (h == 0 && i >= 1 && i < len) ? x * x : 0.f;
+    const float sigma = bwave_sum(sq);
+    const float alpha = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0));
     float tau = 0.f, beta = alpha, scale = 0.f;
     if (sigma > 0.f) {
         const float nrm = sqrtf(alpha * alpha + sigma);
@@ -472,11 +626,28 @@ __device__ void sb_task(int n, int s, int k, const SbWin& W, float* vs, float* w
     }
     const float vi = (i == 0) ? 1.f : (i < len ? x * scale : 0.f);
     if (h == 0) vs[i] = vi;
-    if (k == 0) {
-        if (h == 0 && i < len) W.at(r0 + i, s) = (i == 0) ? beta : 0.f;
-    } else {
+    wave_lds_sync();
+    float vj[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 t = reinterpret_cast<const float4*>(vs + 16 * h)[q];
+        vj[4 * q] = t.x;
+        vj[4 * q + 1] = t.y;
+        vj[4 * q + 2] = t.z;
+        vj[4 * q + 3] = t.w;
+    }
+    // D: p = tau D v, K = -tau/2 p.v, w = p + K v
+    float p0a = 0.f, p1a = 0.f;
+#pragma unroll
+    for (int c = 0; c < 16; c += 2) {
+        p0a += dd[c] * vj[c];
+        p1a += dd[c + 1] * vj[c + 1];
+    }
+    const float pp = tau * half_pair_sum(p0a + p1a);
+    if (k > 0) {
         if (h == 0) bk[0] = (i == 0) ? beta : 0.f;
-        // left-apply to the block's other columns: w(c) = v^T B(:, c)
+        // the new reflector from the left on the block's other columns:
+        // w(c) = v^T B(:, c), reduce-scattered over the half's 32 lanes
         float t[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) t[c] = vi * bk[c];
@@ -492,55 +663,51 @@ __device__ void sb_task(int n, int s, int k, const SbWin& W, float* vs, float* w
         float zz = z[0];
 #pragma unroll
         for (int c = 1; c < 8; ++c) zz = (ri == c) ? z[c] : zz;
-        if (ri < 8) ws[16 * h + 8 * ((lane >> 4) & 1) + ri] = zz;
-        wave_lds_sync();
-        const float tv = tau * vi;
-#pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            if (h == 0 && c == 0) continue;  // the annihilated column
-            bk[c] -= tv * ws[16 * h + c];
-        }
-        const int p0 = r0 - NB;
-        if (i < len) {
-#pragma unroll
-            for (int c = 0; c < 16; ++c) W.at(r0 + i, p0 + 16 * h + c) = bk[c];
-        }
-        wave_lds_sync();  // ws is reused below
+        if (ri < 8) wsB[16 * h + 8 * ((lane >> 4) & 1) + ri] = zz;
     }
-    wave_lds_sync();
-    // two-sided update of the diagonal block D (rows/cols r0 .. r0+len-1)
-    float dd[16], vj[16];
-#pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const int j = 16 * h + c;
-        vj[c] = vs[j];
-        float v = 0.f;
-        if (i < len && j < len) v = (i >= j) ? W.at(r0 + i, r0 + j) : W.at(r0 + j, r0 + i);
-        dd[c] = v;
-    }
-    float pp = 0.f;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) pp += dd[c] * vj[c];
-    pp += __shfl_xor(pp, 32, 64);
-    pp *= tau;
     const float kk = bwave_sum(h == 0 ? pp * vi : 0.f);
     const float K = -0.5f * tau * kk;
     const float wi = pp + K * vi;
-    if (h == 0) ws[i] = wi;
+    if (h == 0) wsD[i] = wi;
     wave_lds_sync();
+    if (k > 0) {
+        const float tv = tau * vi;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const int j = 16 * h + c;
-        const float wj = ws[j];
-        dd[c] -= vi * wj + wi * vj[c];
-        if (i < len && j < len && j <= i) W.at(r0 + i, r0 + j) = dd[c];
+        for (int q = 0; q < 4; ++q) {
+            const float4 t = reinterpret_cast<const float4*>(wsB + 16 * h)[q];
+            const float tw[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = 4 * q + e;
+                if (!(h == 0 && c == 0)) bk[c] -= tv * tw[e];  // column 0 is the annihilated one
+            }
+        }
+        if (rowok) {
+            float* bq = w + (rb - p0 - 16 * h - 15);
+#pragma unroll
+            for (int c = 0; c < 16; ++c) bq[15 - c] = bk[c];
+        }
+    } else if (h == 0 && rowok) {
+        w[rb - s] = (i == 0) ? beta : 0.f;
     }
-    // the reflector: Q2 block column (48 rows, zero outside [j0, j0 + len))
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 t = reinterpret_cast<const float4*>(wsD + 16 * h)[q];
+        const float tw[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = 4 * q + e, j = 16 * h + c;
+            dd[c] -= vi * tw[e] + wi * vj[c];
+            float* dst = (rowok && j < len && j <= i) ? w + (rb - r0 - j) : sink;
+            *dst = dd[c];
+        }
+    }
+    // the reflector as its Q2 block column (48 rows, zero outside [jq, jq +
+    // len)): lane r < 48 holds row r
     const int jq = s % kQ2G;
     const int rr = lane - jq;
     const float vv = __shfl(vi, rr & 31, 64);  // every lane takes part in the shuffle
-    if (lane < kQ2R) q2v[jq * kQ2R + lane] = (rr >= 0 && rr < NB) ? vv : 0.f;
-    if (lane == 0) q2t[jq] = tau;
+    q2val = (rr >= 0 && rr < NB) ? vv : 0.f;
 #pragma unroll
     for (int c = 0; c < 16; ++c) pv[c] = vj[c];
     ptau = tau;
@@ -565,9 +732,8 @@ __global__ void __launch_bounds__(kSbNT) sb2st_kernel(int n, float* __restrict__
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float* BND = BNDall + size_t(b) * n * kBandW;
     float* Q2 = Q2all + size_t(b) * q2_block_offset(n, q2_groups(n)) * kQ2BlockFloats;
-    SbWin W{sb_smem};
-    float* vs = sb_smem + size_t(kSbWR) * kSbWP + wave * 2 * NB;
-    float* ws = vs + NB;
+    float* const win = sb_smem;
+    float* sc = sb_smem + size_t(kSbWR) * kSbWP + wave * kSbScratch;
     const int nsweep = n - 2;
 
     // rows [r_lo, r_hi] of the band to or from the window, all threads,
@@ -576,22 +742,21 @@ __global__ void __launch_bounds__(kSbNT) sb2st_kernel(int n, float* __restrict__
         const int cnt = (r_hi - r_lo + 1) * (kBandW / 4);
         for (int q = tid; q < cnt; q += kSbNT) {
             const int r = r_lo + q / (kBandW / 4), c4 = (q % (kBandW / 4)) * 4;
-            const float4 v = *reinterpret_cast<const float4*>(BND + size_t(r) * kBandW + c4);
-            float* d = W.w + (r & (kSbWR - 1)) * kSbWP + c4;
-            d[0] = v.x;
-            d[1] = v.y;
-            d[2] = v.z;
-            d[3] = v.w;
+            *reinterpret_cast<float4*>(win + (r & (kSbWR - 1)) * kSbWP + c4) =
+                *reinterpret_cast<const float4*>(BND + size_t(r) * kBandW + c4);
         }
     };
     auto store_rows = [&](int r_lo, int r_hi) {
         const int cnt = (r_hi - r_lo + 1) * (kBandW / 4);
         for (int q = tid; q < cnt; q += kSbNT) {
             const int r = r_lo + q / (kBandW / 4), c4 = (q % (kBandW / 4)) * 4;
-            const float* d = W.w + (r & (kSbWR - 1)) * kSbWP + c4;
-            *reinterpret_cast<float4*>(BND + size_t(r) * kBandW + c4) = float4{d[0], d[1], d[2], d[3]};
+            *reinterpret_cast<float4*>(BND + size_t(r) * kBandW + c4) =
+                *reinterpret_cast<const float4*>(win + (r & (kSbWR - 1)) * kSbWP + c4);
         }
     };
+    // this thread's 16-byte piece of the rows read ahead (row pr, columns pc ..)
+    const int pr_off = tid / (kBandW / 4), pc = (tid % (kBandW / 4)) * 4;
+    float* sink = sb_smem + kSbSmemFloats - 4;
 
     for (int s0 = 0; s0 < nsweep; s0 += kSbG) {
         const int ng = min(kSbG, nsweep - s0);
@@ -605,48 +770,54 @@ __global__ void __launch_bounds__(kSbNT) sb2st_kernel(int n, float* __restrict__
             for (int j = 0; j < ng; ++j) top = min(top, sb_task_row(n, s0 + j, t - 2 * j));
             return top;
         };
-        int lo = s0, hi = min(n - 1, s0 + NB);
+        auto hi_of = [&](int t) { return min(n - 1, s0 + (t + 1) * NB); };
+        int lo = s0, hi = hi_of(0);
         __syncthreads();
         load_rows(lo, hi);
+        // read-ahead of the rows step t + 1 adds, (hi(t), hi(t + 1)]: at most NB
+        // rows, one 16-byte piece per thread, in flight during step t's tasks
+        // (read and written by every thread, so that the wait for it is on
+        // every path: a thread with no row reads the last row and writes the
+        // sink)
+        int pr = hi + 1 + pr_off;
+        bool pre = nstep > 1 && pr <= hi_of(1);
+        float4 pf = *reinterpret_cast<const float4*>(BND + size_t(min(pr, n - 1)) * kBandW + pc);
         float pv[16], ptau = 0.f;
 #pragma unroll
         for (int c = 0; c < 16; ++c) pv[c] = 0.f;
         for (int t = 0; t < nstep; ++t) {
-            // the next step's rows (at most NB: one 16-byte piece per thread)
-            // are read while this step's tasks run; nobody touches them yet
-            const int nhi = (t + 1 < nstep) ? min(n - 1, s0 + (t + 2) * NB) : hi;
-            float4 pf = {0.f, 0.f, 0.f, 0.f};
-            const int pr = hi + 1 + tid / (kBandW / 4), pc = (tid % (kBandW / 4)) * 4;
-            const bool pre = pr <= nhi;
-            if (pre) pf = *reinterpret_cast<const float4*>(BND + size_t(pr) * kBandW + pc);
             __syncthreads();
             const int s = s0 + wave, k = t - 2 * wave;
-            if (wave < ng && k >= 0 && k < sb_tasks(n, s)) {
-                float* blk = q2g + size_t(k) * kQ2BlockFloats;
-                sb_task(n, s, k, W, vs, ws, pv, ptau, blk, blk + kQ2R * kQ2G, lane);
-            }
+            const bool act = wave < ng && k >= 0 && k < sb_tasks(n, s);
+            float q2val = 0.f;
+            if (act) sb_task(n, s, k, win, sc, pv, ptau, q2val, lane);
             __syncthreads();
-            // slide: write back the rows no longer needed; the read-ahead rows
-            // go to slots whose rows left the window at least a step ago
-            // (span of the window + NB < kSbWR)
+            // the read-ahead rows go to slots whose rows left the window at
+            // least a step ago (span of the window + NB < kSbWR); they are
+            // written before this step's stores are issued, so waiting for
+            // the read waits only for memory operations issued before the
+            // tasks
+            *reinterpret_cast<float4*>(pre ? win + (pr & (kSbWR - 1)) * kSbWP + pc : sink) = pf;
+            // write back the rows no longer needed
             const int top = (t + 1 < nstep) ? top_of(t + 1) : 0x7fffffff;
-            const int wlo = lo, whi = min(hi, top - 1);
-            if (whi >= wlo) {
-                store_rows(wlo, whi);
+            const int whi = min(hi, top - 1);
+            if (whi >= lo) {
+                store_rows(lo, whi);
                 lo = whi + 1;
             }
-            if (pre) {
-                float* d = W.w + (pr & (kSbWR - 1)) * kSbWP + pc;
-                d[0] = pf.x;
-                d[1] = pf.y;
-                d[2] = pf.z;
-                d[3] = pf.w;
+            if (t + 1 < nstep) hi = hi_of(t + 1);
+            if (act) {
+                float* blk = q2g + size_t(k) * kQ2BlockFloats;
+                const int jq = s % kQ2G;
+                if (lane < kQ2R) blk[jq * kQ2R + lane] = q2val;
+                if (lane == 0) blk[kQ2R * kQ2G + jq] = ptau;
             }
-            hi = nhi;
+            pr = hi + 1 + pr_off;
+            pre = t + 2 < nstep && pr <= hi_of(t + 2);
+            pf = *reinterpret_cast<const float4*>(BND + size_t(min(pr, n - 1)) * kBandW + pc);
         }
         __syncthreads();
         if (hi >= lo) store_rows(lo, hi);
-        __syncthreads();
     }
     __syncthreads();
     // d and e of T from the band
@@ -693,10 +864,13 @@ __device__ __forceinline__ void q2_block_gk(int n, int64_t q, int* g, int* k) {
 }
 
 // Per block (one wave): reflectors that do not exist (sweeps past n - 3, or
-// past their last task) are zeroed; T is formed (forward, columnwise) and
-// -V T stored beside V, so that applying the block is Z <- Z + (-V T) (V^T Z).
+// past their last task) are zeroed; T is formed (forward, columnwise), and V
+// and -V T are stored in the operand order of the 16x16x4 MFMA products that
+// apply the block (Z <- Z + (-V T) (V^T Z)): lane l's 12 values of each
+// contiguous, V_op[l][s] = V(4 s + l/16, l%16) and NVT_op[l][4 rt + s] =
+// (-V T)(16 rt + l%16, 4 s + l/16).  V_op replaces V in place.
 __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float* __restrict__ Q2all) {
-    __shared__ float sv[kQ2R][kQ2G + 1], st[kQ2G][kQ2G + 1], sg[kQ2G], stau[kQ2G];
+    __shared__ float sv[kQ2R][kQ2G + 1], st[kQ2G][kQ2G + 1], snv[kQ2R][kQ2G + 1], sg[kQ2G], stau[kQ2G];
     const int64_t q = blockIdx.x;
     const int b = blockIdx.y, lane = threadIdx.x;
     if (q >= nblk) return;
@@ -708,18 +882,10 @@ __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float*
     q2_block_gk(n, q, &g, &k);
     for (int e = lane; e < kQ2R * kQ2G; e += 64) {
         const int j = e / kQ2R, r = e % kQ2R;
-        const int s = g * kQ2G + j;
-        const bool ex = k < sb_tasks(n, s);
-        const float v = ex ? V[e] : 0.f;
-        sv[r][j] = v;
-        if (!ex) V[e] = 0.f;
+        const bool ex = k < sb_tasks(n, g * kQ2G + j);
+        sv[r][j] = ex ? V[e] : 0.f;
     }
-    if (lane < kQ2G) {
-        const int s = g * kQ2G + lane;
-        const bool ex = k < sb_tasks(n, s);
-        stau[lane] = ex ? tau[lane] : 0.f;
-        if (!ex) tau[lane] = 0.f;
-    }
+    if (lane < kQ2G) stau[lane] = (k < sb_tasks(n, g * kQ2G + lane)) ? tau[lane] : 0.f;
     __syncthreads();
     for (int j = 0; j < kQ2G; ++j) {
         if (lane < j) {
@@ -743,77 +909,160 @@ __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float*
         const int j = e / kQ2R, r = e % kQ2R;
         float acc = 0.f;
         for (int q2 = 0; q2 <= j; ++q2) acc += sv[r][q2] * st[q2][j];
-        NVT[e] = -acc;
+        snv[r][j] = -acc;
     }
+    __syncthreads();
+    const int l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < kQ2R / 4; ++s) V[lane * (kQ2R / 4) + s] = sv[4 * s + lq][l16];
+#pragma unroll
+    for (int rt = 0; rt < 3; ++rt)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) NVT[lane * 12 + 4 * rt + s] = snv[16 * rt + l16][4 * s + lq];
+    if (lane < kQ2G) tau[lane] = stau[lane];
 }
 
 // Z <- Q2 Z on one 16-column slab of one matrix, the slab in LDS (rows past n
 // zero).  Blocks run group by group from the last group, positions in
 // ascending order; group g takes position k at step k + 2 (G - 1 - g), so the
 // blocks of one step touch disjoint rows and every block runs after those it
-// must follow.  Group g belongs to wave (G - 1 - g) mod 8.  A block on a
-// wave: W = V^T Zb (16 x 16, K = 48) and Zb += (-V T) W, as 16x16x4 FP32 MFMA.
+// must follow.  Group g belongs to wave (G - 1 - g) mod (waves per
+// workgroup), which reads its next block's operands while it applies the
+// current one.  A block: W = V^T Zb
+// (16 x 16, K = 48) and Zb += (-V T) W, as 16x16x4 FP32 MFMA, W passed from
+// one product to the next in registers.
+constexpr int kQ2MaxGroups = 160;
+static_assert(kQ2MaxGroups * kQ2G >= 4 * 512 + kBandNB, "group table too small for sb_max_n()");
 constexpr int kQ2NT = 512;
 __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const float* __restrict__ Q2all,
                                                           float* __restrict__ Zall, int64_t nblk) {
     extern __shared__ __align__(16) float q2_smem[];
+    __shared__ int goff[kQ2MaxGroups], snstep;
     const int slab = blockIdx.x, b = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c0 = slab * kQ2S;
     const int rows = n + kQ2R;  // slab rows incl. zero padding
     float* Zs = q2_smem;
-    float* Ws = q2_smem + size_t(rows) * kQ2S + wave * kQ2G * kQ2S;
     float* Z = Zall + size_t(b) * n * ld;
     const float* Q2 = Q2all + size_t(b) * nblk * kQ2BlockFloats;
+    const int NG = q2_groups(n);
+    if (tid == 0) {
+        int o = 0, ns = 0;
+        for (int g = 0; g < NG; ++g) {
+            const int t = sb_tasks(n, g * kQ2G);
+            goff[g] = o;
+            o += t;
+            ns = max(ns, 2 * (NG - 1 - g) + t);
+        }
+        snstep = ns;
+    }
     for (int e = tid; e < rows * kQ2S; e += kQ2NT) {
         const int r = e / kQ2S, c = e % kQ2S;
         Zs[e] = (r < n && c0 + c < n) ? Z[size_t(r) * ld + c0 + c] : 0.f;
     }
-    const int NG = q2_groups(n);
-    int nstep = 0;
-    for (int g = 0; g < NG; ++g) nstep = max(nstep, 2 * (NG - 1 - g) + sb_tasks(n, g * kQ2G));
+    __syncthreads();
+    const int nstep = snstep;
     const int l16 = lane & 15, lq = lane >> 4;
-    for (int t = 0; t < nstep; ++t) {
-        __syncthreads();
-        for (int gi = wave; gi < NG; gi += kQ2NT / 64) {
-            const int g = NG - 1 - gi;
-            const int k = t - 2 * gi;
-            if (k < 0 || k >= sb_tasks(n, g * kQ2G)) continue;
-            const float* blk = Q2 + (q2_block_offset(n, g) + k) * kQ2BlockFloats;
-            const float* V = blk;
-            const float* NVT = blk + kQ2R * kQ2G + kQ2G;
-            const int rb = g * kQ2G + 1 + k * kBandNB;
-            float va[kQ2R / 4], na[3][4];
+    constexpr int NW = kQ2NT / 64;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    // Group index gi = NG - 1 - g is active at step t when 2 gi <= t and
+    // t - 2 gi < its task count; the task count grows with gi, so the active
+    // groups of a step are an interval [lo, hi] whose ends never decrease.
+    auto tasks_of = [&](int gi) { return sb_tasks(n, (NG - 1 - gi) * kQ2G); };
+    int lo = 0;
+    // this wave's first active group at step t (t never decreasing), or -1
+    auto first_of = [&](int t) {
+        const int hi = min(t >> 1, NG - 1);
+        while (lo <= hi && t - 2 * lo >= tasks_of(lo)) ++lo;
+        const int g = lo + ((wv - lo) & (NW - 1));
+        return g <= hi ? g : -1;
+    };
+    auto load_ops = [&](int t, int gi, float4 (&va)[3], float4 (&na)[3]) {
+        const int g = NG - 1 - gi;
+        const float* blk = Q2 + size_t(goff[g] + t - 2 * gi) * kQ2BlockFloats;
+        const float4* V4 = reinterpret_cast<const float4*>(blk + lane * (kQ2R / 4));
+        const float4* N4 = reinterpret_cast<const float4*>(blk + kQ2R * kQ2G + kQ2G + lane * 12);
 #pragma unroll
-            for (int s = 0; s < kQ2R / 4; ++s) va[s] = V[l16 * kQ2R + 4 * s + lq];
+        for (int i = 0; i < 3; ++i) {
+            va[i] = V4[i];
+            na[i] = N4[i];
+        }
+    };
+    // the wave's blocks in order (step by step, groups in order within a
+    // step); the next block's operands are in flight while a block runs, also
+    // across the step barrier
+    auto advance = [&](int& tt, int& g) {  // the block after (tt, g), or tt = nstep
+        if (g >= 0 && g + NW <= min(tt >> 1, NG - 1)) {
+            g += NW;
+            return;
+        }
+        g = -1;
+        while (g < 0 && ++tt < nstep) g = first_of(tt);
+    };
+    int t = 0, gi = first_of(0);
+    if (gi < 0) advance(t, gi);
+    float4 va[3], na[3];
+    if (t < nstep) load_ops(t, gi, va, na);
+    for (int step = 0; step < nstep; ++step) {
+        __syncthreads();
+        while (t == step) {
+            int t2 = t, g2 = gi;
+            advance(t2, g2);
+            float4 vn[3], nn[3];
+            if (t2 < nstep) load_ops(t2, g2, vn, nn);
+            const int g = NG - 1 - gi;
+            const int rb = g * kQ2G + 1 + (t - 2 * gi) * kBandNB;
+            const float vs[12] = {va[0].x, va[0].y, va[0].z, va[0].w, va[1].x, va[1].y,
+                                  va[1].z, va[1].w, va[2].x, va[2].y, va[2].z, va[2].w};
+            const float ns[12] = {na[0].x, na[0].y, na[0].z, na[0].w, na[1].x, na[1].y,
+                                  na[1].z, na[1].w, na[2].x, na[2].y, na[2].z, na[2].w};
+            floatx4 w0 = {0.f, 0.f, 0.f, 0.f}, w1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < kQ2R / 4; s += 2) {
+                const float z0 = Zs[(rb + 4 * s + lq) * kQ2S + l16];
+                const float z1 = Zs[(rb + 4 * s + 4 + lq) * kQ2S + l16];
+                w0 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[s], z0, w0, 0, 0, 0);
+                w1 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[s + 1], z1, w1, 0, 0, 0);
+            }
+            // W as the next product's B operand: register s of lane (lq, l16)
+            // must hold W(4 s + lq, l16), the accumulator holds W(4 lq + r,
+            // l16) in register r, so the 4 x 4 (lq, r) blocks of each column
+            // are transposed across lanes: the off-diagonal 2 x 2 blocks by
+            // v_permlane32_swap, then each 2 x 2 block by v_permlane16_swap
+            float wb[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) wb[r] = w0[r] + w1[r];
+            {
+                auto p0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(wb[0]), __float_as_uint(wb[2]), false, false);
+                auto p1 = __builtin_amdgcn_permlane32_swap(__float_as_uint(wb[1]), __float_as_uint(wb[3]), false, false);
+                auto q0 = __builtin_amdgcn_permlane16_swap(p0[0], p1[0], false, false);
+                auto q1 = __builtin_amdgcn_permlane16_swap(p0[1], p1[1], false, false);
+                wb[0] = __uint_as_float(q0[0]);
+                wb[1] = __uint_as_float(q0[1]);
+                wb[2] = __uint_as_float(q1[0]);
+                wb[3] = __uint_as_float(q1[1]);
+            }
+            floatx4 acc[3];
 #pragma unroll
             for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
-                for (int s = 0; s < 4; ++s) na[rt][s] = NVT[(4 * s + lq) * kQ2R + rt * 16 + l16];
-            floatx4 w = {0.f, 0.f, 0.f, 0.f};
+                for (int r = 0; r < 4; ++r) acc[rt][r] = Zs[(rb + rt * 16 + 4 * lq + r) * kQ2S + l16];
 #pragma unroll
-            for (int s = 0; s < kQ2R / 4; ++s) {
-                const float zb = Zs[(rb + 4 * s + lq) * kQ2S + l16];
-                w = __builtin_amdgcn_mfma_f32_16x16x4f32(va[s], zb, w, 0, 0, 0);
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int rt = 0; rt < 3; ++rt)
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ns[4 * rt + s], wb[s], acc[rt], 0, 0, 0);
+#pragma unroll
+            for (int rt = 0; rt < 3; ++rt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Zs[(rb + rt * 16 + 4 * lq + r) * kQ2S + l16] = acc[rt][r];
+            t = t2;
+            gi = g2;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                va[i] = vn[i];
+                na[i] = nn[i];
             }
-            // W (reflector q = 4 lq + r, column l16) to the wave's scratch
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Ws[(4 * lq + r) * kQ2S + l16] = w[r];
-            wave_lds_sync();
-            float wb[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) wb[s] = Ws[(4 * s + lq) * kQ2S + l16];
-#pragma unroll
-            for (int rt = 0; rt < 3; ++rt) {
-                floatx4 acc;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[r] = Zs[(rb + rt * 16 + 4 * lq + r) * kQ2S + l16];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(na[rt][s], wb[s], acc, 0, 0, 0);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Zs[(rb + rt * 16 + 4 * lq + r) * kQ2S + l16] = acc[r];
-            }
-            wave_lds_sync();
         }
     }
     __syncthreads();
@@ -823,15 +1072,19 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
     }
 }
 
+int q2_prep(int batch, int n, float* Q2, hipStream_t s) {
+    const int64_t nblk = q2_blocks(n);
+    if (nblk <= 0) return CIR_OK;
+    dim3 grid(static_cast<unsigned>(nblk), static_cast<unsigned>(batch));
+    hipLaunchKernelGGL(q2_prep_kernel, grid, dim3(64), 0, s, n, nblk, Q2);
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
 int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s) {
     const int64_t nblk = q2_blocks(n);
     if (nblk <= 0) return CIR_OK;
-    {
-        dim3 grid(static_cast<unsigned>(nblk), static_cast<unsigned>(batch));
-        hipLaunchKernelGGL(q2_prep_kernel, grid, dim3(64), 0, s, n, nblk, const_cast<float*>(Q2));
-        CIR_CHECK_LAUNCH();
-    }
-    const size_t sm = (size_t(n + kQ2R) * kQ2S + size_t(kQ2NT / 64) * kQ2G * kQ2S) * sizeof(float);
+    const size_t sm = size_t(n + kQ2R) * kQ2S * sizeof(float);
     if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "q2_apply: n too large for the LDS slab");
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));

@@ -1547,8 +1547,9 @@ inline size_t larft_smem_bytes(size_t esz) { return (size_t(kBtNB) * (kBtNB + 1)
 // (k_band.hip); the few-matrix whole-GPU path and FP64 reduce in one stage.
 // CIR_EIGH_STAGES=1 or 2 in the environment forces one form where both apply
 // (tests compare them); otherwise the two-stage form is taken from
-// kTwoStageMinN on.
-constexpr int kTwoStageMinN = 1 << 30;
+// kTwoStageMinN on (at 256 matrices per batch the two forms take the same
+// time near n = 1024; at n = 2048 the two-stage form takes 12% less).
+constexpr int kTwoStageMinN = 1024;
 inline bool use_two_stage(size_t esz, int batch, int n) {
     if (esz != sizeof(float) || n > sb_max_n() || n < 2 * kBandNB + 2) return false;
     if (n >= kSingleMinN && batch <= kSingleMaxBatch) return false;
@@ -1633,12 +1634,17 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     if (ld % 4 != 0) return fail(CIR_ERR_ARG, "eigh: the leading dimension must be a multiple of 4");
     if constexpr (sizeof(T) == sizeof(float)) {
         if (L.two) {
-            ProfScope ps(prof, CIR_PH_TRIDIAG, s);
             float* bnd = reinterpret_cast<float*>(work + L.off_bnd);
-            CIR_TRY(sy2sb(batch, n, ld, A, bnd, tau, reinterpret_cast<float*>(work + L.off_p1),
-                          reinterpret_cast<float*>(work + L.off_p2), reinterpret_cast<float*>(work + L.off_yh),
-                          reinterpret_cast<float*>(work + L.off_mm), n, s));
-            CIR_TRY(sb2st(batch, n, bnd, dd, ee, reinterpret_cast<float*>(work + L.off_q2), s));
+            float* q2 = reinterpret_cast<float*>(work + L.off_q2);
+            {
+                ProfScope ps(prof, CIR_PH_TRIDIAG, s);
+                CIR_TRY(sy2sb(batch, n, ld, A, bnd, tau, reinterpret_cast<float*>(work + L.off_p1),
+                              reinterpret_cast<float*>(work + L.off_yh), reinterpret_cast<float*>(work + L.off_mm),
+                              s));
+            }
+            ProfScope ps(prof, CIR_PH_BULGE, s);
+            CIR_TRY(sb2st(batch, n, bnd, dd, ee, q2, s));
+            CIR_TRY(q2_prep(batch, n, q2, s));
         }
     }
     if (L.two) {
@@ -1684,10 +1690,13 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     // stages), then the Householder panels, last block of reflectors first
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&larft_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(larft_smem_bytes(sizeof(T)))));
-    ProfScope pbt(prof, CIR_PH_BACKTRANSFORM, s);
     if constexpr (sizeof(T) == sizeof(float)) {
-        if (L.two) CIR_TRY(q2_apply(batch, n, ld, reinterpret_cast<const float*>(work + L.off_q2), V, s));
+        if (L.two) {
+            ProfScope ps(prof, CIR_PH_BULGE_BT, s);
+            CIR_TRY(q2_apply(batch, n, ld, reinterpret_cast<const float*>(work + L.off_q2), V, s));
+        }
     }
+    ProfScope pbt(prof, CIR_PH_BACKTRANSFORM, s);
     // reflector of column j: unit at row j + off (one stage: the
     // subdiagonal; two stages: the band edge), columns with reflectors
     // 0 .. nref-1

@@ -211,11 +211,11 @@ int check_boxes(const cir_ctx* c, int nframes, const float* box) {
 int auto_chunk(const cir_ctx* c, int nuniq) {
     if (c->p.chunk > 0) return std::min(c->p.chunk, nuniq);
     // H and V of every frame of a chunk plus the eigensolver's per-matrix
-    // panel (64 x ld) and back-transform blocks (two 256 x N products, two
-    // 256 x 256 factors), within half of the free device memory (large boxes
-    // need large batches: one workgroup per matrix fills the GPU only with
-    // hundreds of matrices)
-    const double per = 2.0 * double(c->N) * c->ld * c->esz + (64.0 * c->ld + 512.0 * c->N + 131072.0) * c->esz;
+    // work space (its share of a 64-matrix batch), within half of the free
+    // device memory (large boxes need large batches: one workgroup per matrix
+    // fills the GPU only with hundreds of matrices)
+    const double per = 2.0 * double(c->N) * c->ld * c->esz +
+                       double(eigh_workspace(c->real_prec, 64, c->N, c->ld)) / 64.0;
     size_t fr = 0, tot = 0;
     double budget = 24.0 * (1 << 30);
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::max(budget, 0.5 * double(fr));

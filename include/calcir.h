@@ -182,16 +182,20 @@ typedef struct cir_ctx cir_ctx;
 enum {
     CIR_PH_SITES = 0,         /* field projection + site records            */
     CIR_PH_HAMILTONIAN = 1,   /* exciton Hamiltonian                        */
-    CIR_PH_TRIDIAG = 2,       /* Householder tridiagonalisation             */
+    CIR_PH_TRIDIAG = 2,       /* Householder reduction (to tridiagonal form,
+                                 or to band form where the bulge-chasing
+                                 stage follows)                              */
     CIR_PH_BISECT = 3,        /* tridiagonal eigenvalues                    */
     CIR_PH_INVIT = 4,         /* tridiagonal eigenvectors                   */
-    CIR_PH_BACKTRANSFORM = 5, /* eigenvectors V = Q Z                       */
+    CIR_PH_BACKTRANSFORM = 5, /* eigenvectors V = Q Z (Householder panels)  */
     CIR_PH_IPR = 6,           /* IPR / frequency distribution               */
     CIR_PH_SW = 7,            /* spectral density                           */
     CIR_PH_PROPAGATE = 8,     /* propagation products + phase               */
     CIR_PH_CORRELATE = 9,     /* origin bookkeeping + correlation functions */
     CIR_PH_FINALIZE = 10,     /* C2R lineshapes                             */
-    CIR_NPHASE = 11
+    CIR_PH_BULGE = 11,        /* band -> tridiagonal by bulge chasing       */
+    CIR_PH_BULGE_BT = 12,     /* bulge-chasing reflectors applied to Z      */
+    CIR_NPHASE = 13
 };
 
 /* --- host-side boundary (no GPU needed) --------------------------------- */
