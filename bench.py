@@ -72,6 +72,42 @@ def traffic_per_launch(matrices_per_launch):
         return None
 
 
+def traffic_from(fetch_db, write_db, frames, kernel="q2_apply_kernel"):
+    """The committed traffic file from two rocprofv3 counter runs of
+    `bench.py --frames F --steps 1 --warmup 0 --no-extras --no-cpu
+    --copy-calibration` (FETCH_SIZE in one, WRITE_SIZE in the other; their
+    rocpd databases): each counter's reading for the kernel's dispatch,
+    scaled by what the same counter reads for the 1 GiB calibration copy, per
+    matrix of the dispatch (all distinct frames of the run, one chunk)."""
+    import sqlite3
+
+    def per_dispatch(db, counter):
+        con = sqlite3.connect("file:%s?mode=ro" % db, uri=True)
+        rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                           "where counter_name = ? group by dispatch_id, kernel_name", (counter,)).fetchall()
+        con.close()
+        k = [v for _, name, v in rows if kernel in name]
+        cp = [v for _, name, v in rows if "copyBuffer" in name]
+        if len(k) != 1 or not cp:
+            raise SystemExit("%s: expected one %s dispatch and the calibration copy, found %d and %d"
+                             % (db, kernel, len(k), len(cp)))
+        return float(k[0]), float(max(cp))
+
+    nmat = 150 + (nsamples_for(frames) - 1) * 20
+    fk, fc = per_dispatch(fetch_db, "FETCH_SIZE")
+    wk, wc = per_dispatch(write_db, "WRITE_SIZE")
+    gib_kib = 1024.0 * 1024.0
+    fs, ws = gib_kib / fc, gib_kib / wc
+    hbm = (fk * fs + wk * ws) * 1024.0 / nmat
+    n = 2 * NWATER
+    alg = 2.0 * 4.0 * n * n  # Z read and written once
+    out = {"kernel": kernel, "workload": "2048-OH box, N=2048, ld=2048, %d frames" % frames,
+           "matrices_per_dispatch": nmat, "fetch_kib_raw": fk, "write_kib_raw": wk,
+           "fetch_scale": fs, "write_scale": ws, "copy_fetch_kib_per_gib": fc, "copy_write_kib_per_gib": wc,
+           "hbm_bytes_per_matrix": hbm, "eigenvector_bytes_per_matrix": alg, "traffic_over_eigenvectors": hbm / alg}
+    return out
+
+
 def lineshape(p, ft):
     """(freq, value) rows as written to <outf>_irls.dat (calcIR.cu:893-919)."""
     nt, nz = p.ntcfpoints, p.nzeros
@@ -290,7 +326,16 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="write the traffic file of the dominant kernel from two counter runs (see traffic_from) "
+                         "and exit; needs no GPU")
     args = ap.parse_args()
+    if args.traffic_from:
+        t = traffic_from(args.traffic_from[0], args.traffic_from[1], args.frames)
+        with open(TRAFFIC_FILE, "w") as f:
+            json.dump(t, f, indent=1)
+        print(json.dumps(t, indent=1))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2

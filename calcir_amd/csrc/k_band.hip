@@ -913,8 +913,10 @@ __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float*
     }
     __syncthreads();
     const int l16 = lane & 15, lq = lane >> 4;
+    // V^T as the A operand of the W product: register 4 q + e of lane (lq,
+    // l16) holds V(16 q + 4 lq + e, l16)
 #pragma unroll
-    for (int s = 0; s < kQ2R / 4; ++s) V[lane * (kQ2R / 4) + s] = sv[4 * s + lq][l16];
+    for (int s = 0; s < kQ2R / 4; ++s) V[lane * (kQ2R / 4) + s] = sv[16 * (s >> 2) + 4 * lq + (s & 3)][l16];
 #pragma unroll
     for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
@@ -923,17 +925,27 @@ __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float*
 }
 
 // Z <- Q2 Z on one 16-column slab of one matrix, the slab in LDS (rows past n
-// zero).  Blocks run group by group from the last group, positions in
-// ascending order; group g takes position k at step k + 2 (G - 1 - g), so the
-// blocks of one step touch disjoint rows and every block runs after those it
-// must follow.  Group g belongs to wave (G - 1 - g) mod (waves per
-// workgroup), which reads its next block's operands while it applies the
-// current one.  A block: W = V^T Zb
+// zero).  Blocks apply group by group from the last group, positions in
+// ascending order.  Block (g, k) spans rows 16 g + 1 + 32 k .. 16 g + 47 +
+// 32 k, so it must follow (g, k - 1) and, of the group after it, (g + 1, k - 1)
+// and (g + 1, k); group g takes position k at step k + (G - 1 - g), which
+// orders all of these, and the blocks of one step, (g, k), (g + 1, k + 1), ...,
+// start 48 rows apart and touch disjoint rows.  Group g belongs to wave
+// (G - 1 - g) mod (waves per workgroup), which reads its next block's
+// operands while it applies the current one.  A block: W = V^T Zb
 // (16 x 16, K = 48) and Zb += (-V T) W, as 16x16x4 FP32 MFMA, W passed from
 // one product to the next in registers.
 constexpr int kQ2MaxGroups = 160;
 static_assert(kQ2MaxGroups * kQ2G >= 4 * 512 + kBandNB, "group table too small for sb_max_n()");
-constexpr int kQ2NT = 512;
+constexpr int kQ2NT = 1024;
+// The slab is stored column by column, row r of Z at index r + 3 of its
+// column (block rows then start at multiples of 4, so a lane's 4 rows are one
+// 16-byte access), with a column pitch of 4 mod 64 floats (the 16 columns of
+// one access fall in distinct banks) covering rows n .. n + 47 as zeros.
+__host__ __device__ inline int q2_pitch(int n) {
+    const int p = n + kQ2R + 3;
+    return p + ((4 - p) & 63);
+}
 __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const float* __restrict__ Q2all,
                                                           float* __restrict__ Zall, int64_t nblk) {
     extern __shared__ __align__(16) float q2_smem[];
@@ -941,7 +953,7 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
     const int slab = blockIdx.x, b = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c0 = slab * kQ2S;
-    const int rows = n + kQ2R;  // slab rows incl. zero padding
+    const int P = q2_pitch(n);
     float* Zs = q2_smem;
     float* Z = Zall + size_t(b) * n * ld;
     const float* Q2 = Q2all + size_t(b) * nblk * kQ2BlockFloats;
@@ -952,34 +964,34 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
             const int t = sb_tasks(n, g * kQ2G);
             goff[g] = o;
             o += t;
-            ns = max(ns, 2 * (NG - 1 - g) + t);
+            ns = max(ns, (NG - 1 - g) + t);
         }
         snstep = ns;
     }
-    for (int e = tid; e < rows * kQ2S; e += kQ2NT) {
-        const int r = e / kQ2S, c = e % kQ2S;
-        Zs[e] = (r < n && c0 + c < n) ? Z[size_t(r) * ld + c0 + c] : 0.f;
+    for (int e = tid; e < P * kQ2S; e += kQ2NT) {
+        const int r = e / kQ2S - 3, c = e % kQ2S;
+        Zs[c * P + r + 3] = (r >= 0 && r < n && c0 + c < n) ? Z[size_t(r) * ld + c0 + c] : 0.f;
     }
     __syncthreads();
     const int nstep = snstep;
     const int l16 = lane & 15, lq = lane >> 4;
     constexpr int NW = kQ2NT / 64;
     const int wv = __builtin_amdgcn_readfirstlane(wave);
-    // Group index gi = NG - 1 - g is active at step t when 2 gi <= t and
-    // t - 2 gi < its task count; the task count grows with gi, so the active
+    // Group index gi = NG - 1 - g is active at step t when gi <= t and
+    // t - gi < its task count; the task count grows with gi, so the active
     // groups of a step are an interval [lo, hi] whose ends never decrease.
     auto tasks_of = [&](int gi) { return sb_tasks(n, (NG - 1 - gi) * kQ2G); };
     int lo = 0;
     // this wave's first active group at step t (t never decreasing), or -1
     auto first_of = [&](int t) {
-        const int hi = min(t >> 1, NG - 1);
-        while (lo <= hi && t - 2 * lo >= tasks_of(lo)) ++lo;
+        const int hi = min(t, NG - 1);
+        while (lo <= hi && t - lo >= tasks_of(lo)) ++lo;
         const int g = lo + ((wv - lo) & (NW - 1));
         return g <= hi ? g : -1;
     };
     auto load_ops = [&](int t, int gi, float4 (&va)[3], float4 (&na)[3]) {
         const int g = NG - 1 - gi;
-        const float* blk = Q2 + size_t(goff[g] + t - 2 * gi) * kQ2BlockFloats;
+        const float* blk = Q2 + size_t(goff[g] + t - gi) * kQ2BlockFloats;
         const float4* V4 = reinterpret_cast<const float4*>(blk + lane * (kQ2R / 4));
         const float4* N4 = reinterpret_cast<const float4*>(blk + kQ2R * kQ2G + kQ2G + lane * 12);
 #pragma unroll
@@ -992,7 +1004,7 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
     // step); the next block's operands are in flight while a block runs, also
     // across the step barrier
     auto advance = [&](int& tt, int& g) {  // the block after (tt, g), or tt = nstep
-        if (g >= 0 && g + NW <= min(tt >> 1, NG - 1)) {
+        if (g >= 0 && g + NW <= min(tt, NG - 1)) {
             g += NW;
             return;
         }
@@ -1011,18 +1023,23 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
             float4 vn[3], nn[3];
             if (t2 < nstep) load_ops(t2, g2, vn, nn);
             const int g = NG - 1 - gi;
-            const int rb = g * kQ2G + 1 + (t - 2 * gi) * kBandNB;
+            const int rb = g * kQ2G + 1 + (t - gi) * kBandNB;
             const float vs[12] = {va[0].x, va[0].y, va[0].z, va[0].w, va[1].x, va[1].y,
                                   va[1].z, va[1].w, va[2].x, va[2].y, va[2].z, va[2].w};
             const float ns[12] = {na[0].x, na[0].y, na[0].z, na[0].w, na[1].x, na[1].y,
                                   na[1].z, na[1].w, na[2].x, na[2].y, na[2].z, na[2].w};
+            // this lane's column, from the block's first row
+            float* zc = Zs + l16 * P + rb + 3;
+            floatx4 zq[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) zq[q] = *reinterpret_cast<const floatx4*>(zc + 16 * q + 4 * lq);
             floatx4 w0 = {0.f, 0.f, 0.f, 0.f}, w1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < kQ2R / 4; s += 2) {
-                const float z0 = Zs[(rb + 4 * s + lq) * kQ2S + l16];
-                const float z1 = Zs[(rb + 4 * s + 4 + lq) * kQ2S + l16];
-                w0 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[s], z0, w0, 0, 0, 0);
-                w1 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[s + 1], z1, w1, 0, 0, 0);
+            for (int q = 0; q < 3; ++q) {
+                w0 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[4 * q], zq[q][0], w0, 0, 0, 0);
+                w1 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[4 * q + 1], zq[q][1], w1, 0, 0, 0);
+                w0 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[4 * q + 2], zq[q][2], w0, 0, 0, 0);
+                w1 = __builtin_amdgcn_mfma_f32_16x16x4f32(vs[4 * q + 3], zq[q][3], w1, 0, 0, 0);
             }
             // W as the next product's B operand: register s of lane (lq, l16)
             // must hold W(4 s + lq, l16), the accumulator holds W(4 lq + r,
@@ -1044,18 +1061,14 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
             }
             floatx4 acc[3];
 #pragma unroll
-            for (int rt = 0; rt < 3; ++rt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[rt][r] = Zs[(rb + rt * 16 + 4 * lq + r) * kQ2S + l16];
+            for (int rt = 0; rt < 3; ++rt) acc[rt] = *reinterpret_cast<const floatx4*>(zc + 16 * rt + 4 * lq);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int rt = 0; rt < 3; ++rt)
                     acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ns[4 * rt + s], wb[s], acc[rt], 0, 0, 0);
 #pragma unroll
-            for (int rt = 0; rt < 3; ++rt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Zs[(rb + rt * 16 + 4 * lq + r) * kQ2S + l16] = acc[rt][r];
+            for (int rt = 0; rt < 3; ++rt) *reinterpret_cast<floatx4*>(zc + 16 * rt + 4 * lq) = acc[rt];
             t = t2;
             gi = g2;
 #pragma unroll
@@ -1068,7 +1081,7 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
     __syncthreads();
     for (int e = tid; e < n * kQ2S; e += kQ2NT) {
         const int r = e / kQ2S, c = e % kQ2S;
-        if (c0 + c < n) Z[size_t(r) * ld + c0 + c] = Zs[e];
+        if (c0 + c < n) Z[size_t(r) * ld + c0 + c] = Zs[c * P + r + 3];
     }
 }
 
@@ -1084,7 +1097,7 @@ int q2_prep(int batch, int n, float* Q2, hipStream_t s) {
 int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s) {
     const int64_t nblk = q2_blocks(n);
     if (nblk <= 0) return CIR_OK;
-    const size_t sm = size_t(n + kQ2R) * kQ2S * sizeof(float);
+    const size_t sm = size_t(q2_pitch(n)) * kQ2S * sizeof(float);
     if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "q2_apply: n too large for the LDS slab");
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
