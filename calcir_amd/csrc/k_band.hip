@@ -268,7 +268,8 @@ __global__ void __launch_bounds__(256) sb_symm_kernel(int n, int ld, int r0, con
                                                       const float* __restrict__ Yhall, float* __restrict__ Pall) {
     constexpr int NB = kBandNB;
     __shared__ float tl[4][32][33];
-    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int m = n - r0, nbt = (m + 31) / 32;
     const int I = blockIdx.x * 4 + wave;
     if (I >= nbt) return;
@@ -281,34 +282,33 @@ __global__ void __launch_bounds__(256) sb_symm_kernel(int n, int ld, int r0, con
     // raw loads of tile J: the B operand (Yh rows J*32 + 2q + h) and the A
     // tile, as the A operand itself (row tile) or along the tile's rows for
     // the LDS transpose (column tile, T[2q + h][l32] = A22(J*32 + l32, I*32 + 2q + h))
+    // Every load is issued with its index clamped below m and nothing is
+    // masked until the values are used (the loads of a tile go out back to
+    // back, a tile ahead): rows and columns of the output past m are never
+    // stored, and K indices past m are cancelled by zeroing the B operand.
+    const int ric = min(ri, m - 1);
     auto load = [&](int J, float (&bv)[16], float (&raw)[16]) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int kr = J * 32 + 2 * q + h;
-            bv[q] = (kr < m) ? Yh[size_t(kr) * NB + l32] : 0.f;
-        }
-        if (J < I) {
+        for (int q = 0; q < 16; ++q) bv[q] = Yh[size_t(min(J * 32 + 2 * q + h, m - 1)) * NB + l32];
+        // row tile (J < I): A(J*32 + 2q + h, ri); column tile: A(I*32 + 2q + h,
+        // J*32 + l32) (columns of the column-major lower triangle)
+        const bool rowt = J < I;
+        const int c0 = (rowt ? J : I) * 32 + h;
+        const float* base = A + (rowt ? ric : min(J * 32 + l32, m - 1));
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int kc = J * 32 + 2 * q + h;
-                raw[q] = (ri < m) ? A[size_t(kc) * ld + ri] : 0.f;
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int gr = J * 32 + l32, gc = I * 32 + 2 * q + h;
-                raw[q] = (gr < m && gc < m) ? A[size_t(gc) * ld + gr] : 0.f;
-            }
-        }
+        for (int q = 0; q < 16; ++q) raw[q] = base[size_t(min(c0 + 2 * q, m - 1)) * ld];
     };
     floatx16 acc;
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.f;
-    float bv[16], raw[16];
-    load(0, bv, raw);
-    for (int J = 0; J < nbt; ++J) {
-        float bn[16], rn[16];
-        if (J + 1 < nbt) load(J + 1, bn, rn);
+    // tile J from (bv, raw) while tile J + 1 is read into (bn, rn) (past the
+    // last tile: clamped reads, never used); the two buffer pairs alternate,
+    // so no copy waits for a read in flight
+    auto tile = [&](int J, float (&bv)[16], float (&raw)[16], float (&bn)[16], float (&rn)[16]) {
+        load(J + 1, bn, rn);
+        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
+#pragma unroll
+        for (int q = 0; q < 16; ++q) bv[q] = (J * 32 + 2 * q + h < m) ? bv[q] : 0.f;
         float av[16];
         if (J < I) {
 #pragma unroll
@@ -328,13 +328,13 @@ __global__ void __launch_bounds__(256) sb_symm_kernel(int n, int ld, int r0, con
         }
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[q], bv[q], acc, 0, 0, 0);
-        if (J + 1 < nbt) {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                bv[q] = bn[q];
-                raw[q] = rn[q];
-            }
-        }
+    };
+    float b0[16], r0v[16], b1[16], r1v[16];
+    load(0, b0, r0v);
+    for (int J = 0; J < nbt; J += 2) {
+        tile(J, b0, r0v, b1, r1v);
+        if (J + 1 >= nbt) break;
+        tile(J + 1, b1, r1v, b0, r0v);
     }
     // acc(i, p): lane l holds column p = l32, rows 8 a + 4 h + c
 #pragma unroll
@@ -343,6 +343,10 @@ __global__ void __launch_bounds__(256) sb_symm_kernel(int n, int ld, int r0, con
         if (r < m) P[size_t(r) * 2 * NB + NB + l32] = acc[q];
     }
 }
+
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle, range checking on offsets).
+constexpr int kBufferWord3 = 0x00020000;
 
 // A22 -= W Y^T + Y W^T on the lower triangle, from the row-major panel P
 // (P[r * 2NB + k]: Y(r, k) for k < NB, W(r, k - NB) after).  Wave w of
@@ -357,7 +361,8 @@ __global__ void __launch_bounds__(256) sb_symm_kernel(int n, int ld, int r0, con
 __global__ void __launch_bounds__(256) sb_syr2k_kernel(int n, int ld, int r0, float* __restrict__ Aall,
                                                        const float* __restrict__ Pall) {
     constexpr int NB = kBandNB, PW = 2 * kBandNB;
-    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.y, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = n - r0, nbt = (m + 31) / 32;
     const int br = blockIdx.x * 4 + wave;
     if (br >= nbt) return;
@@ -365,50 +370,63 @@ __global__ void __launch_bounds__(256) sb_syr2k_kernel(int n, int ld, int r0, fl
     const float* P = Pall + size_t(b) * n * PW + size_t(r0) * PW;
     const int h = lane >> 5, l32 = lane & 31;
     const int r = br * 32 + l32;  // row of A22 this lane's column of the output holds
+    // Loads are issued with row and column indices clamped below m and never
+    // masked: an output element past m, or above the diagonal, is not stored,
+    // and no K index runs past the panel.
     auto panel_row = [&](int row, int half, float (&v)[32]) {
-        const float4* src = reinterpret_cast<const float4*>(P + size_t(row < m ? row : 0) * PW + half * NB);
+        const float4* src = reinterpret_cast<const float4*>(P + size_t(min(row, m - 1)) * PW + half * NB);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const float4 t = (row < m) ? src[q] : float4{0.f, 0.f, 0.f, 0.f};
+            const float4 t = src[q];
             v[4 * q] = t.x;
             v[4 * q + 1] = t.y;
             v[4 * q + 2] = t.z;
             v[4 * q + 3] = t.w;
         }
     };
+    // A22 of this matrix as a raw buffer (offsets in bytes, at most n * ld * 4)
+    const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(A, 0, 0x7fffffff, kBufferWord3);
+    const int rc = min(r, m - 1);
     auto load_old = [&](int bc, float (&o)[16]) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int c = bc * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
-            o[q] = (c < m && r < m && r >= c) ? A[size_t(c) * ld + r] : 0.f;
+            o[q] = A[size_t(min(c, m - 1)) * ld + rc];
         }
     };
-    float bv[32], av[32], old[16];
+    float bv[32];
     panel_row(r, 1 - h, bv);
-    panel_row(l32, h, av);
-    load_old(0, old);
-    for (int bc = 0; bc <= br; ++bc) {
-        float an[32], on[16];
-        if (bc < br) {
-            panel_row((bc + 1) * 32 + l32, h, an);
-            load_old(bc + 1, on);
-        }
+    // tile bc from (av, old) while tile bc + 1's operand and old values are
+    // read into (an, on) (past the last tile: clamped reads, never used); the
+    // two buffer pairs alternate, so no copy waits for a read in flight
+    auto tile = [&](int bc, float (&av)[32], float (&old)[16], float (&an)[32], float (&on)[16]) {
+        panel_row((bc + 1) * 32 + l32, h, an);
+        load_old(bc + 1, on);
+        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
         floatx16 acc;
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[q] = 0.f;
 #pragma unroll
         for (int q = 0; q < 32; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[q], bv[q], acc, 0, 0, 0);
+        // buffer stores, an element outside the lower triangle of A22 given
+        // an offset past the buffer's range (the store is dropped): no branch,
+        // so the count of memory operations in flight stays known and no wait
+        // lands on the reads of the next tile
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int c = bc * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
-            if (c < m && r < m && r >= c) A[size_t(c) * ld + r] = old[q] - acc[q];
+            const bool ok = c < m && r < m && r >= c;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(old[q] - acc[q]), rsA,
+                                                  ok ? (c * ld + r) * 4 : int(0x80000000), 0, 0);
         }
-        if (bc < br) {
-#pragma unroll
-            for (int q = 0; q < 32; ++q) av[q] = an[q];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) old[q] = on[q];
-        }
+    };
+    float a0[32], o0[16], a1[32], o1[16];
+    panel_row(l32, h, a0);
+    load_old(0, o0);
+    for (int bc = 0; bc <= br; bc += 2) {
+        tile(bc, a0, o0, a1, o1);
+        if (bc + 1 > br) break;
+        tile(bc + 1, a1, o1, a0, o0);
     }
 }
 
@@ -949,7 +967,7 @@ __host__ __device__ inline int q2_pitch(int n) {
 __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const float* __restrict__ Q2all,
                                                           float* __restrict__ Zall, int64_t nblk) {
     extern __shared__ __align__(16) float q2_smem[];
-    __shared__ int goff[kQ2MaxGroups], snstep;
+    __shared__ int goff[kQ2MaxGroups], done[kQ2MaxGroups], snstep, stuck;
     const int slab = blockIdx.x, b = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c0 = slab * kQ2S;
@@ -963,10 +981,12 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
         for (int g = 0; g < NG; ++g) {
             const int t = sb_tasks(n, g * kQ2G);
             goff[g] = o;
+            done[g] = 0;
             o += t;
             ns = max(ns, (NG - 1 - g) + t);
         }
         snstep = ns;
+        stuck = 0;
     }
     for (int e = tid; e < P * kQ2S; e += kQ2NT) {
         const int r = e / kQ2S - 3, c = e % kQ2S;
@@ -1011,19 +1031,42 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
         g = -1;
         while (g < 0 && ++tt < nstep) g = first_of(tt);
     };
+    // Blocks run in the step order without a barrier per step: block (gi,
+    // k) waits until group gi - 1 has finished position k, or all of its
+    // positions where it has no position k (done[gi - 1] >= min(k + 1, its
+    // task count)); with each wave's own earlier blocks this orders it after
+    // every block whose rows it shares and that must come first.  A wait
+    // points to a block of an earlier step and every wave takes its blocks in
+    // step order, so every wait ends.
     int t = 0, gi = first_of(0);
     if (gi < 0) advance(t, gi);
     float4 va[3], na[3];
-    if (t < nstep) load_ops(t, gi, va, na);
-    for (int step = 0; step < nstep; ++step) {
-        __syncthreads();
-        while (t == step) {
+    load_ops(gi < 0 ? 0 : t, gi < 0 ? 0 : gi, va, na);  // (0, 0) is a block of every matrix
+    while (t < nstep) {
+        {
             int t2 = t, g2 = gi;
             advance(t2, g2);
+            // the next block's operands (this block's again after the last)
             float4 vn[3], nn[3];
-            if (t2 < nstep) load_ops(t2, g2, vn, nn);
+            load_ops(t2 < nstep ? t2 : t, t2 < nstep ? g2 : gi, vn, nn);
+            const int k = t - gi;
+            if (gi > 0) {
+                // (bounded: a wait that ran far past any block's time marks
+                // the slab, which is then written back as NaN, instead of
+                // holding the GPU)
+                const int need = min(k + 1, tasks_of(gi - 1));
+                for (int spin = 0;
+                     __hip_atomic_load(&done[gi - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need; ++spin) {
+                    if (spin > (1 << 22) || stuck) {
+                        stuck = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             const int g = NG - 1 - gi;
-            const int rb = g * kQ2G + 1 + (t - gi) * kBandNB;
+            const int rb = g * kQ2G + 1 + k * kBandNB;
             const float vs[12] = {va[0].x, va[0].y, va[0].z, va[0].w, va[1].x, va[1].y,
                                   va[1].z, va[1].w, va[2].x, va[2].y, va[2].z, va[2].w};
             const float ns[12] = {na[0].x, na[0].y, na[0].z, na[0].w, na[1].x, na[1].y,
@@ -1069,6 +1112,8 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
                     acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ns[4 * rt + s], wb[s], acc[rt], 0, 0, 0);
 #pragma unroll
             for (int rt = 0; rt < 3; ++rt) *reinterpret_cast<floatx4*>(zc + 16 * rt + 4 * lq) = acc[rt];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            if (lane == 0) __hip_atomic_store(&done[gi], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             t = t2;
             gi = g2;
 #pragma unroll
@@ -1079,9 +1124,10 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
         }
     }
     __syncthreads();
+    const bool bad = stuck != 0;
     for (int e = tid; e < n * kQ2S; e += kQ2NT) {
         const int r = e / kQ2S, c = e % kQ2S;
-        if (c0 + c < n) Z[size_t(r) * ld + c0 + c] = Zs[c * P + r + 3];
+        if (c0 + c < n) Z[size_t(r) * ld + c0 + c] = bad ? __builtin_nanf("") : Zs[c * P + r + 3];
     }
 }
 
