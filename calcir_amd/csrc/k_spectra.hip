@@ -664,6 +664,12 @@ static int split_gemm(int gemm_prec, int M, int N, int K, int ks, int ns, size_t
     return CIR_OK;
 }
 
+bool propagate_needs_tuning(int prec, int gemm_prec, int n, int ncols) {
+    if (prec == CIR_PREC_FP64 || ncols > kPrMaxCols || std::min(kPropMaxSplit, std::max(1, n / 256)) < 2) return false;
+    std::lock_guard<std::mutex> lk(prop_tune().mu);
+    return prop_tune().best.find(std::make_tuple(n, ncols, gemm_prec == CIR_PREC_BF16X3)) == prop_tune().best.end();
+}
+
 int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
               const double* w, const void* Yin, void* Yout, void* G, hipStream_t s, bool may_tune) {
     if (n <= 0 || ncols <= 0 || ldy < 2 * ncols || ld < n) return fail(CIR_ERR_ARG, "propagate: bad sizes");

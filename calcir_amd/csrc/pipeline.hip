@@ -18,6 +18,7 @@
 #include <atomic>
 #include <thread>
 #include <cstdint>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -221,8 +222,12 @@ int auto_chunk(const cir_ctx* c, int nuniq) {
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::max(budget, 0.5 * double(fr));
     budget /= std::max(1, c->mem_share);
     int B = int(std::max(1.0, budget / per));
-    B = std::min(B, 512);
-    return std::max(1, std::min(B, nuniq));
+    B = std::max(1, std::min({B, 512, nuniq}));
+    // chunks of nearly equal size: a short last chunk would leave most CUs
+    // idle in the one-workgroup-per-matrix kernels (and at 16 or fewer large
+    // matrices take the one-matrix-at-a-time path)
+    const int nchunk = (nuniq + B - 1) / B;
+    return (nuniq + nchunk - 1) / nchunk;
 }
 
 // ---- checkpoint / resume (batched mode) ------------------------------------
@@ -469,7 +474,9 @@ struct BatchRun {
             CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
                                      w.as<double>(), sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st));
         }
-        for (int u = c0; u < c0 + nb; ++u) {
+        // frame by frame: propagation, origin bookkeeping, re-unitarisation,
+        // correlations
+        auto frame = [&](int u, bool may_tune, hipStream_t st) -> int {
             const int bl = u - c0;
             const char* Vu = V.as<char>() + size_t(bl) * N * ld * esz;
             const double* wu = w.as<double>() + size_t(bl) * N;
@@ -479,7 +486,7 @@ struct BatchRun {
             if (has_prop) {
                 ProfScope ps(pf, CIR_PH_PROPAGATE, st);
                 CIR_TRY(propagate(rp, c->prop_prec, N, ld, 9 * nslot, ldy, double(p.dt), Vu, wu, Ycur, Ynext, G.p,
-                                  st));
+                                  st, may_tune));
             }
             {
                 ProfScope ps(pf, CIR_PH_CORRELATE, st);
@@ -491,8 +498,56 @@ struct BatchRun {
                                   tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), st));
             }
             std::swap(Ycur, Ynext);
+            return CIR_OK;
+        };
+        int u = c0;
+        const int uend = c0 + nb;
+        if (use_graphs()) {
+            // the frames run directly until the propagation's split is
+            // settled (its first call times candidates), the rest of the chunk
+            // is captured into one graph and launched at once
+            while (u < uend && propagate_needs_tuning(rp, c->prop_prec, N, 9 * nslot)) CIR_TRY(frame(u++, true, st));
+            if (uend - u >= 2) {
+                // recorded on a stream of its own (the context's stream may be
+                // the null stream, which cannot be captured), launched on the
+                // context's stream
+                if (!cap) CIR_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+                hipGraph_t g = nullptr;
+                CIR_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                int rc = CIR_OK;
+                for (; u < uend && rc == CIR_OK; ++u) rc = frame(u, false, cap);
+                const hipError_t e = hipStreamEndCapture(cap, &g);
+                if (rc != CIR_OK) {
+                    if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                CIR_HIP(e);
+                hipGraphExec_t ge = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                CIR_HIP(ei);
+                graphs.push_back(ge);  // destroyed once the stream has passed it
+                CIR_HIP(hipGraphLaunch(ge, st));
+            }
         }
+        for (; u < uend; ++u) CIR_TRY(frame(u, true, st));
         return CIR_OK;
+    }
+
+    // Whether the frame loops run as graphs: not while phases are timed (the
+    // per-phase events need the launches one by one), and not under
+    // CIR_NO_GRAPHS=1 in the environment.
+    bool use_graphs() const {
+        if (c->prof.on) return false;
+        const char* e = std::getenv("CIR_NO_GRAPHS");
+        return !(e && std::atoi(e) == 1);
+    }
+    std::vector<hipGraphExec_t> graphs;
+    hipStream_t cap = nullptr;  // capture stream (records, never runs work)
+    ~BatchRun() {
+        if (!graphs.empty() && c) (void)hipStreamSynchronize(c->stream);
+        for (hipGraphExec_t ge : graphs) (void)hipGraphExecDestroy(ge);
+        if (cap) (void)hipStreamDestroy(cap);
     }
 
     // Host copies of the state a checkpoint holds.

@@ -40,6 +40,10 @@ int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, f
 size_t propagate_work_elems(int n, int ncols);
 int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,
               const double* w, const void* Yin, void* Yout, void* G, hipStream_t s, bool may_tune = true);
+// Whether propagate() of this shape would time candidates (false: its split is
+// settled, or the shape has none to choose), so that calls of it can be
+// captured into a graph.
+bool propagate_needs_tuning(int prec, int gemm_prec, int n, int ncols);
 int set_identity(int prec, int n, int ldf, void* F, hipStream_t s);
 int slot_fixup(int prec, int n, int nslot, int ldy, const int* action, const void* sites, const void* Yold,
                void* Ynew, hipStream_t s);

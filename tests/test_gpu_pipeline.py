@@ -542,3 +542,29 @@ def test_cli_rounds_the_frame_spacing_at_the_file_precision(tmp_path):
     p.time_prec = 1000.0
     with pytest.raises(C.CalcirError):
         C.run(p, xq, bq, tq)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("bf16x3", 1e-4)])
+def test_graph_frame_loops_equal_direct_launches(prec, tol, monkeypatch):
+    # each chunk's frame loop (propagation, origin bookkeeping,
+    # re-unitarisation, correlations) runs as one captured graph;
+    # CIR_NO_GRAPHS=1 launches the same kernels one by one.  Several chunks
+    # (chunk 8), re-unitarisation inside the graphs; the graph runs alternate
+    # with direct runs and each is checked against the direct result, to the
+    # run-to-run rounding of the float atomics (2e-14 seen in FP64 spectra,
+    # 1.2e-5 in BF16x3 ones; a wrong frame or buffer in a graph is O(1)).
+    xyz, box, times = frames(128, 60, 1001)
+    p = C.default_params(precision=prec, mode="batched", chunk=8, reunitarise_every=4, **KW)
+    monkeypatch.setenv("CIR_NO_GRAPHS", "1")
+    announce("graph vs direct: direct %s" % prec)
+    ref = C.run(p, xyz, box, times)
+    for rep in range(2):
+        monkeypatch.delenv("CIR_NO_GRAPHS")
+        announce("graph vs direct: graphs %s, run %d" % (prec, rep))
+        g = C.run(p, xyz, box, times)
+        for k in ("tcf_ir", "tcf_vv", "tcf_vh", "ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw"):
+            assert rel(g[k], ref[k]) <= tol, (k, rel(g[k], ref[k]))
+        monkeypatch.setenv("CIR_NO_GRAPHS", "1")
+        d = C.run(p, xyz, box, times)
+        for k in ("tcf_ir", "ftcf_ir", "sw"):
+            assert rel(d[k], ref[k]) <= tol, (k, rel(d[k], ref[k]))
