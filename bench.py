@@ -412,6 +412,8 @@ def main():
                      "traffic": traffic_per_launch(mats_launch), "avg_launch_ms": 1e3 * per_launch,
                      "matrices_per_launch": mats_launch, "flops_per_matrix": fl_bt},
         "phases_s_per_run": {k: v[0] / args.steps for k, v in phases.items()},
+        # launch shapes the library chose by timing on first use (warmup)
+        "tuning": {" ".join(str(x) for x in k): {"choice": v[0], "ms": v[1]} for k, v in C.tuning_report().items()},
         "trajectory_frames_per_s": nmat / elapsed,
         "eigensolver": {"matrices": nmat, "n": n, "seconds": eig_s, "ms_per_matrix": 1e3 * eig_s / nmat,
                         "reduction_flops_per_matrix": fl_red, "reduction_s": red_s,

@@ -127,6 +127,7 @@ SIGNATURES = {
     "cir_reunitarise": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "cir_set_profiling": (_i, [_vp, _i]),
     "cir_phase_times": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll), _i]),
+    "cir_tuning_report": (_i, [ctypes.c_char_p, _i]),
     "cir_destroy": (None, [_vp]),
     "cir_map_coefs": (_i, [_i, _i, _vp]),
     "cir_get_coefs": (_i, [_vp, _vp]),
@@ -479,6 +480,22 @@ def run_streamed(p, times, blocks, natoms, profile=False, coefs=None):
     finally:
         L.cir_destroy(ctx)
     return _results_dict(r, bufs, phases)
+
+
+def tuning_report():
+    """The launch configurations this process has chosen by timing
+    (cir_tuning_report): {(family, bucket...): (choice, [ms per candidate])}."""
+    L = lib()
+    n = L.cir_tuning_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    L.cir_tuning_report(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        head, _, rest = line.partition(" -> ")
+        fam, *shape = head.split()
+        choice, _, times = rest.partition(" (")
+        out[(fam,) + tuple(int(x) for x in shape)] = (int(choice), [float(x) for x in times.rstrip(" ms)").split()])
+    return out
 
 
 def run_many(jobs, nthreads=8):

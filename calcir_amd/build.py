@@ -19,7 +19,7 @@ CLI = os.path.join(HERE, "bin", "calcir")
 
 HIP_SOURCES = ["k_sites.hip", "k_gemm.hip", "k_eigh.hip", "k_band.hip", "k_spectra.hip", "pipeline.hip",
                "adjoint.hip", "noalloc.hip"]
-CXX_SOURCES = ["common.cpp", "host_io.cpp", "synth.cpp", "xtc.cpp", "cli.cpp"]
+CXX_SOURCES = ["common.cpp", "tuner.cpp", "host_io.cpp", "synth.cpp", "xtc.cpp", "cli.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,

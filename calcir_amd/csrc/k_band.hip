@@ -31,6 +31,7 @@
 
 #include "band.hpp"
 #include "gemm.hpp"
+#include "tuner.hpp"
 
 namespace cir {
 
@@ -955,7 +956,6 @@ __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float*
 // one product to the next in registers.
 constexpr int kQ2MaxGroups = 160;
 static_assert(kQ2MaxGroups * kQ2G >= 4 * 512 + kBandNB, "group table too small for sb_max_n()");
-constexpr int kQ2NT = 1024;
 // The slab is stored column by column, row r of Z at index r + 3 of its
 // column (block rows then start at multiples of 4, so a lane's 4 rows are one
 // 16-byte access), with a column pitch of 4 mod 64 floats (the 16 columns of
@@ -964,6 +964,7 @@ __host__ __device__ inline int q2_pitch(int n) {
     const int p = n + kQ2R + 3;
     return p + ((4 - p) & 63);
 }
+template <int kQ2NT>
 __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const float* __restrict__ Q2all,
                                                           float* __restrict__ Zall, int64_t nblk) {
     extern __shared__ __align__(16) float q2_smem[];
@@ -1140,17 +1141,48 @@ int q2_prep(int batch, int n, float* Q2, hipStream_t s) {
     return CIR_OK;
 }
 
+// Workgroup width of q2_apply_kernel: 1024 threads (16 waves, candidate 0)
+// or 512, timed per size bucket on the first call (tuner.hpp) over a copy of
+// the first few matrices' eigenvectors (the kernel works in place).
+template <int NT>
+static int q2_launch(int batch, int n, int ld, const float* Q2, float* Z, int64_t nblk, hipStream_t s) {
+    const size_t sm = size_t(q2_pitch(n)) * kQ2S * sizeof(float);
+    CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel<NT>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
+    dim3 grid(unsigned(ceil_div(n, kQ2S)), unsigned(batch));
+    hipLaunchKernelGGL(q2_apply_kernel<NT>, grid, dim3(NT), sm, s, n, ld, Q2, Z, nblk);
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
 int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s) {
     const int64_t nblk = q2_blocks(n);
     if (nblk <= 0) return CIR_OK;
     const size_t sm = size_t(q2_pitch(n)) * kQ2S * sizeof(float);
     if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "q2_apply: n too large for the LDS slab");
-    CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
-    dim3 grid(unsigned(ceil_div(n, kQ2S)), unsigned(batch));
-    hipLaunchKernelGGL(q2_apply_kernel, grid, dim3(kQ2NT), sm, s, n, ld, Q2, Z, nblk);
-    CIR_CHECK_LAUNCH();
-    return CIR_OK;
+    auto launch = [&](int cand, int nb, float* Zp) {
+        return cand == 0 ? q2_launch<1024>(nb, n, ld, Q2, Zp, nblk, s) : q2_launch<512>(nb, n, ld, Q2, Zp, nblk, s);
+    };
+    const TuneKey key{"q2_apply_threads", {size_bucket(n), 0, 0, 0, 0}};
+    int cand = tune_lookup(key);
+    if (cand < 0 && tune_allowed(s)) {
+        const int nb = std::min(batch, 8);
+        const size_t bytes = size_t(nb) * n * ld * sizeof(float);
+        float* scratch = nullptr;
+        if (hipMalloc(&scratch, bytes) != hipSuccess) {
+            (void)hipGetLastError();  // no room to time: the default width
+            scratch = nullptr;
+        } else {
+            int err = CIR_OK;
+            const hipError_t he = hipMemcpyAsync(scratch, Z, bytes, hipMemcpyDeviceToDevice, s);
+            if (he != hipSuccess) err = hip_fail(he, "hipMemcpyAsync", __FILE__, __LINE__);
+            if (err == CIR_OK) cand = tune_time(key, 2, s, [&](int c) { return launch(c, nb, scratch); }, &err);
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(scratch);
+            if (err != CIR_OK) return err;
+        }
+    }
+    return launch(cand < 0 ? 0 : cand, batch, Z);
 }
 
 }  // namespace cir

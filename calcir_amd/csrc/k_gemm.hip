@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include "gemm.hpp"
+#include "tuner.hpp"
 
 namespace cir {
 
@@ -394,15 +395,30 @@ int gemm(int prec, const GemmArgs& g, hipStream_t s) {
         max_offset(g.K, g.b_k, g.N, g.b_n) >= (int64_t(1) << 31))
         return fail(CIR_ERR_ARG, "gemm: a matrix spans more than 2^31 elements");
     const bool big = g.M >= 128 && g.N >= 128;
-    if (prec == CIR_PREC_BF16X3) {
-        // short-K updates C += A B: 64x64 tiles with the C tile read ahead
-        // of the K loop (the 128x128 tile has no registers left for it).  At
-        // K = 256 (the back-transform's Z -= Y W) the 128x128 tile's operand
-        // reuse wins instead.
-        if (g.beta != 0.0 && g.K <= 128) return launch_layout<64, 64, 32, true, true>(g, s);
-        return big ? launch_layout<128, 128, 32, true>(g, s) : launch_layout<64, 64, 32, true>(g, s);
+    const bool split = prec == CIR_PREC_BF16X3;
+    // short-K updates C += A B in BF16x3: 64x64 tiles with the C tile read
+    // ahead of the K loop (the 128x128 tile has no registers left for it)
+    if (split && g.beta != 0.0 && g.K <= 128) return launch_layout<64, 64, 32, true, true>(g, s);
+    // tile 0: 128x128 (the default where both sides reach 128), 1: 64x64
+    auto launch = [&](int tile) {
+        if (split)
+            return tile == 0 ? launch_layout<128, 128, 32, true>(g, s) : launch_layout<64, 64, 32, true>(g, s);
+        return tile == 0 ? launch_layout<128, 128, 16, false>(g, s) : launch_layout<64, 64, 16, false>(g, s);
+    };
+    if (!big) return launch(1);
+    // Products that overwrite C (beta = 0) can be run again, so the first of
+    // each size bucket times both tiles; updates of C take the choice of
+    // their bucket's product, or the larger tile.
+    TuneKey key{"gemm", {int64_t(prec), size_bucket(g.M), size_bucket(g.N), size_bucket(g.K),
+                         size_bucket(g.batch) * 4 + (g.a_m == 1 ? 2 : 0) + (g.b_n == 1 ? 1 : 0)}};
+    int tile = tune_lookup(key);
+    if (tile < 0 && g.beta == 0.0 && tune_allowed(s)) {
+        int err = CIR_OK;
+        tile = tune_time(key, 2, s, launch, &err);
+        if (err != CIR_OK) return err;
+        return launch(tile);
     }
-    return big ? launch_layout<128, 128, 16, false>(g, s) : launch_layout<64, 64, 16, false>(g, s);
+    return launch(tile < 0 ? 0 : tile);
 }
 
 }  // namespace cir

@@ -10,6 +10,7 @@
 
 #include "gemm.hpp"
 #include "spectra.hpp"
+#include "tuner.hpp"
 
 namespace cir {
 
@@ -513,56 +514,36 @@ __global__ void __launch_bounds__(256) zero_rows2_kernel(int n, int w2, int ld0,
 // by timing the candidates once per process (the first call of a shape whose
 // output does not alias its input); the default guess aims at about 4
 // workgroups per CU with K segments of at least 256.
-struct PropTune {
-    std::mutex mu;
-    std::map<std::tuple<int, int, bool>, int> best;
-};
-static PropTune& prop_tune() {
-    static PropTune t;
-    return t;
-}
-
 static int prop_pair(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
                      const float* Yin, float* Yout, float* G, int ks, hipStream_t s);
+
+// The K split of the two products: 1, 2, 4 or 8 segments (at most n / 256),
+// timed on the first call of each shape (tuner.hpp; the products overwrite
+// Yout from Yin, so they can be run again when Yout does not alias Yin).
+static TuneKey prop_key(int n, int ncols, bool split3) { return TuneKey{"prop_ksplit", {n, ncols, split3 ? 1 : 0, 0, 0}}; }
+static int prop_ncand(int n) {
+    int c = 0;
+    for (int k = 1; k <= std::min(kPropMaxSplit, std::max(1, n / 256)); k *= 2) ++c;
+    return c;
+}
 
 static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double dt, const float* V, const double* w,
                          const float* Yin, float* Yout, float* G, hipStream_t s, bool may_tune) {
     const int nmt_pad = int(ceil_div(ceil_div(n, 32), 8) * 8);
     const int tiles = nmt_pad * int(ceil_div(ncols, 16));
     const int guess = std::max(1, std::min({kPropMaxSplit, int(ceil_div(1024, tiles)), std::max(1, n / 256)}));
-    const auto key = std::make_tuple(n, ncols, split3);
+    const TuneKey key = prop_key(n, ncols, split3);
     int ks = guess;
-    bool tune = false;
-    {
-        std::lock_guard<std::mutex> lk(prop_tune().mu);
-        auto it = prop_tune().best.find(key);
-        if (it != prop_tune().best.end())
-            ks = it->second;
-        else
-            tune = may_tune && Yin != Yout;
-    }
-    if (tune) {
-        EventPair ev;
-        CIR_HIP(ev.create());
-        hipEvent_t e0 = ev.a, e1 = ev.b;
-        float best_ms = 0.f;
-        int best_ks = guess;
-        for (int cand = 1; cand <= std::min(kPropMaxSplit, std::max(1, n / 256)); cand *= 2) {
-            CIR_TRY(prop_pair(split3, n, ld, ncols, ldy, dt, V, w, Yin, Yout, G, cand, s));  // warm
-            CIR_HIP(hipEventRecord(e0, s));
-            CIR_TRY(prop_pair(split3, n, ld, ncols, ldy, dt, V, w, Yin, Yout, G, cand, s));
-            CIR_HIP(hipEventRecord(e1, s));
-            CIR_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            CIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (cand == 1 || ms < best_ms) {
-                best_ms = ms;
-                best_ks = cand;
-            }
-        }
-        ks = best_ks;
-        std::lock_guard<std::mutex> lk(prop_tune().mu);
-        prop_tune().best.emplace(key, ks);
+    const int kept = tune_lookup(key);
+    if (kept >= 0) {
+        ks = 1 << kept;
+    } else if (may_tune && Yin != Yout && prop_ncand(n) > 1 && tune_allowed(s)) {
+        int err = CIR_OK;
+        const int c = tune_time(key, prop_ncand(n), s, [&](int i) {
+            return prop_pair(split3, n, ld, ncols, ldy, dt, V, w, Yin, Yout, G, 1 << i, s);
+        }, &err);
+        if (err != CIR_OK) return err;
+        ks = 1 << c;
     }
     return prop_pair(split3, n, ld, ncols, ldy, dt, V, w, Yin, Yout, G, ks, s);
 }
@@ -665,9 +646,8 @@ static int split_gemm(int gemm_prec, int M, int N, int K, int ks, int ns, size_t
 }
 
 bool propagate_needs_tuning(int prec, int gemm_prec, int n, int ncols) {
-    if (prec == CIR_PREC_FP64 || ncols > kPrMaxCols || std::min(kPropMaxSplit, std::max(1, n / 256)) < 2) return false;
-    std::lock_guard<std::mutex> lk(prop_tune().mu);
-    return prop_tune().best.find(std::make_tuple(n, ncols, gemm_prec == CIR_PREC_BF16X3)) == prop_tune().best.end();
+    if (prec == CIR_PREC_FP64 || ncols > kPrMaxCols || prop_ncand(n) < 2) return false;
+    return tune_lookup(prop_key(n, ncols, gemm_prec == CIR_PREC_BF16X3)) < 0 && tune_allowed(nullptr);
 }
 
 int propagate(int prec, int gemm_prec, int n, int ld, int ncols, int ldy, double dt, const void* V,

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "tuner.hpp"
 #include "ctx.hpp"
 #include "gemm.hpp"
 #include "spectra.hpp"
@@ -940,6 +941,16 @@ extern "C" int cir_set_profiling(cir_ctx* c, int on) {
     if (!c) return fail(CIR_ERR_ARG, "null context");
     c->prof.on = on != 0;
     return CIR_OK;
+}
+
+extern "C" int cir_tuning_report(char* buf, int buflen) {
+    const std::string r = tune_report();
+    if (buf && buflen > 0) {
+        const size_t k = std::min(r.size(), size_t(buflen - 1));
+        std::memcpy(buf, r.data(), k);
+        buf[k] = '\0';
+    }
+    return int(r.size());
 }
 
 extern "C" int cir_phase_times(cir_ctx* c, double* seconds, long long* groups, int nphase) {

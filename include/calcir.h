@@ -298,6 +298,13 @@ int   cir_set_profiling(cir_ctx* ctx, int on);
 /* Device seconds and timed launch groups per phase of the last run
  * (arrays of nphase <= CIR_NPHASE entries). */
 int   cir_phase_times(cir_ctx* ctx, double* seconds, long long* groups, int nphase);
+
+/* The launch configurations chosen so far in this process by timing on first
+ * use (no counterpart in the reference: calcIR launches fixed shapes), as
+ * text, one line per size bucket: family, bucket, choice, candidates' times.
+ * Writes at most buflen bytes (NUL-terminated) and returns the length of the
+ * whole report.  CIR_NO_TUNE=1 in the environment turns the timing off. */
+int   cir_tuning_report(char* buf, int buflen);
 void  cir_destroy(cir_ctx* ctx);
 
 /* --- map coefficients and the lineshape adjoint ---------------------------- */

@@ -486,3 +486,36 @@ def test_reunitarise_restores_the_gram_matrix(prec):
             assert after <= 4 * float(before) ** 2 + floor, (before, after)
             if before_tol == 0.0:
                 assert (out[q] - blocks[q]).abs().max() <= (1e-5 if prec == C.PREC_FP32 else 1e-12) * blocks[q].abs().max()
+
+
+def test_tuner_times_each_size_bucket_once(monkeypatch):
+    # launch shapes timed on first use (the Q2 back-transform's workgroup
+    # width, the GEMM tile, the propagation's K split), one choice per size
+    # bucket for the process; with CIR_NO_TUNE=1 the defaults run and the
+    # eigenpairs are as good
+    rng = np.random.default_rng(5)
+    n, batch = 1100, 17
+    a = rng.standard_normal((batch, n, n)).astype(np.float32)
+    H = (a + np.transpose(a, (0, 2, 1))) * 15.0
+    ld = (n + 3) // 4 * 4
+    Hd = np.zeros((batch, n, ld), np.float32)
+    Hd[:, :, :n] = np.transpose(H, (0, 2, 1))
+    monkeypatch.setenv("CIR_EIGH_STAGES", "2")
+    out = {}
+    for tune in ("0", "1"):
+        monkeypatch.setenv("CIR_NO_TUNE", "1" if tune == "1" else "0")
+        announce("tuner: eigh n=%d batch=%d no_tune=%s" % (n, batch, tune))
+        w, V = S.eigh(torch.from_numpy(Hd).to(DEV).contiguous(), n)
+        out[tune] = (w.cpu().numpy(), V.cpu().double().numpy())
+    rep = C.tuning_report()
+    q2 = [v for k, v in rep.items() if k[0] == "q2_apply_threads" and k[1] == 2048]
+    assert len(q2) == 1 and len(q2[0][1]) == 2 and all(t > 0 for t in q2[0][1]), rep
+    assert any(k[0] == "gemm" for k in rep), rep
+    for tune, (w, V) in out.items():
+        for b in (0, 16):
+            Hb = H[b].astype(np.float64)
+            Vb = V[b][:, :n]
+            scale = np.abs(w[b]).max()
+            assert np.abs(Hb @ Vb - Vb * w[b]).max() / scale < 50 * 6e-8 * np.sqrt(n), tune
+            assert np.abs(Vb.T @ Vb - np.eye(n)).max() < 50 * 6e-8 * np.sqrt(n), tune
+    np.testing.assert_allclose(out["0"][0], out["1"][0], rtol=0, atol=1e-3)
