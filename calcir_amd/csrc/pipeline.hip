@@ -651,6 +651,20 @@ struct StreamRun {
 };
 void stream_release(StreamRun* s) { delete s; }
 
+// A captured eigensolve of the plain control flow, released after its stream
+// has passed it.
+struct PlainGraph {
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr, stream = nullptr;
+    ~PlainGraph() {
+        if (exec) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipGraphExecDestroy(exec);
+        }
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
 int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, const float* times) {
     const cir_params& p = c->p;
     if (p.ckpt_every > 0 || p.resume != 0 || p.max_frames > 0)
@@ -683,6 +697,37 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
     CIR_HIP(hipMemsetAsync(acc.p, 0, acc.bytes, st));
     CIR_HIP(hipMemsetAsync(skip.p, 0, skip.bytes, st));
     double* accp = acc.as<double>();
+    // The eigensolve of every step reads and writes the same buffers with the
+    // same launches (one matrix across the GPU: a few launches per
+    // reflector), so its second call is captured into a graph and later steps
+    // replay it (the first call sets kernel attributes and times its product
+    // tiles).  Not while phases are timed, nor under CIR_NO_GRAPHS=1.
+    const char* ng = std::getenv("CIR_NO_GRAPHS");
+    const bool graphs = !c->prof.on && !(ng && std::atoi(ng) == 1);
+    PlainGraph eg;
+    int ecalls = 0;
+    auto eigh_step = [&]() -> int {
+        if (!graphs || ecalls++ == 0) return eigh_batched(c->prop_prec, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st,
+                                                         &c->prof);
+        if (!eg.exec) {
+            CIR_HIP(hipStreamCreateWithFlags(&eg.cap, hipStreamNonBlocking));
+            hipGraph_t g = nullptr;
+            CIR_HIP(hipStreamBeginCapture(eg.cap, hipStreamCaptureModeThreadLocal));
+            const int rc = eigh_batched(c->prop_prec, 1, N, ld, H.p, V.p, w.as<double>(), work.p, eg.cap, nullptr);
+            const hipError_t e = hipStreamEndCapture(eg.cap, &g);
+            if (rc != CIR_OK) {
+                if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            CIR_HIP(e);
+            const hipError_t ei = hipGraphInstantiate(&eg.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            CIR_HIP(ei);
+            eg.stream = st;
+        }
+        CIR_HIP(hipGraphLaunch(eg.exec, st));
+        return CIR_OK;
+    };
     for (int s = 0; s < ns; ++s) {
         for (int k = 0; k < nt; ++k) {
             const int f = S.fidx[size_t(s) * nt + k];
@@ -698,7 +743,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
                 CIR_TRY(hamiltonian(rp, 1, N, ld, p.nchrom_mol, c->ispecies, c->mc, p.avef, box1.as<float>(), c->boxk,
                                         sites1.p, H.p, st));
             }
-            CIR_TRY(eigh_batched(c->prop_prec, 1, N, ld, H.p, V.p, w.as<double>(), work.p, st, pf));
+            CIR_TRY(eigh_step());
             {
                 ProfScope ps(pf, CIR_PH_IPR, st);
                 CIR_TRY(ipr_hist(rp, 1, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
