@@ -62,6 +62,7 @@ class Params(ctypes.Structure):
         ("max_frames", ctypes.c_int), ("adj_store", ctypes.c_int), ("triclinic", ctypes.c_int),
         ("reunit_every", ctypes.c_int), ("origin_stride", ctypes.c_int),
         ("oh_per_molecule", ctypes.c_int), ("msite_bisector", ctypes.c_int), ("time_prec", ctypes.c_float),
+        ("deterministic", ctypes.c_int),
     ]
 
 

@@ -63,6 +63,7 @@ extern "C" void cir_params_default(cir_params* p) {
     p->oh_per_molecule = 0;
     p->msite_bisector = 0;
     p->time_prec = 1000.0f;
+    p->deterministic = 0;
 }
 
 extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) try {
@@ -111,6 +112,7 @@ extern "C" int cir_params_read(const char* path, cir_params* p, int verbose) try
         else if (!std::strcmp(para, "oh_per_molecule")) std::sscanf(value, "%d", &p->oh_per_molecule);
         else if (!std::strcmp(para, "msite_bisector")) std::sscanf(value, "%d", &p->msite_bisector);
         else if (!std::strcmp(para, "time_precision")) std::sscanf(value, "%f", &p->time_prec);
+        else if (!std::strcmp(para, "deterministic")) std::sscanf(value, "%d", &p->deterministic);
         else if (!std::strcmp(para, "precision")) {
             if (!std::strcmp(value, "fp64")) p->precision = CIR_PREC_FP64;
             else if (!std::strcmp(value, "bf16x3")) p->precision = CIR_PREC_BF16X3;
@@ -158,6 +160,7 @@ extern "C" void cir_params_echo(const cir_params* p) {
     if (p->origin_stride > 0) std::printf("\tTime origins every %d frames\n", p->origin_stride);
     if (p->oh_per_molecule) std::printf("\tChromophore k of a molecule is its O-H%s stretch\n", "(1+k)");
     if (p->msite_bisector) std::printf("\tThree-site water: -1.04 site on the HOH bisector\n");
+    if (p->deterministic) std::printf("\tDeterministic reductions\n");
 }
 
 extern "C" int cir_resolve_model(const char* model, int* imodel) try {

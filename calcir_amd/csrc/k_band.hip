@@ -848,6 +848,9 @@ __global__ void __launch_bounds__(kSbNT) sb2st_kernel(int n, float* __restrict__
     }
 }
 
+bool deterministic_eigh_size(int n) { return n >= 2 * kBandNB + 2 && n <= sb_max_n(); }
+int deterministic_eigh_max() { return sb_max_n(); }
+
 int sb2st(int batch, int n, float* BND, float* d, float* e, float* Q2, hipStream_t s) {
     if (n < 1) return fail(CIR_ERR_ARG, "sb2st: bad size");
     const size_t sm = kSbSmemFloats * sizeof(float);

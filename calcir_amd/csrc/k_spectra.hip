@@ -85,6 +85,77 @@ ipr_hist_kernel(int n, int ld, int nchrom_mol, float avef, float omegaStart, int
     }
 }
 
+// Deterministic form: per mode its bin (unclamped) and IPR / molecular IPR,
+// then one lane per bin sums over the frames in order and over the frame's
+// modes of that bin in order (the bins ascend with the eigenvalues, so those
+// modes are one run, found by binary search).
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+ipr_modes_kernel(int n, int ld, int nchrom_mol, float avef, float omegaStart, int omegaStep, int nomega,
+                 const T* __restrict__ Vall, const double* __restrict__ wall, const double* __restrict__ weight,
+                 int* __restrict__ mbin, double* __restrict__ mval, long long* __restrict__ skipped) {
+    const int b = blockIdx.y, k = blockIdx.x * kSpNT + threadIdx.x;
+    if (k >= n) return;
+    const double wgt = weight ? weight[b] : 1.0;
+    const T* V = Vall + size_t(b) * n * ld;
+    T s4 = T(0), outer = T(0), inner = T(0);
+    int cnt = 0;
+    for (int i = 0; i < n; ++i) {
+        const T c = V[size_t(i) * ld + k];
+        const T c2 = c * c;
+        s4 += c2 * c2;
+        inner += c2;
+        if (++cnt == nchrom_mol) {
+            outer += inner * inner;
+            inner = T(0);
+            cnt = 0;
+        }
+    }
+    const float wi = float(wall[size_t(b) * n + k]) + avef;
+    const int bin = int(roundf((wi - omegaStart) / float(omegaStep)));
+    const size_t e = size_t(b) * n + k;
+    mbin[e] = bin;
+    mval[2 * e] = double(T(1) / s4);
+    mval[2 * e + 1] = double(T(1) / outer);
+    if ((bin < 0 || bin >= nomega) && wgt != 0.0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(skipped), (unsigned long long)(llrint(wgt)));  // integer: any order
+}
+
+__global__ void __launch_bounds__(kSpNT)
+ipr_bins_kernel(int n, int batch, int omegaStep, int nomega, const double* __restrict__ weight,
+                const int* __restrict__ mbin, const double* __restrict__ mval, double* __restrict__ pw,
+                double* __restrict__ rw, double* __restrict__ rmw) {
+    const int q = blockIdx.x * kSpNT + threadIdx.x;
+    if (q >= nomega) return;
+    double sp = 0.0, sr = 0.0, sm = 0.0;
+    for (int b = 0; b < batch; ++b) {
+        const double wgt = weight ? weight[b] : 1.0;
+        if (wgt == 0.0) continue;
+        const int* bins = mbin + size_t(b) * n;
+        int lo = 0, hi = n;  // first mode with bin >= q
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (bins[mid] < q)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const double inv = wgt / double(omegaStep);
+        for (int k = lo; k < n && bins[k] == q; ++k) {
+            const double* v = mval + 2 * (size_t(b) * n + k);
+            sp += inv;
+            sr += inv * v[0];
+            sm += inv * v[1];
+        }
+    }
+    pw[q] += sp;
+    rw[q] += sr;
+    rmw[q] += sm;
+}
+
+size_t ipr_hist_det_bytes(int batch, int n) { return size_t(batch) * n * (sizeof(int) + 2 * sizeof(double)) + 256; }
+size_t spectral_density_det_bytes(int batch, int nomega) { return size_t(batch) * nomega * sizeof(double); }
+
 // |V^T mu|^2 per mode (calcIR.cu:462-467 fused).
 template <typename T>
 __global__ void __launch_bounds__(kSpNT)
@@ -107,15 +178,20 @@ mu2_kernel(int n, int ld, const T* __restrict__ Vall, const T* __restrict__ site
 }
 
 // Lorentzian sum over modes per omega bin (calcIR.cu:1414-1444).
+// (per_frame: frame b's term written to per_frame[b * nomega + q] instead of
+// added to sw, for the fixed-order sum of sw_frames_kernel)
 __global__ void __launch_bounds__(kSpNT)
 sw_kernel(int n, int nomega, float omegaStart, int omegaStep, double gamma, float avef,
           const double* __restrict__ m2all, const double* __restrict__ wall,
-          const double* __restrict__ weight, double* __restrict__ sw) {
+          const double* __restrict__ weight, double* __restrict__ sw, double* __restrict__ per_frame) {
     const int b = blockIdx.y;
     const double wgt = weight ? weight[b] : 1.0;
-    if (wgt == 0.0) return;
     const int q = blockIdx.x * kSpNT + threadIdx.x;
     if (q >= nomega) return;
+    if (wgt == 0.0) {
+        if (per_frame) per_frame[size_t(b) * nomega + q] = 0.0;
+        return;
+    }
     const double om = double(float(omegaStart + float(omegaStep * q)));  // calcIR.cu:271
     const double* m2 = m2all + size_t(b) * n;
     const double* w = wall + size_t(b) * n;
@@ -124,7 +200,19 @@ sw_kernel(int n, int nomega, float omegaStart, int omegaStep, double gamma, floa
         const double dw = om - (w[k] + double(avef));
         acc += m2[k] * gamma / (dw * dw + gamma * gamma) / kPi;
     }
-    atomicAdd(&sw[q], wgt * acc);
+    if (per_frame)
+        per_frame[size_t(b) * nomega + q] = wgt * acc;
+    else
+        atomicAdd(&sw[q], wgt * acc);
+}
+
+__global__ void __launch_bounds__(kSpNT)
+sw_frames_kernel(int batch, int nomega, const double* __restrict__ per_frame, double* __restrict__ sw) {
+    const int q = blockIdx.x * kSpNT + threadIdx.x;
+    if (q >= nomega) return;
+    double acc = 0.0;
+    for (int b = 0; b < batch; ++b) acc += per_frame[size_t(b) * nomega + q];
+    sw[q] += acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -305,9 +393,25 @@ c2r_kernel(int nfft, int nx, const double* __restrict__ X, double* __restrict__ 
 // ---------------------------------------------------------------------------
 int ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart,
              int omegaStep, int nomega, const void* V, const double* w, const double* weight, double* pw,
-             double* rw, double* rmw, long long* skipped, hipStream_t s) {
+             double* rw, double* rmw, long long* skipped, hipStream_t s, void* det) {
     if (batch <= 0 || n <= 0 || ld < n || nomega <= 0 || omegaStep == 0 || nchrom_mol <= 0)
         return fail(CIR_ERR_ARG, "ipr_hist: bad sizes");
+    if (det) {
+        double* mval = static_cast<double*>(det);
+        int* mbin = reinterpret_cast<int*>(mval + 2 * size_t(batch) * n);
+        dim3 grid(unsigned(ceil_div(n, kSpNT)), unsigned(batch));
+        if (prec == CIR_PREC_FP64)
+            hipLaunchKernelGGL(ipr_modes_kernel<double>, grid, dim3(kSpNT), 0, s, n, ld, nchrom_mol, avef, omegaStart,
+                               omegaStep, nomega, static_cast<const double*>(V), w, weight, mbin, mval, skipped);
+        else
+            hipLaunchKernelGGL(ipr_modes_kernel<float>, grid, dim3(kSpNT), 0, s, n, ld, nchrom_mol, avef, omegaStart,
+                               omegaStep, nomega, static_cast<const float*>(V), w, weight, mbin, mval, skipped);
+        CIR_CHECK_LAUNCH();
+        hipLaunchKernelGGL(ipr_bins_kernel, dim3(unsigned(ceil_div(nomega, kSpNT))), dim3(kSpNT), 0, s, n, batch,
+                           omegaStep, nomega, weight, mbin, mval, pw, rw, rmw);
+        CIR_CHECK_LAUNCH();
+        return CIR_OK;
+    }
     const size_t sm = 3 * size_t(nomega) * sizeof(double);
     const int use_lds = sm <= 96 * 1024 ? 1 : 0;
     dim3 grid(unsigned(ceil_div(n, kSpNT)), unsigned(batch));
@@ -330,7 +434,7 @@ int ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, flo
 
 int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, float omegaStart,
                      int omegaStep, int nomega, const void* V, const double* w, const void* sites,
-                     const double* weight, double* sw, double* m2work, hipStream_t s) {
+                     const double* weight, double* sw, double* m2work, hipStream_t s, void* det) {
     if (batch <= 0 || n <= 0 || ld < n || nomega <= 0) return fail(CIR_ERR_ARG, "spectral_density: bad sizes");
     dim3 g1(unsigned(ceil_div(n, kSpNT)), unsigned(batch));
     if (prec == CIR_PREC_FP64)
@@ -342,9 +446,15 @@ int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, f
     CIR_CHECK_LAUNCH();
     const double gamma = kHbar / (double(t1) * 2.0);  // calcIR.cu:1426
     dim3 g2(unsigned(ceil_div(nomega, kSpNT)), unsigned(batch));
+    double* per_frame = static_cast<double*>(det);
     hipLaunchKernelGGL(sw_kernel, g2, dim3(kSpNT), 0, s, n, nomega, omegaStart, omegaStep, gamma, avef, m2work,
-                       w, weight, sw);
+                       w, weight, sw, per_frame);
     CIR_CHECK_LAUNCH();
+    if (per_frame) {
+        hipLaunchKernelGGL(sw_frames_kernel, dim3(unsigned(ceil_div(nomega, kSpNT))), dim3(kSpNT), 0, s, batch, nomega,
+                           per_frame, sw);
+        CIR_CHECK_LAUNCH();
+    }
     return CIR_OK;
 }
 
@@ -533,7 +643,7 @@ static int prop_products(bool split3, int n, int ld, int ncols, int ldy, double 
     const int tiles = nmt_pad * int(ceil_div(ncols, 16));
     const int guess = std::max(1, std::min({kPropMaxSplit, int(ceil_div(1024, tiles)), std::max(1, n / 256)}));
     const TuneKey key = prop_key(n, ncols, split3);
-    int ks = guess;
+    int ks = deterministic_mode() ? 1 : guess;  // one K segment: no atomic accumulation
     const int kept = tune_lookup(key);
     if (kept >= 0) {
         ks = 1 << kept;
@@ -774,13 +884,13 @@ constexpr int kGramRows = 512;  // rows per workgroup of the Gram sums
 // G[s] += sum over this workgroup's rows of conj(y_a) y_b, thread (a, b)
 template <typename T>
 __global__ void __launch_bounds__(128)
-gram_kernel(int n, int nslot, int ldy, const int* __restrict__ mode, const T* __restrict__ Y,
+gram_kernel(int n, int nslot, int ldy, int rpb, const int* __restrict__ mode, const T* __restrict__ Y,
             double* __restrict__ G) {
     const int s = blockIdx.y, t = threadIdx.x;
     if (mode[s] == 0 || t >= 81) return;
     const int a = t / 9, b = t % 9, ncols = 9 * nslot;
     const int ca = 9 * s + a, cb = 9 * s + b;
-    const int i0 = blockIdx.x * kGramRows, i1 = min(n, i0 + kGramRows);
+    const int i0 = blockIdx.x * rpb, i1 = min(n, i0 + rpb);
     double re = 0.0, im = 0.0;
     for (int i = i0; i < i1; ++i) {
         const T* y = Y + size_t(i) * ldy;
@@ -885,13 +995,16 @@ int reunitarise(int prec, int n, int nslot, int ldy, const int* mode, double* G0
     const int64_t ng = int64_t(nslot) * 162;
     hipLaunchKernelGGL(zero_doubles_kernel, dim3(unsigned(ceil_div(ng, 256))), dim3(256), 0, s, ng, G);
     CIR_CHECK_LAUNCH();
-    const dim3 gg(unsigned(ceil_div(n, kGramRows)), unsigned(nslot));
+    // rows per workgroup: all of them in deterministic mode (one adder per
+    // entry, so the atomic sums have a fixed order)
+    const int rpb = deterministic_mode() ? n : kGramRows;
+    const dim3 gg(unsigned(ceil_div(n, rpb)), unsigned(nslot));
     if (prec == CIR_PREC_FP64)
-        hipLaunchKernelGGL(gram_kernel<double>, gg, dim3(128), 0, s, n, nslot, ldy, mode,
+        hipLaunchKernelGGL(gram_kernel<double>, gg, dim3(128), 0, s, n, nslot, ldy, rpb, mode,
                            static_cast<const double*>(Y), G);
     else
-        hipLaunchKernelGGL(gram_kernel<float>, gg, dim3(128), 0, s, n, nslot, ldy, mode, static_cast<const float*>(Y),
-                           G);
+        hipLaunchKernelGGL(gram_kernel<float>, gg, dim3(128), 0, s, n, nslot, ldy, rpb, mode,
+                           static_cast<const float*>(Y), G);
     CIR_CHECK_LAUNCH();
     hipLaunchKernelGGL(reunit_coef_kernel, dim3(unsigned(ceil_div(nslot, 64))), dim3(64), 0, s, nslot, mode, G, G0, C);
     CIR_CHECK_LAUNCH();

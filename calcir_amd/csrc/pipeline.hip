@@ -293,6 +293,7 @@ int job_key(const cir_ctx* c, const Schedule& S, int nframes, const float* xyz_d
     f.val(p.oh_per_molecule);
     f.val(p.msite_bisector);
     f.val(p.time_prec);
+    f.val(p.deterministic);
     f.val(c->mc);
     f.val(S.nslot);
     f.val(int64_t(S.uniq.size()));
@@ -375,6 +376,7 @@ struct BatchRun {
     // re-unitarisation: mode per (distinct frame, slot): 1 at an origin
     // (record G0), 2 every reunit_every-th propagated point of a sample
     DevBuf rmode, rg0, rwork;
+    DevBuf det;  // fixed-order sums of the histograms and the spectral density (deterministic key)
     std::vector<char> rany;
     void* Ycur = nullptr;
     void* Ynext = nullptr;
@@ -408,6 +410,8 @@ struct BatchRun {
         CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
         CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
         CIR_TRY(skip.alloc(sizeof(long long)));
+        if (p.deterministic)
+            CIR_TRY(det.alloc(std::max(ipr_hist_det_bytes(B, N), spectral_density_det_bytes(B, nom))));
         R = p.reunit_every;
         if (R > 0) {
             std::vector<int> hmode(size_t(nu) * nslot, 0);
@@ -468,12 +472,13 @@ struct BatchRun {
             ProfScope ps(pf, CIR_PH_IPR, st);
             CIR_TRY(ipr_hist(rp, nb, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p, w.as<double>(),
                              wmult.as<double>() + c0, accp + nom, accp + 2 * nom, accp + 3 * nom,
-                             skip.as<long long>(), st));
+                             skip.as<long long>(), st, det.p));
         }
         {
             ProfScope ps(pf, CIR_PH_SW, st);
             CIR_TRY(spectral_density(rp, nb, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
-                                     w.as<double>(), sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st));
+                                     w.as<double>(), sites.p, worig.as<double>() + c0, accp, m2.as<double>(), st,
+                                     det.p));
         }
         // frame by frame: propagation, origin bookkeeping, re-unitarisation,
         // correlations
@@ -677,7 +682,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
     const int rp = c->real_prec;
     hipStream_t st = c->stream;
     const int ldf = 2 * N;
-    DevBuf xyz1, box1, sites1, sites0, H, V, w, work, m2, F, G, Y, out, acc, skip;
+    DevBuf xyz1, box1, sites1, sites0, H, V, w, work, m2, F, G, Y, out, acc, skip, det;
     CIR_TRY(xyz1.alloc(size_t(c->natoms) * 3 * sizeof(float)));
     CIR_TRY(box1.alloc(box_floats(c) * sizeof(float)));
     CIR_TRY(sites1.alloc(size_t(CIR_NSITE) * N * esz));
@@ -693,6 +698,7 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
     CIR_TRY(out.alloc(size_t(ns) * nt * 6 * sizeof(double)));
     CIR_TRY(acc.alloc(4 * size_t(nom) * sizeof(double)));
     CIR_TRY(skip.alloc(sizeof(long long)));
+    if (p.deterministic) CIR_TRY(det.alloc(std::max(ipr_hist_det_bytes(1, N), spectral_density_det_bytes(1, nom))));
     CIR_HIP(hipMemsetAsync(out.p, 0, out.bytes, st));
     CIR_HIP(hipMemsetAsync(acc.p, 0, acc.bytes, st));
     CIR_HIP(hipMemsetAsync(skip.p, 0, skip.bytes, st));
@@ -748,12 +754,12 @@ int run_plain(cir_ctx* c, int nframes, const float* xyz_d, const float* box_d, c
                 ProfScope ps(pf, CIR_PH_IPR, st);
                 CIR_TRY(ipr_hist(rp, 1, N, ld, p.nchrom_mol, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
                                  w.as<double>(), nullptr, accp + nom, accp + 2 * nom, accp + 3 * nom,
-                                 skip.as<long long>(), st));
+                                 skip.as<long long>(), st, det.p));
             }
             if (k == 0) {
                 ProfScope ps(pf, CIR_PH_SW, st);
                 CIR_TRY(spectral_density(rp, 1, N, ld, p.t1, p.avef, p.omegaStart, p.omegaStep, nom, V.p,
-                                         w.as<double>(), sites1.p, nullptr, accp, m2.as<double>(), st));
+                                         w.as<double>(), sites1.p, nullptr, accp, m2.as<double>(), st, det.p));
                 CIR_TRY(set_identity(rp, N, ldf, F.p, st));
                 CIR_HIP(hipMemcpyAsync(sites0.p, sites1.p, sites1.bytes, hipMemcpyDeviceToDevice, st));
             } else {
@@ -847,6 +853,16 @@ extern "C" int cir_create(const cir_params* p, int natoms, void* stream, cir_ctx
     c->prop_prec = p->precision;
     c->esz = (c->real_prec == CIR_PREC_FP64) ? sizeof(double) : sizeof(float);
     c->stream = as_stream(stream);
+    if (p->deterministic != 0 && p->deterministic != 1) {
+        delete c;
+        return fail(CIR_ERR_ARG, "deterministic must be 0 or 1");
+    }
+    if (p->deterministic && (c->real_prec == CIR_PREC_FP64 || !deterministic_eigh_size(c->N))) {
+        const int n = c->N;
+        delete c;
+        return fail(CIR_ERR_ARG, "deterministic needs precision fp32 or bf16x3 and 66 <= N <= " +
+                                     std::to_string(deterministic_eigh_max()) + " (N = " + std::to_string(n) + ")");
+    }
     *out = c;
     return CIR_OK;
 } catch (...) {
@@ -873,6 +889,7 @@ extern "C" int cir_run_device(cir_ctx* c, int nframes, const float* xyz_d, const
     }
     c->have = false;
     reset_phases(c);
+    DeterministicScope det(c->p.deterministic != 0);
     EventPair ev;
     CIR_HIP(ev.create());
     CIR_HIP(hipEventRecord(ev.a, c->stream));
@@ -897,6 +914,7 @@ extern "C" int cir_stream_begin(cir_ctx* c, int nframes, const float* times) try
                     "cir_stream_begin: checkpoint_every, resume and max_frames need the whole trajectory (cir_run)");
     c->have = false;
     reset_phases(c);
+    DeterministicScope det(c->p.deterministic != 0);
     std::unique_ptr<StreamRun, StreamDeleter> s(new StreamRun());
     CIR_HIP(hipEventCreate(&s->e0));
     CIR_HIP(hipEventRecord(s->e0, c->stream));
@@ -922,6 +940,7 @@ int stream_push(cir_ctx* c, int first, int count, const float* xyz, const float*
     for (size_t i = 0; i < fx * count; ++i)
         if (!std::isfinite(xyz[i])) return fail(CIR_ERR_NONFINITE, "non-finite coordinate in the trajectory");
     CIR_TRY(check_boxes(c, count, box));
+    DeterministicScope det(c->p.deterministic != 0);
     BatchRun& r = s->run;
     int st = CIR_OK;
     for (int k = 0; k < count && st == CIR_OK; ++k) {

@@ -26,12 +26,19 @@ size_t eigh_workspace(int prec, int batch, int n, int ld);
 int eigh_batched(int prec, int batch, int n, int ld, void* A, void* V, double* w, void* work, hipStream_t s,
                  Prof* prof = nullptr);
 
+// det (optional, deterministic sums): ipr_hist_det_bytes(batch, n) /
+// spectral_density_det_bytes(batch, nomega) of scratch; the sums then run in
+// a fixed order (per bin, frames in order, a frame's modes in order: its
+// eigenvalues ascend, so the modes of one bin are contiguous) instead of by
+// float atomics.
+size_t ipr_hist_det_bytes(int batch, int n);
+size_t spectral_density_det_bytes(int batch, int nomega);
 int ipr_hist(int prec, int batch, int n, int ld, int nchrom_mol, float avef, float omegaStart, int omegaStep,
              int nomega, const void* V, const double* w, const double* weight, double* pw, double* rw,
-             double* rmw, long long* skipped, hipStream_t s);
+             double* rmw, long long* skipped, hipStream_t s, void* det = nullptr);
 int spectral_density(int prec, int batch, int n, int ld, float t1, float avef, float omegaStart, int omegaStep,
                      int nomega, const void* V, const double* w, const void* sites, const double* weight,
-                     double* sw, double* m2work, hipStream_t s);
+                     double* sw, double* m2work, hipStream_t s, void* det = nullptr);
 // Yout = V exp(+i diag(w) dt/hbar) V^T Yin, Y row-major n x ldy with Re in
 // columns [0, ncols) and Im in [ncols, 2 ncols); G: propagate_work_elems()
 // reals of work.  Yout may alias Yin.  may_tune: the first call of a shape may

@@ -29,13 +29,27 @@ std::map<TuneKey, Entry>& tune_map() {
 
 }  // namespace
 
+namespace {
+thread_local bool t_det = false;
+}
+
+bool deterministic_mode() {
+    if (t_det) return true;
+    const char* e = std::getenv("CIR_DETERMINISTIC");
+    return e && std::atoi(e) == 1;
+}
+DeterministicScope::DeterministicScope(bool on) : prev(t_det) { t_det = prev || on; }
+DeterministicScope::~DeterministicScope() { t_det = prev; }
+
 int tune_lookup(const TuneKey& key) {
+    if (deterministic_mode()) return -1;
     std::lock_guard<std::mutex> lk(tune_mu());
     auto it = tune_map().find(key);
     return it == tune_map().end() ? -1 : it->second.choice;
 }
 
 bool tune_allowed(hipStream_t s) {
+    if (deterministic_mode()) return false;
     const char* e = std::getenv("CIR_NO_TUNE");
     if (e && std::atoi(e) == 1) return false;
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;

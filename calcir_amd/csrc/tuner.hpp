@@ -46,4 +46,18 @@ int tune_time(const TuneKey& key, int ncand, hipStream_t s, const std::function<
 // candidates' times in ms.
 std::string tune_report();
 
+// Deterministic mode of the calling thread (a run with the deterministic key
+// on, or CIR_DETERMINISTIC=1 in the environment): launch shapes are the fixed
+// defaults (no timing, no kept choices), the eigensolver takes its two-stage
+// form, the propagation one K segment, and the reductions a fixed order.
+bool deterministic_mode();
+struct DeterministicScope {
+    bool prev;
+    explicit DeterministicScope(bool on);
+    ~DeterministicScope();
+};
+// Sizes the two-stage eigensolver takes (the deterministic key's range).
+bool deterministic_eigh_size(int n);
+int deterministic_eigh_max();
+
 }  // namespace cir

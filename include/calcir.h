@@ -154,6 +154,12 @@ typedef struct cir_params {
                                   the XTC file's own coordinate precision).  Default 1000
                                   (GROMACS' default); the CLI sets it from the file
                                   (cir_xtc_precision)                                   */
+    int   deterministic;       /* key "deterministic": 1 = results bitwise reproducible
+                                  run to run (no float atomics, no timed launch shapes:
+                                  two-stage eigensolver, one K segment in the
+                                  propagation, histograms and densities summed in a
+                                  fixed order); needs precision fp32 or bf16x3 and
+                                  66 <= N <= 2080.  Default 0                           */
 } cir_params;
 
 /* Results of a run, all arrays caller-allocated (host).  NULL members are

@@ -568,3 +568,30 @@ def test_graph_frame_loops_equal_direct_launches(prec, tol, monkeypatch):
         d = C.run(p, xyz, box, times)
         for k in ("tcf_ir", "ftcf_ir", "sw"):
             assert rel(d[k], ref[k]) <= tol, (k, rel(d[k], ref[k]))
+
+
+@pytest.mark.parametrize("prec,mode", [("bf16x3", "batched"), ("fp32", "plain")])
+def test_deterministic_runs_are_bitwise_equal(prec, mode):
+    # deterministic 1: two-stage eigensolver (no float atomics), one K segment
+    # in the propagation, histograms/density summed in a fixed order, default
+    # launch shapes; two runs give the same bits, and agree with the default
+    # run to the FP32 bar
+    xyz, box, times = frames(128, 60, 1001)
+    kw = dict(KW, chunk=16, reunitarise_every=4) if mode == "batched" else dict(KW, nsamples=1, ntcfpoints=8)
+    p = C.default_params(precision=prec, mode=mode, deterministic=1, **kw)
+    announce("deterministic %s %s: two runs" % (prec, mode))
+    a = C.run(p, xyz, box, times)
+    b = C.run(p, xyz, box, times)
+    for k in ("tcf_ir", "tcf_vv", "tcf_vh", "ftcf_ir", "ftcf_vv", "ftcf_vh", "sw", "pw", "rw", "rmw"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    d = C.run(C.default_params(precision=prec, mode=mode, **kw), xyz, box, times)
+    for k in ("tcf_ir", "ftcf_ir", "sw"):
+        assert rel(a[k], d[k]) < 1e-4, (k, rel(a[k], d[k]))
+
+
+def test_deterministic_needs_fp32_and_the_two_stage_sizes():
+    xyz, box, times = frames(16, 4, 1)  # N = 32 < 66
+    for prec in ("fp64", "fp32"):
+        p = C.default_params(precision=prec, deterministic=1, **dict(KW, nsamples=1, ntcfpoints=2))
+        with pytest.raises(C.CalcirError, match="deterministic needs"):
+            C.run(p, xyz, box, times)

@@ -243,10 +243,13 @@ def test_cli_usage_and_input_errors(tmp_path):
 
 def test_rule_and_precision_keys_parse(tmp_path):
     f = tmp_path / "input.inp"
-    f.write_text("nchrom_mol 1\nnatom_mol 3\noh_per_molecule 1\nmsite_bisector 1\ntime_precision 10000\n")
+    f.write_text("nchrom_mol 1\nnatom_mol 3\noh_per_molecule 1\nmsite_bisector 1\ntime_precision 10000\n"
+                 "deterministic 1\n")
     p = C.read_params(str(f))
     assert (p.nchrom_mol, p.natom_mol, p.oh_per_molecule, p.msite_bisector) == (1, 3, 1, 1)
     assert p.time_prec == 10000.0
+    assert p.deterministic == 1
+    assert C.default_params().deterministic == 0
 
 
 def test_trajectory_shapes_are_checked_before_the_library_reads_them():
