@@ -40,10 +40,11 @@ int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P, 
 int sb2st(int batch, int n, float* BND, float* d, float* e, float* Q2, hipStream_t s);
 // The blocks of Q2 into compact-WY form (T factors, operand order of
 // q2_apply), in place.
-int q2_prep(int batch, int n, float* Q2, hipStream_t s);
+int q2_prep(int batch, int n, float* Q2, hipStream_t s, bool bf16);
 // Z <- Q2 Z for the eigenvectors Z of the tridiagonal matrix (row-major,
 // Z[i * ld + k] = component i of vector k), batch stride n * ld; Q2 as
 // q2_prep leaves it.
-int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s);
+// bf16: the blocks' products as split-BF16x3 MFMA (q2_prep with the same flag).
+int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s, bool bf16);
 
 }  // namespace cir

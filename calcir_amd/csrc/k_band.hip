@@ -866,6 +866,24 @@ int sb2st(int batch, int n, float* BND, float* d, float* e, float* Q2, hipStream
 namespace cir {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Four FP32 values as BF16 hi = bf16(x) and lo = bf16(x - hi) (round to
+// nearest even), packed as the 16x16x16 BF16 MFMA operand.
+__device__ __forceinline__ void split_bf16(float a, float b, float c, float d, shortx4& hi, shortx4& lo) {
+    bf16x4 h, l;
+    h[0] = __bf16(a);
+    h[1] = __bf16(b);
+    h[2] = __bf16(c);
+    h[3] = __bf16(d);
+    l[0] = __bf16(a - float(h[0]));
+    l[1] = __bf16(b - float(h[1]));
+    l[2] = __bf16(c - float(h[2]));
+    l[3] = __bf16(d - float(h[3]));
+    hi = __builtin_bit_cast(shortx4, h);
+    lo = __builtin_bit_cast(shortx4, l);
+}
 
 // ---------------------------------------------------------------------------
 // Q2: the bulge-chasing reflectors as compact-WY blocks.
@@ -891,7 +909,7 @@ __device__ __forceinline__ void q2_block_gk(int n, int64_t q, int* g, int* k) {
 // apply the block (Z <- Z + (-V T) (V^T Z)): lane l's 12 values of each
 // contiguous, V_op[l][s] = V(4 s + l/16, l%16) and NVT_op[l][4 rt + s] =
 // (-V T)(16 rt + l%16, 4 s + l/16).  V_op replaces V in place.
-__global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float* __restrict__ Q2all) {
+__global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float* __restrict__ Q2all, int bf16) {
     __shared__ float sv[kQ2R][kQ2G + 1], st[kQ2G][kQ2G + 1], snv[kQ2R][kQ2G + 1], sg[kQ2G], stau[kQ2G];
     const int64_t q = blockIdx.x;
     const int b = blockIdx.y, lane = threadIdx.x;
@@ -939,10 +957,14 @@ __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float*
     // l16) holds V(16 q + 4 lq + e, l16)
 #pragma unroll
     for (int s = 0; s < kQ2R / 4; ++s) V[lane * (kQ2R / 4) + s] = sv[16 * (s >> 2) + 4 * lq + (s & 3)][l16];
+    // -V T as the A operand of the update, register 4 rt + s of lane (lq,
+    // l16): FP32 16x16x4 steps take NV(16 rt + l16, 4 s + lq), split-BF16
+    // 16x16x16 products NV(16 rt + l16, 4 lq + s)
 #pragma unroll
     for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) NVT[lane * 12 + 4 * rt + s] = snv[16 * rt + l16][4 * s + lq];
+        for (int s = 0; s < 4; ++s)
+            NVT[lane * 12 + 4 * rt + s] = bf16 ? snv[16 * rt + l16][4 * lq + s] : snv[16 * rt + l16][4 * s + lq];
     if (lane < kQ2G) tau[lane] = stau[lane];
 }
 
@@ -967,7 +989,7 @@ __host__ __device__ inline int q2_pitch(int n) {
     const int p = n + kQ2R + 3;
     return p + ((4 - p) & 63);
 }
-template <int kQ2NT>
+template <int kQ2NT, bool BF>
 __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const float* __restrict__ Q2all,
                                                           float* __restrict__ Zall, int64_t nblk) {
     extern __shared__ __align__(16) float q2_smem[];
@@ -1080,6 +1102,35 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
             floatx4 zq[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) zq[q] = *reinterpret_cast<const floatx4*>(zc + 16 * q + 4 * lq);
+            floatx4 acc[3];
+            if constexpr (BF) {
+                // split-BF16x3 (x = hi + lo, products lo*hi + hi*lo + hi*hi
+                // accumulated in FP32) on 16x16x16 BF16 MFMA: lane (lq, l16)
+                // holds k = 4 lq .. 4 lq + 3 of its row (A) or column (B), so
+                // W = V^T Zb comes out as the update's B operand, W(4 lq + r,
+                // l16) in register r, with no transpose
+                floatx4 w = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    shortx4 vh, vl, zh, zl;
+                    split_bf16(vs[4 * q], vs[4 * q + 1], vs[4 * q + 2], vs[4 * q + 3], vh, vl);
+                    split_bf16(zq[q][0], zq[q][1], zq[q][2], zq[q][3], zh, zl);
+                    w = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(vl, zh, w, 0, 0, 0);
+                    w = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(vh, zl, w, 0, 0, 0);
+                    w = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(vh, zh, w, 0, 0, 0);
+                }
+                shortx4 wh, wl;
+                split_bf16(w[0], w[1], w[2], w[3], wh, wl);
+#pragma unroll
+                for (int rt = 0; rt < 3; ++rt) {
+                    acc[rt] = *reinterpret_cast<const floatx4*>(zc + 16 * rt + 4 * lq);
+                    shortx4 nh, nl;
+                    split_bf16(ns[4 * rt], ns[4 * rt + 1], ns[4 * rt + 2], ns[4 * rt + 3], nh, nl);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nl, wh, acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nh, wl, acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nh, wh, acc[rt], 0, 0, 0);
+                }
+            } else {
             floatx4 w0 = {0.f, 0.f, 0.f, 0.f}, w1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
@@ -1106,7 +1157,6 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
                 wb[2] = __uint_as_float(q1[0]);
                 wb[3] = __uint_as_float(q1[1]);
             }
-            floatx4 acc[3];
 #pragma unroll
             for (int rt = 0; rt < 3; ++rt) acc[rt] = *reinterpret_cast<const floatx4*>(zc + 16 * rt + 4 * lq);
 #pragma unroll
@@ -1114,6 +1164,7 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
 #pragma unroll
                 for (int rt = 0; rt < 3; ++rt)
                     acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ns[4 * rt + s], wb[s], acc[rt], 0, 0, 0);
+            }
 #pragma unroll
             for (int rt = 0; rt < 3; ++rt) *reinterpret_cast<floatx4*>(zc + 16 * rt + 4 * lq) = acc[rt];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -1135,11 +1186,11 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
     }
 }
 
-int q2_prep(int batch, int n, float* Q2, hipStream_t s) {
+int q2_prep(int batch, int n, float* Q2, hipStream_t s, bool bf16) {
     const int64_t nblk = q2_blocks(n);
     if (nblk <= 0) return CIR_OK;
     dim3 grid(static_cast<unsigned>(nblk), static_cast<unsigned>(batch));
-    hipLaunchKernelGGL(q2_prep_kernel, grid, dim3(64), 0, s, n, nblk, Q2);
+    hipLaunchKernelGGL(q2_prep_kernel, grid, dim3(64), 0, s, n, nblk, Q2, bf16 ? 1 : 0);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
@@ -1147,26 +1198,30 @@ int q2_prep(int batch, int n, float* Q2, hipStream_t s) {
 // Workgroup width of q2_apply_kernel: 1024 threads (16 waves, candidate 0)
 // or 512, timed per size bucket on the first call (tuner.hpp) over a copy of
 // the first few matrices' eigenvectors (the kernel works in place).
-template <int NT>
+template <int NT, bool BF>
 static int q2_launch(int batch, int n, int ld, const float* Q2, float* Z, int64_t nblk, hipStream_t s) {
     const size_t sm = size_t(q2_pitch(n)) * kQ2S * sizeof(float);
-    CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel<NT>),
+    CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel<NT, BF>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
     dim3 grid(unsigned(ceil_div(n, kQ2S)), unsigned(batch));
-    hipLaunchKernelGGL(q2_apply_kernel<NT>, grid, dim3(NT), sm, s, n, ld, Q2, Z, nblk);
+    hipLaunchKernelGGL((q2_apply_kernel<NT, BF>), grid, dim3(NT), sm, s, n, ld, Q2, Z, nblk);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }
 
-int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s) {
+int q2_apply(int batch, int n, int ld, const float* Q2, float* Z, hipStream_t s, bool bf16) {
     const int64_t nblk = q2_blocks(n);
     if (nblk <= 0) return CIR_OK;
     const size_t sm = size_t(q2_pitch(n)) * kQ2S * sizeof(float);
     if (sm > 160 * 1024) return fail(CIR_ERR_ARG, "q2_apply: n too large for the LDS slab");
     auto launch = [&](int cand, int nb, float* Zp) {
-        return cand == 0 ? q2_launch<1024>(nb, n, ld, Q2, Zp, nblk, s) : q2_launch<512>(nb, n, ld, Q2, Zp, nblk, s);
+        if (bf16)
+            return cand == 0 ? q2_launch<1024, true>(nb, n, ld, Q2, Zp, nblk, s)
+                             : q2_launch<512, true>(nb, n, ld, Q2, Zp, nblk, s);
+        return cand == 0 ? q2_launch<1024, false>(nb, n, ld, Q2, Zp, nblk, s)
+                         : q2_launch<512, false>(nb, n, ld, Q2, Zp, nblk, s);
     };
-    const TuneKey key{"q2_apply_threads", {size_bucket(n), 0, 0, 0, 0}};
+    const TuneKey key{"q2_apply_threads", {size_bucket(n), bf16 ? 1 : 0, 0, 0, 0}};
     int cand = tune_lookup(key);
     if (cand < 0 && tune_allowed(s)) {
         const int nb = std::min(batch, 8);

@@ -1646,7 +1646,7 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
             }
             ProfScope ps(prof, CIR_PH_BULGE, s);
             CIR_TRY(sb2st(batch, n, bnd, dd, ee, q2, s));
-            CIR_TRY(q2_prep(batch, n, q2, s));
+            CIR_TRY(q2_prep(batch, n, q2, s, gemm_prec == CIR_PREC_BF16X3));
         }
     }
     if (L.two) {
@@ -1695,7 +1695,8 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
     if constexpr (sizeof(T) == sizeof(float)) {
         if (L.two) {
             ProfScope ps(prof, CIR_PH_BULGE_BT, s);
-            CIR_TRY(q2_apply(batch, n, ld, reinterpret_cast<const float*>(work + L.off_q2), V, s));
+            CIR_TRY(q2_apply(batch, n, ld, reinterpret_cast<const float*>(work + L.off_q2), V, s,
+                             gemm_prec == CIR_PREC_BF16X3));
         }
     }
     ProfScope pbt(prof, CIR_PH_BACKTRANSFORM, s);

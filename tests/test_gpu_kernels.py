@@ -508,7 +508,7 @@ def test_tuner_times_each_size_bucket_once(monkeypatch):
         w, V = S.eigh(torch.from_numpy(Hd).to(DEV).contiguous(), n)
         out[tune] = (w.cpu().numpy(), V.cpu().double().numpy())
     rep = C.tuning_report()
-    q2 = [v for k, v in rep.items() if k[0] == "q2_apply_threads" and k[1] == 2048]
+    q2 = [v for k, v in rep.items() if k[0] == "q2_apply_threads" and k[1] == 2048 and k[2] == 0]  # FP32 blocks
     assert len(q2) == 1 and len(q2[0][1]) == 2 and all(t > 0 for t in q2[0][1]), rep
     assert any(k[0] == "gemm" for k in rep), rep
     for tune, (w, V) in out.items():
