@@ -1066,15 +1066,16 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
     // step order, so every wait ends.
     int t = 0, gi = first_of(0);
     if (gi < 0) advance(t, gi);
-    float4 va[3], na[3];
-    load_ops(gi < 0 ? 0 : t, gi < 0 ? 0 : gi, va, na);  // (0, 0) is a block of every matrix
-    while (t < nstep) {
+    // one block from operands (va, na) while the next block's are read into
+    // (vn, nn); the two buffer pairs alternate, so no copy waits for a read
+    // in flight
+    auto block = [&](const float4 (&va)[3], const float4 (&na)[3], float4 (&vn)[3], float4 (&nn)[3]) {
         {
             int t2 = t, g2 = gi;
             advance(t2, g2);
             // the next block's operands (this block's again after the last)
-            float4 vn[3], nn[3];
             load_ops(t2 < nstep ? t2 : t, t2 < nstep ? g2 : gi, vn, nn);
+            __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the block
             const int k = t - gi;
             if (gi > 0) {
                 // (bounded: a wait that ran far past any block's time marks
@@ -1171,12 +1172,14 @@ __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const fl
             if (lane == 0) __hip_atomic_store(&done[gi], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             t = t2;
             gi = g2;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                va[i] = vn[i];
-                na[i] = nn[i];
-            }
         }
+    };
+    float4 va[3], na[3], vb[3], nb[3];
+    load_ops(gi < 0 ? 0 : t, gi < 0 ? 0 : gi, va, na);  // (0, 0) is a block of every matrix
+    while (t < nstep) {
+        block(va, na, vb, nb);
+        if (t >= nstep) break;
+        block(vb, nb, va, na);
     }
     __syncthreads();
     const bool bad = stuck != 0;
