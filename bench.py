@@ -86,12 +86,14 @@ def traffic_from(fetch_db, write_db, frames, kernel="q2_apply_kernel"):
         rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
                            "where counter_name = ? group by dispatch_id, kernel_name", (counter,)).fetchall()
         con.close()
+        # The largest dispatch of the kernel is the run's one chunk; smaller
+        # ones time its launch shapes over a few matrices on first use.
         k = [v for _, name, v in rows if kernel in name]
         cp = [v for _, name, v in rows if "copyBuffer" in name]
-        if len(k) != 1 or not cp:
-            raise SystemExit("%s: expected one %s dispatch and the calibration copy, found %d and %d"
+        if not k or not cp:
+            raise SystemExit("%s: expected a %s dispatch and the calibration copy, found %d and %d"
                              % (db, kernel, len(k), len(cp)))
-        return float(k[0]), float(max(cp))
+        return float(max(k)), float(max(cp))
 
     nmat = 150 + (nsamples_for(frames) - 1) * 20
     fk, fc = per_dispatch(fetch_db, "FETCH_SIZE")

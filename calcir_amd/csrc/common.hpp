@@ -137,6 +137,10 @@ struct ProfScope {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// XCDs of an MI355X: a launch's workgroups are dispatched to them in turn
+// (workgroup i to XCD i mod 8), each XCD with its own L2.
+constexpr int kXcds = 8;
+
 }  // namespace cir
 
 // Device-side helpers (HIP translation units only) ----------------------------

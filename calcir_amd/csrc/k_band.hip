@@ -31,6 +31,7 @@
 
 #include "band.hpp"
 #include "gemm.hpp"
+#include "split_bf16.hpp"
 #include "tuner.hpp"
 
 namespace cir {
@@ -866,24 +867,6 @@ int sb2st(int batch, int n, float* BND, float* d, float* e, float* Q2, hipStream
 namespace cir {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef short shortx4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-
-// Four FP32 values as BF16 hi = bf16(x) and lo = bf16(x - hi) (round to
-// nearest even), packed as the 16x16x16 BF16 MFMA operand.
-__device__ __forceinline__ void split_bf16(float a, float b, float c, float d, shortx4& hi, shortx4& lo) {
-    bf16x4 h, l;
-    h[0] = __bf16(a);
-    h[1] = __bf16(b);
-    h[2] = __bf16(c);
-    h[3] = __bf16(d);
-    l[0] = __bf16(a - float(h[0]));
-    l[1] = __bf16(b - float(h[1]));
-    l[2] = __bf16(c - float(h[2]));
-    l[3] = __bf16(d - float(h[3]));
-    hi = __builtin_bit_cast(shortx4, h);
-    lo = __builtin_bit_cast(shortx4, l);
-}
 
 // ---------------------------------------------------------------------------
 // Q2: the bulge-chasing reflectors as compact-WY blocks.
@@ -991,10 +974,18 @@ __host__ __device__ inline int q2_pitch(int n) {
 }
 template <int kQ2NT, bool BF>
 __global__ void __launch_bounds__(kQ2NT) q2_apply_kernel(int n, int ld, const float* __restrict__ Q2all,
-                                                          float* __restrict__ Zall, int64_t nblk) {
+                                                          float* __restrict__ Zall, int64_t nblk, int nmat) {
     extern __shared__ __align__(16) float q2_smem[];
     __shared__ int goff[kQ2MaxGroups], done[kQ2MaxGroups], snstep, stuck;
-    const int slab = blockIdx.x, b = blockIdx.y;
+    // Workgroup i runs on XCD i mod 8 (round-robin dispatch); XCD x takes the
+    // x-th eighth of the (matrix, slab) list in order, so the slabs that run
+    // together on one XCD belong to one matrix, start together, and read the
+    // same block operands through that XCD's L2.
+    const int nslab = (n + kQ2S - 1) / kQ2S;
+    const int64_t total = int64_t(nslab) * nmat, per = (total + kXcds - 1) / kXcds;
+    const int64_t pos = int64_t(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (pos >= total) return;
+    const int b = int(pos / nslab), slab = int(pos % nslab);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c0 = slab * kQ2S;
     const int P = q2_pitch(n);
@@ -1206,8 +1197,9 @@ static int q2_launch(int batch, int n, int ld, const float* Q2, float* Z, int64_
     const size_t sm = size_t(q2_pitch(n)) * kQ2S * sizeof(float);
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel<NT, BF>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
-    dim3 grid(unsigned(ceil_div(n, kQ2S)), unsigned(batch));
-    hipLaunchKernelGGL((q2_apply_kernel<NT, BF>), grid, dim3(NT), sm, s, n, ld, Q2, Z, nblk);
+    const int64_t total = int64_t(ceil_div(n, kQ2S)) * batch;
+    dim3 grid(unsigned(ceil_div(total, int64_t(kXcds)) * kXcds));
+    hipLaunchKernelGGL((q2_apply_kernel<NT, BF>), grid, dim3(NT), sm, s, n, ld, Q2, Z, nblk, batch);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }

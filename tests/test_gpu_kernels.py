@@ -44,6 +44,16 @@ def test_gemm_vs_torch(prec, shape, ta, tb):
     assert err < GEMM_TOL[prec], err
 
 
+@pytest.mark.parametrize("shape", [(2, 256, 144, 512), (2, 260, 132, 36), (1, 128, 128, 4), (3, 132, 388, 100)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_bf16x3_default_tile_vs_torch(monkeypatch, shape, ta, tb):
+    # with timing off the first candidate runs: for these shapes (sizes and
+    # strides multiples of 4) the double-buffered kernel with 16-byte operand
+    # reads, over partial tiles in M, N and K and all four operand layouts
+    monkeypatch.setenv("CIR_NO_TUNE", "1")
+    test_gemm_vs_torch(C.PREC_BF16X3, shape, ta, tb)
+
+
 FIELD_CASES = [(1, 0, 0), (0, 3, 1), (1, 1, 0), (0, 2, 1)]
 
 
