@@ -686,9 +686,10 @@ int gemm(int prec, const GemmArgs& g, hipStream_t s) {
     // short-K updates C += A B in BF16x3: 64x64 tiles with the C tile read
     // ahead of the K loop (the 128x128 tile has no registers left for it)
     if (split && g.beta != 0.0 && g.K <= 128) return launch_layout<64, 64, 32, true, true>(g, s);
-    // tile 0: 128x128 (the default where both sides reach 128), 1: 64x64;
-    // BF16x3 products whose operands allow 16-byte reads have a third
-    // candidate, first: the double-buffered 128x128 kernel
+    // tile 0: 128x128 (the default where both sides reach 128), 1: 64x64
+    // (the default for smaller products); BF16x3 products whose operands
+    // allow 16-byte reads have a third candidate, first and the default at
+    // every size: the double-buffered 128x128 kernel
     const bool vec = split && x3v_ok(g);
     auto launch = [&](int tile) {
         if (vec) {
@@ -699,7 +700,7 @@ int gemm(int prec, const GemmArgs& g, hipStream_t s) {
             return tile == 0 ? launch_layout<128, 128, 32, true>(g, s) : launch_layout<64, 64, 32, true>(g, s);
         return tile == 0 ? launch_layout<128, 128, 16, false>(g, s) : launch_layout<64, 64, 16, false>(g, s);
     };
-    if (!big) return launch(vec ? 2 : 1);
+    if (!big) return launch(vec ? 0 : 1);
     // Products that overwrite C (beta = 0) can be run again, so the first of
     // each size bucket times every tile; updates of C take the choice of
     // their bucket's product, or the first candidate.
