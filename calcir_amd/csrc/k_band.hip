@@ -257,6 +257,22 @@ __global__ void __launch_bounds__(kPanNT) sb_panel_kernel(int n, int ld, int j0,
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+// Workgroup x of matrix b in a grid of gx workgroups per matrix over batch
+// matrices, launched one-dimensional (xcd_grid): workgroup i runs on XCD i
+// mod 8 (round-robin dispatch) and XCD x takes the x-th eighth of the
+// (matrix, workgroup) list, so a matrix's workgroups run together on one XCD
+// and share its panel through that XCD's L2.  False for the padding past the
+// list.
+__device__ __forceinline__ bool xcd_slot(int gx, int batch, int& x, int& b) {
+    const int64_t total = int64_t(gx) * batch, per = (total + kXcds - 1) / kXcds;
+    const int64_t pos = int64_t(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (pos >= total) return false;
+    b = int(pos / gx);
+    x = int(pos % gx);
+    return true;
+}
+inline dim3 xcd_grid(int gx, int batch) { return dim3(unsigned(ceil_div(int64_t(gx) * batch, int64_t(kXcds)) * kXcds)); }
+
 // X = A22 Yh on the lower triangle of A22 (m x m, column-major, pitch ld,
 // first row and column r0), X row-major with pitch 2 NB in the W half of the
 // row-major panel P (rows r0.., P[r * 2NB + NB + p]); Yh row-major, pitch NB
@@ -267,13 +283,16 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // wave's LDS tile.  The next tile's loads are in flight while a tile's MFMA
 // chain runs.  32x32x2 FP32 MFMA, K = 32 per tile.
 __global__ void __launch_bounds__(256) sb_symm_kernel(int n, int ld, int r0, const float* __restrict__ Aall,
-                                                      const float* __restrict__ Yhall, float* __restrict__ Pall) {
+                                                      const float* __restrict__ Yhall, float* __restrict__ Pall,
+                                                      int batch) {
     constexpr int NB = kBandNB;
     __shared__ float tl[4][32][33];
-    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int m = n - r0, nbt = (m + 31) / 32;
-    const int I = blockIdx.x * 4 + wave;
+    int x, b;
+    if (!xcd_slot((nbt + 3) / 4, batch, x, b)) return;
+    const int I = x * 4 + wave;
     if (I >= nbt) return;
     const float* A = Aall + size_t(b) * n * ld + size_t(r0) * ld + r0;
     const float* Yh = Yhall + size_t(b) * n * NB;
@@ -361,12 +380,14 @@ constexpr int kBufferWord3 = 0x00020000;
 // next tile's A values and operand are in flight while a tile's MFMA chain
 // runs.
 __global__ void __launch_bounds__(256) sb_syr2k_kernel(int n, int ld, int r0, float* __restrict__ Aall,
-                                                       const float* __restrict__ Pall) {
+                                                       const float* __restrict__ Pall, int batch) {
     constexpr int NB = kBandNB, PW = 2 * kBandNB;
-    const int b = blockIdx.y, lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = n - r0, nbt = (m + 31) / 32;
-    const int br = blockIdx.x * 4 + wave;
+    int x, b;
+    if (!xcd_slot((nbt + 3) / 4, batch, x, b)) return;
+    const int br = x * 4 + wave;
     if (br >= nbt) return;
     float* A = Aall + size_t(b) * n * ld + size_t(r0) * ld + r0;
     const float* P = Pall + size_t(b) * n * PW + size_t(r0) * PW;
@@ -478,9 +499,9 @@ int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P, 
                                P, Yh);
         CIR_CHECK_LAUNCH();
         const int nbt = int(ceil_div(m, 32));
-        const dim3 grid(unsigned(ceil_div(nbt, 4)), unsigned(batch));
+        const dim3 grid = xcd_grid(int(ceil_div(nbt, 4)), batch);
         // X = A22 Yh into the W half of P
-        hipLaunchKernelGGL(sb_symm_kernel, grid, dim3(256), 0, s, n, ld, r0, A, Yh, P);
+        hipLaunchKernelGGL(sb_symm_kernel, grid, dim3(256), 0, s, n, ld, r0, A, Yh, P, batch);
         CIR_CHECK_LAUNCH();
         const int64_t ps = int64_t(n) * 2 * NB;
         GemmArgs g;
@@ -524,7 +545,7 @@ int sy2sb(int batch, int n, int ld, float* A, float* BND, float* tau, float* P, 
         g.beta = 1.0;
         CIR_TRY(gemm(CIR_PREC_FP32, g, s));
         // A22 -= W Y^T + Y W^T (lower triangle)
-        hipLaunchKernelGGL(sb_syr2k_kernel, grid, dim3(256), 0, s, n, ld, r0, A, P);
+        hipLaunchKernelGGL(sb_syr2k_kernel, grid, dim3(256), 0, s, n, ld, r0, A, P, batch);
         CIR_CHECK_LAUNCH();
     }
     // the last diagonal block (n - j0 <= NB + 1 rows) is already banded
