@@ -1606,12 +1606,8 @@ inline EighWork eigh_layout(size_t esz, int batch, int n, int ld) {
     o = al(o + esz * batch * kBtPass * size_t(n));
     L.off_s1 = o;  // single-matrix tridiagonalisation state
     o = al(o + esz * tri1_elems(n));
-    // inverse iteration runs over at most 32 GiB of scratch at a time: one
-    // lane per eigenvalue and a few waves per SIMD at N = 2048 are latency
-    // bound, and 8 GiB (81 matrices at N = 2048) left 2.5 waves per SIMD and
-    // cost 3% more eigensolver time than 32 GiB (325 matrices)
     const size_t per = invit_scratch_bytes(n);
-    const size_t budget = size_t(32) << 30;
+    const size_t budget = size_t(8) << 30;
     int sub = int(std::max<size_t>(1, budget / per));
     L.sub = std::min(sub, batch);
     L.off_scr = o;
