@@ -1392,15 +1392,31 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
         if (fabs(a) < pivmin) a = (a < 0.0) ? -pivmin : pivmin;
         const double alast = a;
         double scale = 1.0, big = 0.0, yi = hash_uniform(uint32_t(mb), uint32_t(n - 1));
+        // The solves' loops read each row's factors (and vector) from the
+        // scratch kPf rows ahead, into a ring of kPf register slots: a slot is
+        // read again for the row kPf further on as soon as its row is solved,
+        // so a row's reads are in flight while the kPf - 1 rows before it are
+        // solved.  Each pass is a chain of dependent rows, and at a few waves
+        // per SIMD the read latency of one row at a time set the kernel's
+        // time.  Rows of the last partial ring are solved one at a time.
+        constexpr int kPf = 4;
+        double rr[kPf], rl[kPf], rx[kPf];
+        int rf[kPf];
         for (int it = 0; it < 2; ++it) {
             if (it == 1) {
                 // forward with the kept factorisation: P L^{-1} (scale x)
                 yi = X[0] * scale;
-                for (int i = 0; i < n - 1; ++i) {
-                    const double yn = X[size_t(i + 1) * 64] * scale;
-                    const double l = Lm[size_t(i) * 64];
+                // row i: X(i + 1), L(i), F(i)
+                auto ld = [&](int j, int i) {
+                    i = min(i, n - 2);
+                    rx[j] = X[size_t(i + 1) * 64];
+                    rl[j] = Lm[size_t(i) * 64];
+                    rf[j] = F[size_t(i) * 64];
+                };
+                auto row = [&](int i, double xn, double l, int f) {
+                    const double yn = xn * scale;
                     double yo;
-                    if (F[size_t(i) * 64] == 0) {
+                    if (f == 0) {
                         yo = yi;
                         yi = yn - l * yi;
                     } else {
@@ -1408,21 +1424,40 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                         yi = yi - l * yn;
                     }
                     X[size_t(i) * 64] = yo;
+                };
+                const int nfull = ((n - 1) / kPf) * kPf;  // rows 0 .. nfull-1 through the ring
+                if (nfull > 0) {
+#pragma unroll
+                    for (int j = 0; j < kPf; ++j) ld(j, j);
                 }
+                for (int i0 = 0; i0 < nfull; i0 += kPf) {
+#pragma unroll
+                    for (int j = 0; j < kPf; ++j) {
+                        row(i0 + j, rx[j], rl[j], rf[j]);
+                        ld(j, i0 + j + kPf);
+                    }
+                }
+                for (int i = nfull; i < n - 1; ++i)
+                    row(i, X[size_t(i + 1) * 64], Lm[size_t(i) * 64], F[size_t(i) * 64]);
             }
             // backward: U x = y, u1/u2 rebuilt from T and the flags
             double x1 = yi / alast, x2 = 0.0;
             X[size_t(n - 1) * 64] = x1;
             double nrm2 = x1 * x1;
             big = x1;
-            unsigned char fi = (n > 1) ? F[size_t(n - 2) * 64] : 0;
-            for (int i = n - 2; i >= 0; --i) {
-                unsigned char fp = 0;
-                double lp = 0.0;
-                if (i > 0) {
-                    fp = F[size_t(i - 1) * 64];
-                    lp = Lm[size_t(i - 1) * 64];
-                }
+            int fi = (n > 1) ? F[size_t(n - 2) * 64] : 0;
+            // row i: R(i), L(i - 1), F(i - 1) (rows i > 0), X(i) (second iteration)
+            auto ld = [&](int j, int i) {
+                i = max(i, 0);
+                const int ip = max(i - 1, 0);
+                rr[j] = R[size_t(i) * 64];
+                rl[j] = Lm[size_t(ip) * 64];
+                rf[j] = F[size_t(ip) * 64];
+                rx[j] = (it == 1) ? X[size_t(i) * 64] : 0.0;
+            };
+            auto row = [&](int i, double r, double lpv, int fpv, double xr) {
+                const int fp = (i > 0) ? fpv : 0;
+                const double lp = (i > 0) ? lpv : 0.0;
                 double u1, u2;
                 if (fi) {
                     u1 = sd[i + 1] - lam;
@@ -1431,14 +1466,32 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
                     u1 = (i > 0 && fp) ? -lp * se[i] : se[i];
                     u2 = 0.0;
                 }
-                const double rhs = (it == 0) ? hash_uniform(uint32_t(mb), uint32_t(i)) : X[size_t(i) * 64];
-                const double xi = (rhs - u1 * x1 - u2 * x2) * R[size_t(i) * 64];
+                const double rhs = (it == 0) ? hash_uniform(uint32_t(mb), uint32_t(i)) : xr;
+                const double xi = (rhs - u1 * x1 - u2 * x2) * r;
                 X[size_t(i) * 64] = xi;
                 nrm2 += xi * xi;
                 if (fabs(xi) >= fabs(big)) big = xi;  // first largest component in ascending order
                 x2 = x1;
                 x1 = xi;
                 fi = fp;
+            };
+            // rows n-2 .. n-1-nfull through the ring, then the rest one at a time
+            const int nfull = ((n - 1) / kPf) * kPf;
+            if (nfull > 0) {
+#pragma unroll
+                for (int j = 0; j < kPf; ++j) ld(j, n - 2 - j);
+            }
+            for (int i0 = n - 2; i0 > n - 2 - nfull; i0 -= kPf) {
+#pragma unroll
+                for (int j = 0; j < kPf; ++j) {
+                    row(i0 - j, rr[j], rl[j], rf[j], rx[j]);
+                    ld(j, i0 - j - kPf);
+                }
+            }
+            for (int i = n - 2 - nfull; i >= 0; --i) {
+                const int ip = max(i - 1, 0);
+                row(i, R[size_t(i) * 64], Lm[size_t(ip) * 64], F[size_t(ip) * 64],
+                    (it == 1) ? X[size_t(i) * 64] : 0.0);
             }
             // orthogonalise against the cluster's earlier vectors
             if (mb > c) {
