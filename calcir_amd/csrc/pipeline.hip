@@ -212,18 +212,21 @@ int check_boxes(const cir_ctx* c, int nframes, const float* box) {
 
 int auto_chunk(const cir_ctx* c, int nuniq) {
     if (c->p.chunk > 0) return std::min(c->p.chunk, nuniq);
-    // H and V of every frame of a chunk plus the eigensolver's per-matrix
-    // work space (its share of a 64-matrix batch), within half of the free
-    // device memory (large boxes need large batches: one workgroup per matrix
-    // fills the GPU only with hundreds of matrices)
-    const double per = 2.0 * double(c->N) * c->ld * c->esz +
-                       double(eigh_workspace(c->real_prec, 64, c->N, c->ld)) / 64.0;
+    // H and V of every frame of a chunk plus the eigensolver's work space for
+    // the chunk, within half of the free device memory (large boxes need
+    // large batches: one workgroup per matrix fills the GPU only with
+    // hundreds of matrices).  The work space is counted at the chunk's size:
+    // its inverse-iteration scratch is capped, not per matrix (at N = 8192,
+    // 1.7 GB per matrix, which counted per matrix held the 8192-OH box to
+    // 60 matrices per chunk).
+    const double hv = 2.0 * double(c->N) * c->ld * c->esz;
+    auto need = [&](int B) { return double(B) * hv + double(eigh_workspace(c->real_prec, B, c->N, c->ld)); };
     size_t fr = 0, tot = 0;
     double budget = 24.0 * (1 << 30);
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::max(budget, 0.5 * double(fr));
     budget /= std::max(1, c->mem_share);
-    int B = int(std::max(1.0, budget / per));
-    B = std::max(1, std::min({B, 512, nuniq}));
+    int B = std::max(1, std::min(512, nuniq));
+    while (B > 1 && need(B) > budget) B = std::max(1, std::min(B - 1, int(double(B) * budget / need(B))));
     // chunks of nearly equal size: a short last chunk would leave most CUs
     // idle in the one-workgroup-per-matrix kernels (and at 16 or fewer large
     // matrices take the one-matrix-at-a-time path)
