@@ -308,8 +308,51 @@ slot_fixup_kernel(int n, int nslot, int ldy, const int* __restrict__ action, con
 // vectors ordered mu_x, mu_y, mu_z, a_xx, a_yy, a_zz, a_xy, a_yz, a_zx:
 //   p = 0..2 IR (a,a); 3..5 iiFii (a,a); 6..8 ijFij (a,a);
 //   p = 9..14 iiFjj: (4,3) (5,3) (3,4) (5,4) (3,5) (4,5).
-// One block per slot: the 15 bilinear forms, combined into IR, VV and VH
-// (calcIR.cu:646-648, 736-745) and stored at out[6*target[s] ...].
+// One row's terms of the 15 forms (Re, Im) into acc.
+template <typename T>
+__device__ __forceinline__ void corr_row(const T (&m)[9], const T (&yr)[9], const T (&yi)[9], double (&acc)[30]) {
+#pragma unroll
+    for (int p = 0; p < 15; ++p) {
+        const int pb = (p < 9) ? p : ((p == 9 || p == 14) ? 4 : (p == 10 || p == 12) ? 5 : 3);
+        const int pa = (p < 9) ? p : ((p == 9 || p == 10) ? 3 : (p == 11 || p == 12) ? 4 : 5);
+        acc[2 * p] += double(m[pb] * yr[pa]);
+        acc[2 * p + 1] += double(m[pb] * yi[pa]);
+    }
+}
+
+// The block's sums of acc, combined into IR, VV and VH (calcIR.cu:646-648,
+// 736-745) and stored at o[0..5].
+template <int NT>
+__device__ __forceinline__ void corr_finish(double (&acc)[30], double (*red)[30], double* o) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int p = 0; p < 30; ++p) {
+        const double v = wave_sum(acc[p]);
+        if (lane == 0) red[wave][p] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double t[30];
+        for (int p = 0; p < 30; ++p) {
+            t[p] = 0.0;
+            for (int w = 0; w < NT / 64; ++w) t[p] += red[w][p];
+        }
+        double ir[2] = {0, 0}, iiii[2] = {0, 0}, ijij[2] = {0, 0}, iijj[2] = {0, 0};
+        for (int c = 0; c < 2; ++c) {
+            for (int p = 0; p < 3; ++p) ir[c] += t[2 * p + c];
+            for (int p = 3; p < 6; ++p) iiii[c] += t[2 * p + c];
+            for (int p = 6; p < 9; ++p) ijij[c] += t[2 * p + c];
+            for (int p = 9; p < 15; ++p) iijj[c] += t[2 * p + c];
+        }
+        for (int c = 0; c < 2; ++c) {
+            o[c] = ir[c];
+            o[2 + c] = (3.0 * iiii[c] + iijj[c] + 4.0 * ijij[c]) / 15.0;
+            o[4 + c] = (2.0 * iiii[c] - iijj[c] + 6.0 * ijij[c]) / 30.0;
+        }
+    }
+}
+
+// One block per slot: the 15 bilinear forms, stored at out[6*target[s] ...].
 template <typename T>
 __global__ void __launch_bounds__(kSpNT)
 correlate_kernel(int n, int nslot, int ldy, const int* __restrict__ action, const int* __restrict__ target,
@@ -325,44 +368,69 @@ correlate_kernel(int n, int nslot, int ldy, const int* __restrict__ action, cons
     for (int i = tid; i < n; i += kSpNT) {
         const T* yr = Y + size_t(i) * ldy + 9 * s;
         const T* yi = yr + ncols;
-        T m[9];
+        T m[9], vr[9], vi[9];
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            m[b] = sites[size_t(CIR_SITE_MU + b) * n + i];
+            vr[b] = yr[b];
+            vi[b] = yi[b];
+        }
+        corr_row(m, vr, vi, acc);
+    }
+    corr_finish<kSpNT>(acc, red, out + 6 * size_t(target ? target[s] : 0));
+}
+
+// slot_fixup_kernel and correlate_kernel in one launch, one block per slot:
+// each row's nine columns of the slot are set as the action says (idle: zero,
+// origin: M(t), hold: the old state; propagated: left as the propagation wrote
+// them) and, for an origin or a propagated slot, the forms summed from the
+// same values in the same order as correlate_kernel sums them.
+template <typename T>
+__global__ void __launch_bounds__(kSpNT)
+slot_correlate_kernel(int n, int nslot, int ldy, const int* __restrict__ action, const int* __restrict__ target,
+                      const T* __restrict__ sites, const T* __restrict__ Yold, T* __restrict__ Ynew,
+                      double* __restrict__ out) {
+    __shared__ double red[kSpNT / 64][30];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int act = action[s];
+    const int ncols = 9 * nslot;
+    double acc[30];
+#pragma unroll
+    for (int p = 0; p < 30; ++p) acc[p] = 0.0;
+    for (int i = tid; i < n; i += kSpNT) {
+        T* yr = Ynew + size_t(i) * ldy + 9 * s;
+        T* yi = yr + ncols;
+        if (act == 0 || act == 3) {
+            const T* orr = Yold + size_t(i) * ldy + 9 * s;
+#pragma unroll
+            for (int b = 0; b < 9; ++b) {
+                yr[b] = (act == 3) ? orr[b] : T(0);
+                yi[b] = (act == 3) ? orr[ncols + b] : T(0);
+            }
+            continue;
+        }
+        T m[9], vr[9], vi[9];
 #pragma unroll
         for (int b = 0; b < 9; ++b) m[b] = sites[size_t(CIR_SITE_MU + b) * n + i];
+        if (act == 1) {
 #pragma unroll
-        for (int p = 0; p < 15; ++p) {
-            const int pb = (p < 9) ? p : ((p == 9 || p == 14) ? 4 : (p == 10 || p == 12) ? 5 : 3);
-            const int pa = (p < 9) ? p : ((p == 9 || p == 10) ? 3 : (p == 11 || p == 12) ? 4 : 5);
-            acc[2 * p] += double(m[pb] * yr[pa]);
-            acc[2 * p + 1] += double(m[pb] * yi[pa]);
-        }
-    }
-    const int lane = tid & 63, wave = tid >> 6;
+            for (int b = 0; b < 9; ++b) {
+                vr[b] = m[b];
+                vi[b] = T(0);
+                yr[b] = vr[b];
+                yi[b] = T(0);
+            }
+        } else {
 #pragma unroll
-    for (int p = 0; p < 30; ++p) {
-        const double v = wave_sum(acc[p]);
-        if (lane == 0) red[wave][p] = v;
+            for (int b = 0; b < 9; ++b) {
+                vr[b] = yr[b];
+                vi[b] = yi[b];
+            }
+        }
+        corr_row(m, vr, vi, acc);
     }
-    __syncthreads();
-    if (tid == 0) {
-        double t[30];
-        for (int p = 0; p < 30; ++p) {
-            t[p] = 0.0;
-            for (int w = 0; w < kSpNT / 64; ++w) t[p] += red[w][p];
-        }
-        double ir[2] = {0, 0}, iiii[2] = {0, 0}, ijij[2] = {0, 0}, iijj[2] = {0, 0};
-        for (int c = 0; c < 2; ++c) {
-            for (int p = 0; p < 3; ++p) ir[c] += t[2 * p + c];
-            for (int p = 3; p < 6; ++p) iiii[c] += t[2 * p + c];
-            for (int p = 6; p < 9; ++p) ijij[c] += t[2 * p + c];
-            for (int p = 9; p < 15; ++p) iijj[c] += t[2 * p + c];
-        }
-        double* o = out + 6 * size_t(target ? target[s] : 0);
-        for (int c = 0; c < 2; ++c) {
-            o[c] = ir[c];
-            o[2 + c] = (3.0 * iiii[c] + iijj[c] + 4.0 * ijij[c]) / 15.0;
-            o[4 + c] = (2.0 * iiii[c] - iijj[c] + 6.0 * ijij[c]) / 30.0;
-        }
-    }
+    if (act != 1 && act != 2) return;
+    corr_finish<kSpNT>(acc, red, out + 6 * size_t(target[s]));
 }
 
 // cuFFT C2R contract (calcIR.cu:790-791): y[k] = Re X0 + 2 sum_j Re(X_j
@@ -855,6 +923,20 @@ int slot_fixup(int prec, int n, int nslot, int ldy, const int* action, const voi
         hipLaunchKernelGGL(slot_fixup_kernel<float>, dim3(unsigned(ceil_div(tot, kSpNT))), dim3(kSpNT), 0, s, n,
                            nslot, ldy, action, static_cast<const float*>(sites), static_cast<const float*>(Yold),
                            static_cast<float*>(Ynew));
+    CIR_CHECK_LAUNCH();
+    return CIR_OK;
+}
+
+int slot_correlate(int prec, int n, int nslot, int ldy, const int* action, const int* target, const void* sites,
+                   const void* Yold, void* Ynew, double* out, hipStream_t s) {
+    if (prec == CIR_PREC_FP64)
+        hipLaunchKernelGGL(slot_correlate_kernel<double>, dim3(nslot), dim3(kSpNT), 0, s, n, nslot, ldy, action,
+                           target, static_cast<const double*>(sites), static_cast<const double*>(Yold),
+                           static_cast<double*>(Ynew), out);
+    else
+        hipLaunchKernelGGL(slot_correlate_kernel<float>, dim3(nslot), dim3(kSpNT), 0, s, n, nslot, ldy, action,
+                           target, static_cast<const float*>(sites), static_cast<const float*>(Yold),
+                           static_cast<float*>(Ynew), out);
     CIR_CHECK_LAUNCH();
     return CIR_OK;
 }

@@ -499,12 +499,16 @@ struct BatchRun {
             }
             {
                 ProfScope ps(pf, CIR_PH_CORRELATE, st);
-                CIR_TRY(slot_fixup(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot, su, Ycur, Ynext, st));
-                if (R > 0 && rany[u])
+                const int* au = act.as<int>() + size_t(u) * nslot;
+                const int* tu = tgt.as<int>() + size_t(u) * nslot;
+                if (R > 0 && rany[u]) {
+                    CIR_TRY(slot_fixup(rp, N, nslot, ldy, au, su, Ycur, Ynext, st));
                     CIR_TRY(reunitarise(rp, N, nslot, ldy, rmode.as<int>() + size_t(u) * nslot, rg0.as<double>(),
                                         Ynext, rwork.as<double>(), st));
-                CIR_TRY(correlate(rp, N, nslot, ldy, act.as<int>() + size_t(u) * nslot,
-                                  tgt.as<int>() + size_t(u) * nslot, su, Ynext, out.as<double>(), st));
+                    CIR_TRY(correlate(rp, N, nslot, ldy, au, tu, su, Ynext, out.as<double>(), st));
+                } else {
+                    CIR_TRY(slot_correlate(rp, N, nslot, ldy, au, tu, su, Ycur, Ynext, out.as<double>(), st));
+                }
             }
             std::swap(Ycur, Ynext);
             return CIR_OK;

@@ -54,6 +54,10 @@ bool propagate_needs_tuning(int prec, int gemm_prec, int n, int ncols);
 int set_identity(int prec, int n, int ldf, void* F, hipStream_t s);
 int slot_fixup(int prec, int n, int nslot, int ldy, const int* action, const void* sites, const void* Yold,
                void* Ynew, hipStream_t s);
+// slot_fixup then correlate in one launch (one block per slot), for frames
+// with no re-unitarisation between the two.
+int slot_correlate(int prec, int n, int nslot, int ldy, const int* action, const int* target, const void* sites,
+                   const void* Yold, void* Ynew, double* out, hipStream_t s);
 int correlate(int prec, int n, int nslot, int ldy, const int* action, const int* target, const void* sites,
               const void* Y, double* out, hipStream_t s);
 int spectrum_c2r(int nspec, int nfft, int nx, const double* X, double* y, hipStream_t s);
