@@ -914,7 +914,7 @@ __device__ __forceinline__ void q2_block_gk(int n, int64_t q, int* g, int* k) {
 // contiguous, V_op[l][s] = V(4 s + l/16, l%16) and NVT_op[l][4 rt + s] =
 // (-V T)(16 rt + l%16, 4 s + l/16).  V_op replaces V in place.
 __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float* __restrict__ Q2all, int bf16) {
-    __shared__ float sv[kQ2R][kQ2G + 1], st[kQ2G][kQ2G + 1], snv[kQ2R][kQ2G + 1], sg[kQ2G], stau[kQ2G];
+    __shared__ float sv[kQ2R][kQ2G + 1], st[kQ2G][kQ2G + 1], snv[kQ2R][kQ2G + 1], sgm[kQ2G][kQ2G + 1], stau[kQ2G];
     const int64_t q = blockIdx.x;
     const int b = blockIdx.y, lane = threadIdx.x;
     if (q >= nblk) return;
@@ -931,16 +931,21 @@ __global__ void __launch_bounds__(64) q2_prep_kernel(int n, int64_t nblk, float*
     }
     if (lane < kQ2G) stau[lane] = (k < sb_tasks(n, g * kQ2G + lane)) ? tau[lane] : 0.f;
     __syncthreads();
+    // the Gram matrix V^T V above its diagonal, all entries at once (4 per
+    // lane, each a 48-term dot product), then T column by column from it
+    for (int e = lane; e < kQ2G * kQ2G; e += 64) {
+        const int i = e / kQ2G, j = e % kQ2G;
+        if (i < j) {
+            float acc = 0.f;
+            for (int r = 0; r < kQ2R; ++r) acc += sv[r][i] * sv[r][j];
+            sgm[i][j] = acc;
+        }
+    }
+    __syncthreads();
     for (int j = 0; j < kQ2G; ++j) {
         if (lane < j) {
             float acc = 0.f;
-            for (int r = 0; r < kQ2R; ++r) acc += sv[r][lane] * sv[r][j];
-            sg[lane] = acc;
-        }
-        __syncthreads();
-        if (lane < j) {
-            float acc = 0.f;
-            for (int q2 = lane; q2 < j; ++q2) acc += st[lane][q2] * sg[q2];
+            for (int q2 = lane; q2 < j; ++q2) acc += st[lane][q2] * sgm[q2][j];
             st[lane][j] = -stau[j] * acc;
         } else if (lane == j) {
             st[j][j] = stau[j];
