@@ -6,7 +6,10 @@
 // compile-time constants; the next k tile is prefetched into registers while
 // the current one is multiplied from LDS.  BF16x3 splits each FP32 element
 // once, when it is staged, into a bf16 high part and a bf16 residual and
-// issues three MFMAs per fragment pair, accumulating in FP32.
+// issues three MFMAs per fragment pair, accumulating in FP32.  Where strides
+// and sizes are multiples of 4, BF16x3 products take gemm_x3v_kernel:
+// 16-byte operand reads, two K tiles of reads in flight, double-buffered
+// LDS, and a matrix's tiles on one XCD.
 
 #include <algorithm>
 
