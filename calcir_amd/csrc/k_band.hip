@@ -27,6 +27,7 @@
 //         rows r0..r0+l-1 zero, row r0+l one, rows below the reflector, the
 //         layout the compact-WY back-transform reads (offset NB)
 
+#include <algorithm>
 #include <cfloat>
 
 #include "band.hpp"
@@ -1223,10 +1224,17 @@ static int q2_launch(int batch, int n, int ld, const float* Q2, float* Z, int64_
     const size_t sm = size_t(q2_pitch(n)) * kQ2S * sizeof(float);
     CIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&q2_apply_kernel<NT, BF>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(sm)));
-    const int64_t total = int64_t(ceil_div(n, kQ2S)) * batch;
-    dim3 grid(unsigned(ceil_div(total, int64_t(kXcds)) * kXcds));
-    hipLaunchKernelGGL((q2_apply_kernel<NT, BF>), grid, dim3(NT), sm, s, n, ld, Q2, Z, nblk, batch);
-    CIR_CHECK_LAUNCH();
+    // the one-dimensional grid holds at most 2^32 - 1 threads: very large
+    // batches go in several launches
+    const int64_t nslab = ceil_div(n, kQ2S);
+    const int64_t most = std::max<int64_t>(1, ((int64_t(1) << 32) - 1) / (int64_t(NT) * nslab) - kXcds);
+    for (int64_t b0 = 0; b0 < batch; b0 += most) {
+        const int nb = int(std::min<int64_t>(most, batch - b0));
+        dim3 grid(unsigned(ceil_div(nslab * nb, int64_t(kXcds)) * kXcds));
+        hipLaunchKernelGGL((q2_apply_kernel<NT, BF>), grid, dim3(NT), sm, s, n, ld, Q2 + size_t(b0) * nblk * kQ2BlockFloats,
+                           Z + size_t(b0) * n * ld, nblk, nb);
+        CIR_CHECK_LAUNCH();
+    }
     return CIR_OK;
 }
 
