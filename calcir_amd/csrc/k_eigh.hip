@@ -1535,9 +1535,11 @@ invit_kernel(int n, int ld, const T* __restrict__ dall, const T* __restrict__ ea
 // T factor of a compact-WY block of kb <= kBtNB reflectors from its Gram
 // matrix G = Y^T Y: T(l,l) = tau_l, T(0:l, l) = -tau_l T(0:l, 0:l) G(0:l, l).
 // G and T are kBtNB x kBtNB blocks with row pitch ldt inside per-matrix
-// arrays of bstride elements.
+// arrays of bstride elements.  Column l's triangular product takes four
+// lanes per row (each a quarter of the terms, summed by two shuffles), so
+// the longest chain per column is l / 4 products instead of l.
 template <typename T>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(4 * kBtNB)
 larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __restrict__ Gall,
              T* __restrict__ Tall, int ldt, int bstride) {
     // T is built in LDS, transposed with a padded pitch (Tt[c*kLtP + r] =
@@ -1548,24 +1550,26 @@ larft_kernel(int n, int kb, int j0, const T* __restrict__ tauall, const T* __res
     T* const Tt = reinterpret_cast<T*>(lsm);
     T* const col = Tt + kBtNB * kLtP;
     const int b = blockIdx.x, tid = threadIdx.x;
+    const int row = tid >> 2, part = tid & 3;
     const T* tau = tauall + size_t(b) * n + j0;
     const T* G = Gall + size_t(b) * bstride;
     T* Tm = Tall + size_t(b) * bstride;
-    for (int q = tid; q < kBtNB * kLtP; q += 128) Tt[q] = T(0);
+    for (int q = tid; q < kBtNB * kLtP; q += 4 * kBtNB) Tt[q] = T(0);
     __syncthreads();
     for (int l = 0; l < kb; ++l) {
         const T tl = tau[l];
         if (tid < l) col[tid] = -tl * G[tid * ldt + l];
         __syncthreads();
-        if (tid < l) {
-            T acc = T(0);
-            for (int q = tid; q < l; ++q) acc += Tt[q * kLtP + tid] * col[q];
-            Tt[l * kLtP + tid] = acc;
-        }
+        T acc = T(0);
+        if (row < l)
+            for (int q = row + part; q < l; q += 4) acc += Tt[q * kLtP + row] * col[q];
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        if (row < l && part == 0) Tt[l * kLtP + row] = acc;
         if (tid == 0) Tt[l * kLtP + l] = tl;
         __syncthreads();
     }
-    for (int q = tid; q < kBtNB * kBtNB; q += 128) {
+    for (int q = tid; q < kBtNB * kBtNB; q += 4 * kBtNB) {
         const int r = q / kBtNB, c = q % kBtNB;
         Tm[r * ldt + c] = Tt[c * kLtP + r];
     }
@@ -1805,11 +1809,11 @@ int eigh_impl(int gemm_prec, int batch, int n, int ld, T* A, T* V, double* w, un
         CIR_TRY(mm(kb, kb, mrows, Y, ld, 1, mat, Y, 1, ld, mat, G, P, 1, pp, 1.0, 0.0));
         // T1, T2 from the diagonal blocks of G; T12 = -T1 (G12 T2), with
         // G12 T2 staged in W2
-        hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(128), larft_smem_bytes(sizeof(T)), s, n, kb1, j0, tau,
+        hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(4 * kBtNB), larft_smem_bytes(sizeof(T)), s, n, kb1, j0, tau,
                            G, Tm, P, int(pp));
         CIR_CHECK_LAUNCH();
         if (kb2 > 0) {
-            hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(128), larft_smem_bytes(sizeof(T)), s, n, kb2,
+            hipLaunchKernelGGL(larft_kernel<T>, dim3(batch), dim3(4 * kBtNB), larft_smem_bytes(sizeof(T)), s, n, kb2,
                                j0 + kb1, tau, G + int64_t(kb1) * P + kb1, Tm + int64_t(kb1) * P + kb1, P, int(pp));
             CIR_CHECK_LAUNCH();
             CIR_TRY(mm(kb1, kb2, kb2, G + kb1, P, 1, pp, Tm + int64_t(kb1) * P + kb1, P, 1, pp, W2, P, 1, pn, 1.0,
