@@ -10,27 +10,52 @@
 
 namespace cir {
 
+// Stream-ordered allocation for one host thread.  hipFree waits for every
+// stream of the device, so concurrent jobs that free their buffers with it
+// stall one another until all queued work has drained.  While a thread has
+// `tl_pool` set, DevBuf takes its buffers from that memory pool on that
+// stream and frees them in the stream's order (cir_run_many's jobs).
+struct PoolStream {
+    hipMemPool_t pool;
+    hipStream_t stream;
+};
+inline thread_local const PoolStream* tl_pool = nullptr;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    hipStream_t fs = nullptr;  // stream a pooled buffer is freed on
+    bool pooled = false;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (pooled)
+                (void)hipFreeAsync(p, fs);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        fs = nullptr;
+        pooled = false;
     }
     int alloc(size_t n) {
         release();
         if (n == 0) n = 256;
-        hipError_t e = hipMalloc(&p, n);
+        const PoolStream* ps = tl_pool;
+        const hipError_t e = ps ? hipMallocFromPoolAsync(&p, n, ps->pool, ps->stream) : hipMalloc(&p, n);
         if (e != hipSuccess) {
             p = nullptr;
-            return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+            return hip_fail(e, ps ? "hipMallocFromPoolAsync" : "hipMalloc", __FILE__, __LINE__);
         }
         bytes = n;
+        if (ps) {
+            pooled = true;
+            fs = ps->stream;
+        }
         return CIR_OK;
     }
     template <typename T>

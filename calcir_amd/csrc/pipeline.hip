@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include <cstdint>
 #include <cstdlib>
@@ -1050,10 +1051,43 @@ extern "C" int cir_run(cir_ctx* c, int nframes, const float* xyz, const float* b
     return cir::host_exception("cir_run");
 }
 
+// The memory pool of cir_run_many's jobs on the current device, created on
+// first use and kept until the process ends.  It holds freed memory for the
+// next job (release threshold: all) until the call trims it.  Null when the
+// device has no stream-ordered allocator: the jobs then use hipMalloc.
+static hipMemPool_t job_pool() {
+    static std::mutex m;
+    static std::vector<std::pair<int, hipMemPool_t>> pools;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(m);
+    for (const auto& q : pools)
+        if (q.first == dev) return q.second;
+    hipMemPoolProps pr{};
+    pr.allocType = hipMemAllocationTypePinned;
+    pr.handleTypes = hipMemHandleTypeNone;
+    pr.location.type = hipMemLocationTypeDevice;
+    pr.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    uint64_t keep = UINT64_MAX;
+    if (hipMemPoolCreate(&pool, &pr) != hipSuccess ||
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+        (void)hipGetLastError();
+        if (pool) (void)hipMemPoolDestroy(pool);
+        pool = nullptr;
+    }
+    pools.emplace_back(dev, pool);
+    return pool;
+}
+
 // Independent jobs (boxes of any size) at once on one GPU: each job runs on
 // its own stream, issued by a pool of host threads; the largest jobs start
 // first.  Kernels of different jobs overlap on the device, which fills the
-// GPU when single small boxes cannot.
+// GPU when single small boxes cannot.  The jobs' buffers come from a memory
+// pool in their streams' order (hipFree would wait for the other jobs' work).
 extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, const float* const* xyz,
                             const float* const* box, const float* const* times, int nthreads) try {
     if (njob < 0 || (njob > 0 && (!ctxs || !nframes || !xyz || !box || !times)))
@@ -1092,20 +1126,31 @@ extern "C" int cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, 
     std::vector<int> status(njob, CIR_OK);
     std::vector<std::string> detail(njob);
     std::atomic<int> next{0};
+    hipMemPool_t pool = job_pool();
     auto worker = [&]() {
         for (int q = next++; q < njob; q = next++) {
             const int j = order[q];
+            const PoolStream ps{pool, ctxs[j]->stream};
+            tl_pool = pool ? &ps : nullptr;
             status[j] = cir_run(ctxs[j], nframes[j], xyz[j], box[j], times[j]);
+            tl_pool = nullptr;
             if (status[j] != CIR_OK) detail[j] = cir_last_error();
         }
     };
-    std::vector<std::thread> pool;
+    std::vector<std::thread> threads;
     try {
-        for (int t = 0; t < nt; ++t) pool.emplace_back(worker);
+        for (int t = 0; t < nt; ++t) threads.emplace_back(worker);
     } catch (...) {  // fewer threads than asked: the ones started take all jobs
     }
-    if (pool.empty()) worker();
-    for (auto& t : pool) t.join();
+    if (threads.empty()) worker();
+    for (auto& t : threads) t.join();
+    if (pool) {
+        // the jobs' buffers are freed in their streams' order: once the
+        // streams have passed the frees, the pool gives the memory back
+        for (int j = 0; j < njob; ++j) (void)hipStreamSynchronize(ctxs[j]->stream);
+        (void)hipMemPoolTrimTo(pool, 0);
+        (void)hipGetLastError();
+    }
     for (int j = 0; j < njob; ++j) {
         ctxs[j]->stream = saved[j];
         ctxs[j]->mem_share = saved_share[j];

@@ -295,7 +295,10 @@ int   cir_stream_end(cir_ctx* ctx);
  * job j is cir_run(ctxs[j], nframes[j], xyz[j], box[j], times[j]); each job
  * runs on its own stream (a context bound to the null stream gets one for the
  * call), issued by nthreads host threads (0 = 8), largest jobs first.  The
- * contexts must be distinct.  Returns the first failing job's status. */
+ * jobs' device buffers come from a stream-ordered memory pool of the library
+ * (hipFree would wait for every other job's queued work); the pool gives the
+ * memory back before the call returns.  The contexts must be distinct.
+ * Returns the first failing job's status. */
 int   cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, const float* const* xyz,
                    const float* const* box, const float* const* times, int nthreads);
 int   cir_get_results(cir_ctx* ctx, cir_results* r);

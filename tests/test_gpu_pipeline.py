@@ -355,6 +355,44 @@ def test_run_many_uneven_boxes_equal_single_runs():
         assert g["nsteps"] == ref["nsteps"] == 16
 
 
+def test_run_many_gives_back_its_pooled_memory_and_names_a_failed_job():
+    # cir_run_many's jobs take their buffers from a stream-ordered memory pool
+    # (the 1024-water job alone holds several hundred MB at once): after the
+    # call the device memory is free again, also when a job fails, and the
+    # other jobs' results equal their single runs
+    import torch
+
+    sizes = [(1024, "bf16x3", "batched"), (256, "fp32", "plain"), (128, "fp64", "batched")]
+    jobs = []
+    for j, (nw, prec, mode) in enumerate(sizes):
+        xyz, box, times = frames(nw, 16, 4100 + j)
+        p = C.default_params(precision=prec, mode=mode, **dict(KW, nsamples=2, ntcfpoints=6, sampleEvery=0.05))
+        jobs.append((p, xyz, box, times))
+    announce("run_many pooled: warm-up")
+    C.run_many(jobs, nthreads=3)  # first calls time launch shapes and load code
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    announce("run_many pooled: memory")
+    outs = C.run_many(jobs, nthreads=3)
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    assert free0 - free1 < (64 << 20), (free0, free1)
+    for (p, xyz, box, times), g, (nw, prec, mode) in zip(jobs, outs, sizes):
+        announce("run_many pooled: single check nwater=%d %s %s" % (nw, prec, mode))
+        ref = C.run(p, xyz, box, times)
+        tol = 1e-12 if prec == "fp64" else 5e-5
+        for k in ("tcf_ir", "tcf_vv", "tcf_vh"):
+            assert rel(g[k], ref[k]) < tol, (nw, prec, k, rel(g[k], ref[k]))
+    bad = jobs[2][1].copy()
+    bad[3, 5, 1] = np.nan
+    announce("run_many pooled: a failing job")
+    with pytest.raises(C.CalcirError, match="job 1"):
+        C.run_many([jobs[0], (jobs[2][0], bad, jobs[2][2], jobs[2][3])], nthreads=2)
+    torch.cuda.synchronize()
+    free2 = torch.cuda.mem_get_info()[0]
+    assert free0 - free2 < (64 << 20), (free0, free2)
+
+
 def test_reunitarisation_in_the_pipeline():
     # FP64: F is unitary to rounding, so restoring the Gram matrices changes
     # nothing beyond 1e-12; BF16x3 with it stays within 1e-3 of FP64 and close
