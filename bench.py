@@ -145,9 +145,9 @@ CPU_POINTS = 8
 
 
 def host_info():
-    """Threads the CPU legs use and what the host is: the GPU box allots a
-    share of its cores to each GPU and says so in OMP_NUM_THREADS, which the
-    CPU legs follow (os.cpu_count() where it is unset)."""
+    """Threads the CPU legs use and what the host is: a shared GPU host may
+    allot each GPU a share of its cores and say so in OMP_NUM_THREADS, which
+    the CPU legs follow (the process's CPU affinity where it is unset)."""
     nproc = os.cpu_count() or 1
     try:
         affinity = len(os.sched_getaffinity(0))
