@@ -5,10 +5,12 @@ frame subsets, appends and corrupt input.
 
 No XTC file written by GROMACS is available in this environment, so the
 decoder is pinned by (a) a byte-exact hand-assembled uncompressed frame, (b)
-a compressed 10-atom frame assembled bit by bit from the published
-xdr3dfcoord layout (mixed-radix triples, run flags, a run of small
-differences with the first-pair exchange), independent of this project's
-writer, and (c) round trips through the writer.  Agreement with files that
+compressed frames assembled bit by bit from the published xdr3dfcoord layout
+(mixed-radix triples, run flags, runs of small differences with the
+first-pair exchange, run lengths kept under a flag bit 0, the small-triple
+precision raised and lowered between runs, the per-axis large-range path),
+independent of this project's writer, and (c) round trips through the
+writer.  Agreement with files that
 GROMACS itself wrote stays parity unpinned (none exist here)."""
 
 import os
@@ -367,3 +369,117 @@ def test_hand_assembled_compressed_frame(tmp_path):
     assert np.array_equal(xyz[1], want)
     assert list(steps) == [7, 7] and np.array_equal(box[0], np.float32([3.0, 3.5, 4.0]))
     assert C.xtc_precision(f, 0) == prec
+
+
+# magicints of the published xdr3dfcoord code from index 9 (FIRSTIDX) on:
+# triples of values below magicints[i] pack into i bits
+_MAGIC = {9: 8, 10: 10, 11: 12, 12: 16, 13: 20}
+
+
+def _compressed_frame(natoms, prec, minint, maxint, smallidx, payload):
+    frame = struct.pack(">iiif", 1995, natoms, 3, 0.25) + struct.pack(">9f", 5.0, 0, 0, 0, 5.0, 0, 0, 0, 5.0)
+    frame += struct.pack(">i", natoms) + struct.pack(">f", prec)
+    frame += struct.pack(">3i", *[int(x) for x in minint]) + struct.pack(">3i", *[int(x) for x in maxint])
+    frame += struct.pack(">i", smallidx) + struct.pack(">i", len(payload))
+    return frame + payload + b"\0" * (-len(payload) % 4)
+
+
+class _Coder:
+    """Plain atoms and runs of small atoms in the xdr3dfcoord bit layout,
+    with the decoder's expected output: a flag bit 0 keeps the previous run
+    length (3 x small atoms; 0 at the start), a flag bit 1 is followed by 5
+    bits of run + is_smaller + 1 (is_smaller in {-1, 0, 1}); the small atoms
+    of a run use the precision index in force before the flag's change; the
+    first small atom is emitted before its plain atom; after the flag the
+    index moves by is_smaller and the offset smallnum becomes
+    magicints[index] / 2 (up) or the previous 'smaller' (down)."""
+
+    def __init__(self, minint, maxint, smallidx, large):
+        self.bits = _Bits()
+        self.minint = np.array(minint, np.int64)
+        self.sizes = [int(x) for x in np.array(maxint, np.int64) - self.minint + 1]
+        self.large = large
+        self.bitsize = _bits_of_product(self.sizes)
+        self.axbits = [s.bit_length() for s in self.sizes]
+        self.idx = smallidx
+        self.smallnum = _MAGIC[smallidx] // 2
+        self.smaller = _MAGIC[max(9, smallidx - 1)] // 2
+        self.run = 0
+        self.out = []
+
+    def atom(self, c, small=(), is_smaller=0):
+        c = np.array(c, np.int64)
+        off = [int(x) for x in c - self.minint]
+        if self.large:
+            for k in range(3):
+                self.bits.put(self.axbits[k], off[k])
+        else:
+            _pack_ints(self.bits, self.bitsize, self.sizes, off)
+        run = 3 * len(small)
+        if run == self.run and is_smaller == 0:
+            self.bits.put(1, 0)
+        else:
+            self.bits.put(1, 1)
+            self.bits.put(5, run + is_smaller + 1)
+        self.run = run
+        prev = c
+        emitted = []
+        for k, t in enumerate(small):
+            assert all(0 <= v < _MAGIC[self.idx] for v in t)
+            _pack_ints(self.bits, self.idx, [_MAGIC[self.idx]] * 3, list(t))
+            s = np.array(t, np.int64) + prev - self.smallnum
+            emitted += [s, c] if k == 0 else [s]
+            prev = s
+        self.out += emitted if small else [c]
+        self.idx += is_smaller
+        if is_smaller < 0:
+            self.smallnum = self.smaller
+            self.smaller = _MAGIC[self.idx - 1] // 2 if self.idx > 9 else 0
+        elif is_smaller > 0:
+            self.smaller = self.smallnum
+            self.smallnum = _MAGIC[self.idx] // 2
+
+
+def _check_decoded(tmp_path, name, natoms, prec, minint, maxint, smallidx, coder):
+    f = tmp_path / name
+    f.write_bytes(_compressed_frame(natoms, prec, minint, maxint, smallidx, coder.bits.bytes()))
+    assert C.xtc_scan(f) == (natoms, 1)
+    xyz = C.xtc_read(f)[0]
+    assert len(coder.out) == natoms
+    inv = np.float32(1.0 / prec)
+    want = (np.array(coder.out, np.int64).astype(np.float32) * inv).astype(np.float32)
+    assert np.array_equal(xyz[0], want), (xyz[0] / inv, np.array(coder.out))
+
+
+def test_hand_assembled_frame_with_changing_small_precision(tmp_path):
+    # runs at precision index 10, then 11 (the first flag raises it), then
+    # 10 again (the second lowers it), each run decoded at the index in force
+    # before its flag; the third atom's run repeats the second's length under
+    # a flag bit 0, the fourth's flag ends the runs; 10 atoms at precision
+    # 1000
+    minint, maxint = (-2000, -1500, -1000), (3000, 3500, 4000)
+    cd = _Coder(minint, maxint, 10, large=False)
+    cd.atom((100, 200, 300), small=[(1, 9, 4), (7, 0, 8)], is_smaller=1)
+    cd.atom((-1900, 3400, 3900), small=[(11, 0, 6)], is_smaller=-1)
+    cd.atom((2999, -1499, 17), small=[(0, 9, 9)])
+    cd.atom((0, 0, 0))
+    cd.atom((-2000, -1500, -1000))
+    cd.atom((3000, 3500, 4000))
+    _check_decoded(tmp_path, "adapt.xtc", 10, 1000.0, minint, maxint, 10, cd)
+
+
+def test_hand_assembled_frame_on_the_large_range_path(tmp_path):
+    # an axis spanning more than 0xffffff integers codes every plain atom
+    # axis by axis (bit length of each axis' size) instead of one mixed-radix
+    # triple; runs and their small triples are as on the normal path.  11
+    # atoms at precision 1e6, integers within 2^24 so that they are exact
+    # in FP32
+    minint, maxint = (-9000000, -100, 5), (9000000, 250000, 70000)
+    cd = _Coder(minint, maxint, 12, large=True)
+    cd.atom((1234567, 99, 60000), small=[(15, 0, 8)])
+    for c in [(-9000000, -100, 5), (9000000, 250000, 70000), (0, 0, 6), (-42, 1717, 999),
+              (8388607, -99, 12345), (-8388607, 123456, 54321)]:
+        cd.atom(c)
+    cd.atom((5, 5, 5), is_smaller=-1)   # a flag with no run that lowers the index to 11
+    cd.atom((700, 800, 900), small=[(11, 3, 0)], is_smaller=-1)
+    _check_decoded(tmp_path, "large.xtc", 11, 1e6, minint, maxint, 12, cd)
