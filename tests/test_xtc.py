@@ -15,6 +15,8 @@ GROMACS itself wrote stays parity unpinned (none exist here)."""
 
 import os
 import struct
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -483,3 +485,56 @@ def test_hand_assembled_frame_on_the_large_range_path(tmp_path):
     cd.atom((5, 5, 5), is_smaller=-1)   # a flag with no run that lowers the index to 11
     cd.atom((700, 800, 900), small=[(11, 3, 0)], is_smaller=-1)
     _check_decoded(tmp_path, "large.xtc", 11, 1e6, minint, maxint, 12, cd)
+
+
+_FUZZ = r"""
+import os, sys
+import numpy as np
+import calcir_amd as C
+d, natoms, iters, seed = sys.argv[1], int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4])
+rng = np.random.default_rng(seed)
+xyz, box, times = C.synth_trajectory((natoms + 3) // 4, 3, seed)
+xyz = xyz[:, :natoms]
+good_p, bad_p = os.path.join(d, "good.xtc"), os.path.join(d, "bad.xtc")
+C.xtc_write(good_p, xyz, box, times)
+good = np.frombuffer(open(good_p, "rb").read(), np.uint8)
+refused = read = 0
+for it in range(iters):
+    b = good.copy()
+    kind = it % 4
+    if kind == 0:
+        for _ in range(rng.integers(1, 4)):
+            b[rng.integers(0, b.size)] ^= np.uint8(1 << rng.integers(0, 8))
+    elif kind == 1:
+        b = b[: rng.integers(0, b.size)]
+    elif kind == 2:
+        p = rng.integers(0, b.size - 4)
+        b[p:p + 4] = rng.integers(0, 256, 4)
+    else:
+        p = rng.integers(60, b.size)
+        b[p:] = rng.integers(0, 256, b.size - p)
+    open(bad_p, "wb").write(b.tobytes())
+    try:
+        na, nf = C.xtc_scan(bad_p)
+        if na == natoms and 1 <= nf <= 3:
+            C.xtc_read(bad_p)
+            C.xtc_precision(bad_p, 0)
+            read += 1
+    except C.CalcirError:
+        refused += 1
+print("done", iters, refused, read, flush=True)
+"""
+
+
+@pytest.mark.parametrize("natoms", [9, 10, 160])
+def test_mutated_files_are_read_or_refused(tmp_path, natoms):
+    # bit flips, truncations, overwritten words and random tails of a valid
+    # file: every call returns or raises CalcirError, none reads out of
+    # bounds or crashes (run in a child process, so a crash fails this test
+    # instead of ending the session)
+    r = subprocess.run([sys.executable, "-c", _FUZZ, str(tmp_path), str(natoms), "600", str(natoms)],
+                       capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, PYTHONPATH=os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+    assert r.returncode == 0, r.stderr[-2000:]
+    done, iters, refused, read = r.stdout.split()
+    assert (done, iters) == ("done", "600") and int(refused) > 50 and int(read) > 50, r.stdout
