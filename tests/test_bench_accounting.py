@@ -116,3 +116,14 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_a_failing_extra_is_recorded_in_its_place(capsys):
+    assert bench.guarded("ok", lambda: {"value": 1}) == {"value": 1}
+
+    def boom():
+        raise C.CalcirError(2, "out of memory")
+
+    r = bench.guarded("8192-OH box", boom)
+    assert set(r) == {"error"} and "out of memory" in r["error"]
+    assert "8192-OH box failed" in capsys.readouterr().err
