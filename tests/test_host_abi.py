@@ -183,6 +183,32 @@ def test_writers_produce_the_thirteen_files(tmp_path):
     assert row[1] == pytest.approx(np.float32(res["ftcf_ir"][0]) / (factor * n), rel=1e-5)
 
 
+def test_lineshape_rows_keep_the_reference_loop_bounds(tmp_path):
+    # calcIR.cu:899-919: the first loop runs i = n/2 .. n-1, the second
+    # i = 0 .. ntcfpoints + nzeros/2 - 1 (operator precedence: not n/2), so
+    # with a window holding every frequency the file has n - n/2 +
+    # ntcfpoints + nzeros/2 rows, the second loop's last ones at frequencies
+    # below avef - (n/2) factor taken from bins the first loop also wrote
+    nt, nz = 10, 21
+    p = C.default_params(outf=str(tmp_path / "w"), ntcfpoints=nt, nzeros=nz, avef=3400.0, omegaStart=-1e7,
+                         omegaStop=1e7, dt=0.01)
+    n = nt + nz
+    res = _fake_results(p)
+    res["ftcf_ir"] = np.arange(n, dtype=np.float64) + 1.0  # bin i holds i + 1
+    C.write_outputs(p, res)
+    ls = np.loadtxt(tmp_path / "w_irls.dat")
+    assert ls.shape[0] == (n - n // 2) + nt + nz // 2 == 36
+    factor = np.float32(2 * 3.14159265359 * 5.308837367 / (np.float32(0.01) * np.float32(n)))
+    scale = factor * np.float32(n)
+    bins = np.rint(ls[:, 1] * scale).astype(int) - 1
+    assert list(bins) == list(range(n // 2, n)) + list(range(0, nt + nz // 2))
+    # frequencies: (n - i) factor + avef in the first loop, avef - i factor
+    # in the second, so the second loop's bins past n/2 lie below avef - (n/2) factor
+    assert np.all(ls[n - n // 2:, 0] <= 3400.0 + 1e-3)
+    assert ls[-1, 0] == pytest.approx(3400.0 - (nt + nz // 2 - 1) * factor, rel=1e-5)  # "%g"
+    assert ls[-1, 0] < 3400.0 - (n // 2) * factor
+
+
 def test_synthetic_trajectory_is_seeded_and_rigid():
     a = C.synth_trajectory(64, 6, 42)
     b = C.synth_trajectory(64, 6, 42)
