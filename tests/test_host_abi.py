@@ -191,7 +191,7 @@ def test_lineshape_rows_keep_the_reference_loop_bounds(tmp_path):
     # below avef - (n/2) factor taken from bins the first loop also wrote
     nt, nz = 10, 21
     p = C.default_params(outf=str(tmp_path / "w"), ntcfpoints=nt, nzeros=nz, avef=3400.0, omegaStart=-1e7,
-                         omegaStop=1e7, dt=0.01)
+                         omegaStop=1e7, omegaStep=1000000, dt=0.01)  # (a coarse grid for the distributions)
     n = nt + nz
     res = _fake_results(p)
     res["ftcf_ir"] = np.arange(n, dtype=np.float64) + 1.0  # bin i holds i + 1
