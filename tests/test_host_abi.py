@@ -298,3 +298,37 @@ def test_trajectory_shapes_are_checked_before_the_library_reads_them():
         C.check_trajectory(pt, xyz, box, times)
     with pytest.raises(ValueError):
         C.run_many([(p, xyz, box, times[:1])])
+
+
+@pytest.mark.parametrize("over,natoms,status", [
+    (dict(map=2), 16, 5),                    # calcIR.cu:1207-1242 leaves the map arrays unset: refused
+    (dict(map=-1), 16, 5),
+    (dict(model="spce"), 16, 3),             # calcIR.cu:327-330
+    (dict(species="H2S"), 16, 4),            # calcIR.cu:336-339
+    (dict(), 15, 1),                         # natoms not a multiple of natom_mol
+    (dict(natom_mol=2), 16, 1),
+    (dict(ntcfpoints=0), 16, 1),
+    (dict(omegaStart=4000.0, omegaStop=3000.0), 16, 1),
+    (dict(reunitarise_every=-1), 16, 1),
+    (dict(triclinic=2), 16, 1),
+])
+def test_create_refuses_bad_parameters_before_looking_for_a_device(over, natoms, status):
+    kw = dict(species="H2O")
+    kw.update(over)
+    p = C.default_params(**kw)
+    L = C.lib()
+    c = ctypes.c_void_p()
+    assert L.cir_create(ctypes.byref(p), natoms, None, ctypes.byref(c)) == status
+    assert not c.value
+    assert L.cir_last_error().decode()
+
+
+def test_create_with_good_parameters_needs_a_device():
+    p = C.default_params(species="H2O")
+    L = C.lib()
+    c = ctypes.c_void_p()
+    s = L.cir_create(ctypes.byref(p), 16, None, ctypes.byref(c))
+    if s == 0:  # a machine with a GPU
+        L.cir_destroy(c)
+    else:
+        assert s == 11 and not c.value  # CIR_ERR_NODEVICE
