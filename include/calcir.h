@@ -297,7 +297,9 @@ int   cir_stream_end(cir_ctx* ctx);
  * call), issued by nthreads host threads (0 = 8), largest jobs first.  The
  * jobs' device buffers come from a stream-ordered memory pool of the library
  * (hipFree would wait for every other job's queued work); the pool gives the
- * memory back before the call returns.  The contexts must be distinct.
+ * memory back before the call returns, after waiting for the jobs' streams
+ * (a caller's stream bound to a context included).  The contexts must be
+ * distinct.
  * Returns the first failing job's status. */
 int   cir_run_many(int njob, cir_ctx* const* ctxs, const int* nframes, const float* const* xyz,
                    const float* const* box, const float* const* times, int nthreads);
