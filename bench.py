@@ -215,8 +215,12 @@ def cpu_baseline(xyz, box, times, nthreads):
     r = O.run(O.F32, cfg, xyz[:k], box[:k], times[:k])
     dt = time.perf_counter() - t0
     return {"value": r["nsteps"] / dt, "unit": "frames/s", "cores": nthreads, "kind": "port",
-            "sample": "2048-OH box, 1 origin x %d tcf points (1 F=I step + %d propagated steps), FP32 oracle, "
-                      "literal F-matrix propagation (3 complex products per step), %.1f s" % (k, k - 1, dt)}
+            "sample": "%d-OH box, 1 origin x %d tcf points (1 F=I step + %d propagated steps), FP32 oracle, "
+                      "literal F-matrix propagation (3 complex products per step), %.1f s"
+                      % (2 * (xyz.shape[1] // 4), k, k - 1, dt),
+            "phases_s": {"field_hamiltonian": r["t_hamiltonian"], "eigh": r["t_eigh"],
+                         "ipr_spectral_density": r["t_hist"], "propagate_correlate": r["t_propagate"],
+                         "finalize": r["t_finalize"]}}
 
 
 def adjoint_bench(C):

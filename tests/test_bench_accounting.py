@@ -127,3 +127,17 @@ def test_a_failing_extra_is_recorded_in_its_place(capsys):
     r = bench.guarded("8192-OH box", boom)
     assert set(r) == {"error"} and "out of memory" in r["error"]
     assert "8192-OH box failed" in capsys.readouterr().err
+
+
+def test_cpu_baseline_reports_its_phases():
+    # the CPU leg on a small box (32 waters, the bench's 8 tcf points): the
+    # per-phase seconds of the oracle run sum to no more than its wall time
+    xyz, box, times = C.synth_trajectory(32, 8, 77, box_nm=C.synth_box_nm(32))
+    r = bench.cpu_baseline(xyz, box, times, 2)
+    assert r["kind"] == "port" and r["cores"] == 2 and r["unit"] == "frames/s" and r["value"] > 0
+    ph = r["phases_s"]
+    assert set(ph) == {"field_hamiltonian", "eigh", "ipr_spectral_density", "propagate_correlate", "finalize"}
+    assert all(v >= 0 for v in ph.values())
+    wall = 8 / r["value"]
+    assert sum(ph.values()) <= wall * 1.05 + 1e-3
+    assert r["sample"].startswith("64-OH box")
